@@ -1,0 +1,115 @@
+/*
+ * lsjk.h -- C-ABI of liblsjk, the MI355X (gfx950) direct Coulomb/exchange build.
+ *
+ * This is the boundary the host C++ IntBox::coulomb / IntBox::exchange dispatch into, in the
+ * place of the reference's TeraChem seam (reference src/cpp/intbox/intbox.cpp:213-329 ->
+ * src/cpp/intbox/tc/{coulomb,exchange}.cpp -> TCIntBox -> IBox* C API,
+ * src/cpp/intbox/tc/tcintbox/tcintbox.cpp:20-415).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - All matrices are row-major FP64 in the Cartesian representation of their bases, Cartesian
+ *     components in lexical order (xx, xy, xz, yy, yz, zz) as in reference
+ *     src/lightspeed/am.hpp:48-68.  Pure <-> Cartesian transforms stay on the caller's side, as
+ *     in reference coulombCPU (src/cpp/intbox/cpu/coulomb.cpp:26-35, 186-193).
+ *   - Matrix arguments named *_dev are device pointers (HBM) on the library's device; the build
+ *     functions ACCUMULATE into their output (J += ..., K += ...), like IBoxJFockGen.
+ *   - Every function returns 0 on success and a nonzero LSJK_E* code on failure; the message is
+ *     available from lsjk_last_error() (per thread).  Nothing is printed.
+ *   - One process-wide session (like TCIntBox, tcintbox.cpp:18-27); calls are serialised
+ *     internally and run on the library's own HIP stream; each build call returns after its
+ *     kernels completed.
+ */
+#ifndef LSJK_H
+#define LSJK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSJK_OK 0
+#define LSJK_E_INVALID 1   /* bad argument (shape, angular momentum, null pointer, ...) */
+#define LSJK_E_HIP 2       /* HIP runtime error */
+#define LSJK_E_NOINIT 3    /* lsjk_init not called */
+#define LSJK_E_UNSUPPORTED 4
+
+typedef struct lsjk_basis lsjk_basis;
+typedef struct lsjk_pairlist lsjk_pairlist;
+
+/* Replaces IBoxStartup(ngpu, devIds, pinnedBytes, align) (reference tcintbox.cpp:44-50):
+ * select the device, create the stream, upload the Rys quadrature table. */
+int lsjk_init(int device);
+
+/* Replaces the TCIntBox teardown (~TCIntBox / IBoxShutdown, reference tcintbox.cpp:52-55, run
+ * from the interpreter exit hook src/python/exit_hooks.cpp:14-18): release every device
+ * resource of the session. */
+int lsjk_finalize(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char* lsjk_last_error(void);
+
+/* Number of visible GPUs (0 if the HIP runtime finds none); needs no session.  The
+ * counterpart of the device enumeration in ResourceList::build (reference
+ * src/cpp/env/resource_list.cpp:22-37). */
+int lsjk_device_count(int* count);
+
+/* Replaces IBoxInitShell(L, atomIdx, nprim, es, cs) per shell + IBoxUpdateCoors(threpair, xyz)
+ * (reference tcintbox.cpp:99-110): a primitive-level basis (reference Basis, basis.hpp:425-609).
+ * c, e: coefficients (normalisation included) and exponents; xyz: nprim x 3 (bohr); L: angular
+ * momentum (0..2); cart: Cartesian offset of each primitive's shell (cartIdx); ncart: total
+ * Cartesian functions.  Data are copied. */
+int lsjk_basis_create(int nprim, const double* c, const double* e, const double* xyz, const int* L,
+                      const int* cart, int ncart, lsjk_basis** out);
+int lsjk_basis_destroy(lsjk_basis* b);
+
+/* A Schwarz pair list (reference PairList, pair_list.hpp:25-220) over (b1, b2): npair primitive
+ * pairs (p1 index into b1, p2 into b2, float bound), grouped by (L1, L2) class in L1-major order
+ * and sorted by bound descending inside each class (PairList::build_schwarz,
+ * src/cpp/core/pair_list.cpp:152-173).  symmetric: b1 and b2 are the same basis and p2 <= p1.
+ * Device-side Hermite and K-invert views are built once here (the role of TCIntBox::set_basis,
+ * tcintbox.cpp:66-115, which caches the TeraChem shell-pair data). */
+int lsjk_pairlist_create(const lsjk_basis* b1, const lsjk_basis* b2, int symmetric, int npair, const int* p1,
+                         const int* p2, const float* bound, lsjk_pairlist** out);
+int lsjk_pairlist_destroy(lsjk_pairlist* pl);
+
+/* Replaces IBoxJFockGen(thresp, thredp, scalfr, scallr, omega, P, fock) per Ewald task
+ * (reference tcintbox.cpp:261-278, call at :270, from tc/coulomb.cpp:14-63), with the semantics of
+ * IntBox::coulomb (src/lightspeed/intbox.hpp:721-771):
+ *   J12_dev (ncart1 x ncart2) += sum_34 (12|O|34) D34_dev (ncart3 x ncart4),
+ *   O(r) = sum_i scales[i] erf(omegas[i] r)/r  (omegas[i] == -1: 1/r), nterm <= 8.
+ * thredp is accepted for the API; plain FP64 evaluation uses thresp (intbox.hpp:736-742). */
+int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
+                 const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev);
+
+/* Replaces IBoxKFockGen / IBoxKFockSym (reference tcintbox.cpp:299-334, called from
+ * tc/exchange.cpp:14-75), with the semantics of IntBox::exchange (intbox.hpp:773-828):
+ *   K13_dev (ncart1 x ncart3) += sum_24 (12|O|34) D24_dev (ncart2 x ncart4).
+ * D24symm: the caller asserts D24 symmetric and bases 1==3, 2==4 (the effective flag of
+ * exchange.cpp:73-81); with pl12 == pl34 the library then halves the work. */
+int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
+                  int nterm, const double* scales, const double* omegas, float thresp, float thredp,
+                  double* K13_dev);
+
+/* Device memory helpers for callers without their own HIP allocator. */
+int lsjk_malloc(void** ptr, size_t bytes);
+int lsjk_free(void* ptr);
+int lsjk_memcpy_htod(void* dst, const void* src, size_t bytes);
+int lsjk_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int lsjk_memset_zero(void* dst, size_t bytes);
+int lsjk_synchronize(void);
+
+/* Instrumentation of the last build call: quartets evaluated per class (1024 counters: J at
+ * [L12*5 + L34], K at [256 + ((L1*5+L2)*5+L3)*5+L4]; enabled by lsjk_set_counting(1)) and the
+ * device time of each phase in milliseconds (HIP events on the library stream):
+ * ms[0] = whole call, ms[1] = quartet kernels only. */
+int lsjk_set_counting(int enable);
+int lsjk_last_counts(uint64_t* counts1024);
+int lsjk_last_timing(double* ms2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSJK_H */

@@ -1,0 +1,807 @@
+// C-ABI of liblsjk: session, device-resident basis / pair-list layouts, and the orchestration of
+// the J and K kernels on the library stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/lsjk.h"
+#include "internal.hpp"
+#include "kernels.hpp"
+
+namespace {
+#include "../common/rys_table.inc"
+}
+
+struct lsjk_basis : public lsjk::Basis {};
+struct lsjk_pairlist : public lsjk::PairList {};
+
+namespace {
+
+struct LsjkError : public std::runtime_error {
+    int code;
+    LsjkError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                       \
+    do {                                                                                                \
+        hipError_t e_ = (x);                                                                            \
+        if (e_ != hipSuccess)                                                                           \
+            throw LsjkError(LSJK_E_HIP, std::string("HIP error '") + hipGetErrorString(e_) + "' at " #x); \
+    } while (0)
+
+thread_local std::string g_err;
+std::recursive_mutex g_mu;
+
+struct Session {
+    bool init = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double* d_rys_coef = nullptr;
+    double* d_rys_r2 = nullptr;
+    double* d_rys_w = nullptr;
+    lsjk::RysTable rys{};
+    lsjk::CallScalars* d_sc = nullptr;
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool counting = false;
+    uint64_t counts[1024] = {0};
+    double ms[2] = {0.0, 0.0};
+};
+Session S;
+
+template <typename T>
+T* dev_upload(const std::vector<T>& v)
+{
+    T* p = nullptr;
+    size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+    HIPCHK(hipMalloc((void**)&p, bytes));
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+
+template <typename T>
+void dev_free(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void require_init()
+{
+    if (!S.init) throw LsjkError(LSJK_E_NOINIT, "lsjk: lsjk_init has not been called");
+}
+
+// Bump allocator over one grown workspace buffer (256-byte aligned pieces).
+struct Workspace {
+    std::vector<std::pair<void**, size_t>> req;
+    template <typename T>
+    void add(T** p, size_t count) { req.push_back({(void**)p, std::max<size_t>(1, count) * sizeof(T)}); }
+    void commit()
+    {
+        size_t total = 0;
+        for (auto& r : req) total += (r.second + 255) & ~size_t(255);
+        if (total > S.ws_bytes) {
+            if (S.ws) HIPCHK(hipFree(S.ws));
+            S.ws = nullptr;
+            S.ws_bytes = 0;
+            HIPCHK(hipMalloc(&S.ws, total));
+            S.ws_bytes = total;
+        }
+        size_t off = 0;
+        for (auto& r : req) {
+            *r.first = (char*)S.ws + off;
+            off += (r.second + 255) & ~size_t(255);
+        }
+    }
+};
+
+int ret_error(const std::exception& e, int code)
+{
+    g_err = e.what();
+    return code;
+}
+
+template <typename F>
+int guarded(F&& f)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    try {
+        f();
+        g_err.clear();
+        return LSJK_OK;
+    } catch (const LsjkError& e) {
+        return ret_error(e, e.code);
+    } catch (const std::exception& e) {
+        return ret_error(e, LSJK_E_INVALID);
+    }
+}
+
+void check_ewald(int nterm, const double* scales, const double* omegas, lsjk::Ewald& ew)
+{
+    if (nterm < 0 || (nterm > 0 && (!scales || !omegas)))
+        throw LsjkError(LSJK_E_INVALID, "lsjk: invalid Ewald operator arrays");
+    ew.n = 0;
+    for (int i = 0; i < nterm; i++) {
+        // terms with a zero scale or zero omega contribute nothing (coulomb.cpp:70-73)
+        if (scales[i] == 0.0 || omegas[i] == 0.0) continue;
+        if (ew.n == lsjk::kMaxEwald) throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk: more than 8 Ewald terms");
+        ew.alpha[ew.n] = scales[i];
+        ew.omega[ew.n] = omegas[i];
+        ew.n++;
+    }
+}
+
+// ------------------------------------------------------------------ pair-list layouts
+
+void build_pairlist(lsjk_pairlist* pl)
+{
+    using namespace lsjk;
+    const Basis* b1 = pl->b1;
+    const Basis* b2 = pl->b2;
+    const int n = pl->npair;
+    // classes: contiguous (L1, L2) runs, bound descending inside each
+    pl->cls_L1.clear();
+    pl->cls_L2.clear();
+    pl->cls_start.clear();
+    for (int k = 0; k < n; k++) {
+        int L1 = b1->h_L[pl->h_p1[k]], L2 = b2->h_L[pl->h_p2[k]];
+        if (k == 0 || L1 != pl->cls_L1.back() || L2 != pl->cls_L2.back()) {
+            for (size_t c = 0; c < pl->cls_L1.size(); c++)
+                if (pl->cls_L1[c] == L1 && pl->cls_L2[c] == L2)
+                    throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: pairs are not grouped by (L1, L2) class");
+            pl->cls_L1.push_back(L1);
+            pl->cls_L2.push_back(L2);
+            pl->cls_start.push_back(k);
+        } else if (pl->h_bound[k] > pl->h_bound[k - 1]) {
+            throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: bounds not sorted descending within a class");
+        }
+    }
+    pl->cls_start.push_back(n);
+    pl->maxbound = 0.0f;
+    for (size_t c = 0; c + 1 < pl->cls_start.size(); c++)
+        if (pl->cls_start[c + 1] > pl->cls_start[c]) pl->maxbound = std::max(pl->maxbound, pl->h_bound[pl->cls_start[c]]);
+
+    // pair prefactors K = c1 c2 exp(-e1 e2 / p R^2)
+    std::vector<double> K(n);
+    for (int k = 0; k < n; k++) {
+        int i = pl->h_p1[k], j = pl->h_p2[k];
+        double e1 = b1->h_e[i], e2 = b2->h_e[j];
+        double dx = b1->h_xyz[3 * i] - b2->h_xyz[3 * j];
+        double dy = b1->h_xyz[3 * i + 1] - b2->h_xyz[3 * j + 1];
+        double dz = b1->h_xyz[3 * i + 2] - b2->h_xyz[3 * j + 2];
+        K[k] = b1->h_c[i] * b2->h_c[j] * std::exp(-e1 * e2 / (e1 + e2) * (dx * dx + dy * dy + dz * dz));
+    }
+
+    // Hermite order
+    std::vector<int> order;
+    order.reserve(n);
+    std::vector<int> hpos(n);
+    size_t hoff = 0;
+    for (int L = 0; L <= 2 * kMaxL; L++) {
+        pl->hstart[L] = (int)order.size();
+        pl->hoff[L] = hoff;
+        std::vector<int> grp;
+        for (int k = 0; k < n; k++)
+            if (b1->h_L[pl->h_p1[k]] + b2->h_L[pl->h_p2[k]] == L) grp.push_back(k);
+        std::stable_sort(grp.begin(), grp.end(), [&](int a, int b) { return pl->h_bound[a] > pl->h_bound[b]; });
+        for (int k : grp) {
+            hpos[k] = (int)order.size();
+            order.push_back(k);
+        }
+        hoff += grp.size() * (size_t)nherm(L);
+    }
+    pl->hstart[2 * kMaxL + 1] = (int)order.size();
+    pl->hoff[2 * kMaxL + 1] = hoff;
+    std::vector<double> hgeom(4 * (size_t)n);
+    pl->h_hb.assign(n, 0.0f);
+    for (int h = 0; h < n; h++) {
+        int k = order[h];
+        int i = pl->h_p1[k], j = pl->h_p2[k];
+        double e1 = b1->h_e[i], e2 = b2->h_e[j], p = e1 + e2, ip = 1.0 / p;
+        hgeom[h] = p;
+        for (int d = 0; d < 3; d++) hgeom[(size_t)(d + 1) * n + h] = (e1 * b1->h_xyz[3 * i + d] + e2 * b2->h_xyz[3 * j + d]) * ip;
+        pl->h_hb[h] = pl->h_bound[k];
+    }
+
+    // K-invert partner lists: rows = basis-1 primitives
+    struct Ent {
+        int row, part;
+        float b;
+    };
+    std::vector<Ent> ents;
+    ents.reserve(pl->symmetric ? 2 * (size_t)n : (size_t)n);
+    for (int k = 0; k < n; k++) {
+        ents.push_back({pl->h_p1[k], pl->h_p2[k], pl->h_bound[k]});
+        if (pl->symmetric && pl->h_p1[k] != pl->h_p2[k]) ents.push_back({pl->h_p2[k], pl->h_p1[k], pl->h_bound[k]});
+    }
+    const int maxLp = b2->maxL;
+    std::sort(ents.begin(), ents.end(), [&](const Ent& a, const Ent& b) {
+        if (a.row != b.row) return a.row < b.row;
+        int la = b2->h_L[a.part], lb = b2->h_L[b.part];
+        if (la != lb) return la < lb;
+        if (a.b != b.b) return a.b > b.b;
+        return a.part < b.part;
+    });
+    pl->nrow = b1->nprim;
+    pl->kmaxL2 = maxLp;
+    pl->h_row_start.assign((size_t)pl->nrow * (maxLp + 1) + 1, 0);
+    {
+        std::vector<int> cnt((size_t)pl->nrow * (maxLp + 1), 0);
+        for (auto& e : ents) cnt[(size_t)e.row * (maxLp + 1) + b2->h_L[e.part]]++;
+        for (size_t s = 0; s < cnt.size(); s++) pl->h_row_start[s + 1] = pl->h_row_start[s] + cnt[s];
+    }
+    const int ne = (int)ents.size();
+    pl->nent = ne;
+    std::vector<float> eb(ne);
+    std::vector<int> eprim(ne), ecart(ne), eblk(ne);
+    std::vector<double> eg(8 * (size_t)std::max(ne, 1));
+    for (int q = 0; q < ne; q++) {
+        const Ent& e = ents[q];
+        int r = e.row, s = e.part;
+        double er = b1->h_e[r], es = b2->h_e[s], p = er + es, ip = 1.0 / p;
+        double R2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            double dd = b1->h_xyz[3 * r + d] - b2->h_xyz[3 * s + d];
+            R2 += dd * dd;
+        }
+        eb[q] = e.b;
+        eprim[q] = s;
+        ecart[q] = b2->h_cart[s];
+        eblk[q] = b2->h_blk[s];
+        eg[q] = p;
+        for (int d = 0; d < 3; d++) eg[(size_t)(1 + d) * ne + q] = (er * b1->h_xyz[3 * r + d] + es * b2->h_xyz[3 * s + d]) * ip;
+        eg[(size_t)4 * ne + q] = b1->h_c[r] * b2->h_c[s] * std::exp(-er * es * ip * R2);
+        for (int d = 0; d < 3; d++) eg[(size_t)(5 + d) * ne + q] = b2->h_xyz[3 * s + d];
+    }
+    pl->h_rows_by_L.assign(kMaxL + 1, {});
+    for (int r = 0; r < pl->nrow; r++) {
+        if (pl->h_row_start[(size_t)(r + 1) * (maxLp + 1)] > pl->h_row_start[(size_t)r * (maxLp + 1)])
+            pl->h_rows_by_L[b1->h_L[r]].push_back(r);
+    }
+
+    // uploads
+    pl->d_p1 = dev_upload(pl->h_p1);
+    pl->d_p2 = dev_upload(pl->h_p2);
+    pl->d_bound = dev_upload(pl->h_bound);
+    pl->d_hpos = dev_upload(hpos);
+    pl->d_K = dev_upload(K);
+    pl->d_hb = dev_upload(pl->h_hb);
+    pl->d_hgeom = dev_upload(hgeom);
+    pl->d_row_start = dev_upload(pl->h_row_start);
+    pl->d_ebound = dev_upload(eb);
+    pl->d_eprim = dev_upload(eprim);
+    pl->d_ecart = dev_upload(ecart);
+    pl->d_eblk = dev_upload(eblk);
+    pl->d_egeom = dev_upload(eg);
+    pl->d_rows_by_L.assign(kMaxL + 1, nullptr);
+    for (int L = 0; L <= kMaxL; L++) pl->d_rows_by_L[L] = dev_upload(pl->h_rows_by_L[L]);
+}
+
+void free_pairlist(lsjk_pairlist* pl)
+{
+    dev_free(pl->d_p1);
+    dev_free(pl->d_p2);
+    dev_free(pl->d_bound);
+    dev_free(pl->d_hpos);
+    dev_free(pl->d_K);
+    dev_free(pl->d_hb);
+    dev_free(pl->d_hgeom);
+    dev_free(pl->d_row_start);
+    dev_free(pl->d_ebound);
+    dev_free(pl->d_eprim);
+    dev_free(pl->d_ecart);
+    dev_free(pl->d_eblk);
+    dev_free(pl->d_egeom);
+    for (auto& p : pl->d_rows_by_L) dev_free(p);
+    for (auto& kv : pl->jtasks) dev_free(kv.second.first);
+    pl->jtasks.clear();
+}
+
+// J task list for (bra list, ket list): per (L12, L34) group pair, tiles of 64 bra pairs x chunks of
+// kJChunk ket pairs, heaviest (largest bounds) first.
+constexpr int kJChunk = 1024;
+
+struct JSegments {
+    int* d_tasks = nullptr;
+    int off[5][5] = {{0}};
+    int cnt[5][5] = {{0}};
+};
+
+JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
+{
+    lsjk_pairlist* pl12 = const_cast<lsjk_pairlist*>(pl12c);
+    JSegments seg;
+    std::vector<lsjk::JTask> tasks;
+    for (int L12 = 0; L12 <= 4; L12++)
+        for (int L34 = 0; L34 <= 4; L34++) {
+            seg.off[L12][L34] = (int)tasks.size();
+            int b0 = pl12->hstart[L12], b1 = pl12->hstart[L12 + 1];
+            int k0 = pl34->hstart[L34], k1 = pl34->hstart[L34 + 1];
+            if (b1 <= b0 || k1 <= k0) continue;
+            for (int b = b0; b < b1; b += 64)
+                for (int k = k0; k < k1; k += kJChunk)
+                    tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + kJChunk, k1)});
+            seg.cnt[L12][L34] = (int)tasks.size() - seg.off[L12][L34];
+        }
+    auto it = pl12->jtasks.find(pl34);
+    if (it == pl12->jtasks.end()) {
+        lsjk::JTask* d = nullptr;
+        size_t bytes = std::max<size_t>(1, tasks.size()) * sizeof(lsjk::JTask);
+        HIPCHK(hipMalloc((void**)&d, bytes));
+        if (!tasks.empty()) HIPCHK(hipMemcpy(d, tasks.data(), tasks.size() * sizeof(lsjk::JTask), hipMemcpyHostToDevice));
+        it = pl12->jtasks.insert({pl34, {(int*)d, (int)tasks.size()}}).first;
+    }
+    seg.d_tasks = it->second.first;
+    return seg;
+}
+
+void finish_call(bool count)
+{
+    HIPCHK(hipStreamSynchronize(S.stream));
+    float m0 = 0.0f, m1 = 0.0f;
+    HIPCHK(hipEventElapsedTime(&m0, S.ev[0], S.ev[3]));
+    HIPCHK(hipEventElapsedTime(&m1, S.ev[1], S.ev[2]));
+    S.ms[0] = m0;
+    S.ms[1] = m1;
+    if (count) {
+        lsjk::CallScalars h;
+        HIPCHK(hipMemcpy(&h, S.d_sc, sizeof(h), hipMemcpyDeviceToHost));
+        std::memcpy(S.counts, h.counts, sizeof(S.counts));
+    } else {
+        std::memset(S.counts, 0, sizeof(S.counts));
+    }
+}
+
+}  // namespace
+
+// ====================================================================== C-ABI
+
+extern "C" {
+
+const char* lsjk_last_error(void) { return g_err.c_str(); }
+
+int lsjk_device_count(int* count)
+{
+    return guarded([&] {
+        if (!count) throw LsjkError(LSJK_E_INVALID, "lsjk_device_count: null pointer");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+
+int lsjk_init(int device)
+{
+    return guarded([&] {
+        if (S.init) {
+            if (S.device != device) throw LsjkError(LSJK_E_INVALID, "lsjk_init: session already open on another device");
+            return;
+        }
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw LsjkError(LSJK_E_INVALID, "lsjk_init: no such device");
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+        std::vector<double> coef(lsjk_rys_coef, lsjk_rys_coef + sizeof(lsjk_rys_coef) / sizeof(double));
+        std::vector<double> r2(lsjk_rys_asym_r2, lsjk_rys_asym_r2 + sizeof(lsjk_rys_asym_r2) / sizeof(double));
+        std::vector<double> w(lsjk_rys_asym_w, lsjk_rys_asym_w + sizeof(lsjk_rys_asym_w) / sizeof(double));
+        S.d_rys_coef = dev_upload(coef);
+        S.d_rys_r2 = dev_upload(r2);
+        S.d_rys_w = dev_upload(w);
+        S.rys.coef = S.d_rys_coef;
+        S.rys.asym_r2 = S.d_rys_r2;
+        S.rys.asym_w = S.d_rys_w;
+        for (int n = 0; n < 10; n++) {
+            S.rys.offset[n] = lsjk_rys_offset[n];
+            S.rys.nbin[n] = lsjk_rys_nbin[n];
+            S.rys.h_inv[n] = (n > 0) ? 1.0 / lsjk_rys_h[n] : 0.0;
+            S.rys.Tc[n] = lsjk_rys_Tc[n];
+        }
+        HIPCHK(hipMalloc((void**)&S.d_sc, sizeof(lsjk::CallScalars)));
+        for (auto& e : S.ev) HIPCHK(hipEventCreate(&e));
+        S.device = device;
+        S.init = true;
+    });
+}
+
+int lsjk_finalize(void)
+{
+    return guarded([&] {
+        if (!S.init) return;
+        (void)hipStreamSynchronize(S.stream);
+        dev_free(S.d_rys_coef);
+        dev_free(S.d_rys_r2);
+        dev_free(S.d_rys_w);
+        dev_free(S.d_sc);
+        if (S.ws) (void)hipFree(S.ws);
+        S.ws = nullptr;
+        S.ws_bytes = 0;
+        for (auto& e : S.ev)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(S.stream);
+        S = Session();
+    });
+}
+
+int lsjk_basis_create(int nprim, const double* c, const double* e, const double* xyz, const int* L, const int* cart,
+                      int ncart, lsjk_basis** out)
+{
+    return guarded([&] {
+        require_init();
+        if (!out || nprim < 0 || (nprim > 0 && (!c || !e || !xyz || !L || !cart)))
+            throw LsjkError(LSJK_E_INVALID, "lsjk_basis_create: null argument");
+        lsjk_basis* b = new lsjk_basis();
+        try {
+            b->nprim = nprim;
+            b->ncart = ncart;
+            b->h_c.assign(c, c + nprim);
+            b->h_e.assign(e, e + nprim);
+            b->h_xyz.assign(xyz, xyz + 3 * (size_t)nprim);
+            b->h_L.assign(L, L + nprim);
+            b->h_cart.assign(cart, cart + nprim);
+            b->maxL = 0;
+            for (int i = 0; i < nprim; i++) {
+                if (L[i] < 0 || L[i] > lsjk::kMaxL)
+                    throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk_basis_create: angular momentum above d is not supported");
+                if (cart[i] < 0 || cart[i] + lsjk::ncart(L[i]) > ncart)
+                    throw LsjkError(LSJK_E_INVALID, "lsjk_basis_create: Cartesian offset out of range");
+                b->maxL = std::max(b->maxL, L[i]);
+            }
+            // blocks: distinct Cartesian offsets (one per shell)
+            b->h_blk.resize(nprim);
+            for (int i = 0; i < nprim; i++) {
+                if (i == 0 || cart[i] != cart[i - 1]) {
+                    b->h_blk_cart.push_back(cart[i]);
+                    b->h_blk_L.push_back(L[i]);
+                } else if (L[i] != L[i - 1]) {
+                    throw LsjkError(LSJK_E_INVALID, "lsjk_basis_create: primitives sharing a Cartesian offset differ in L");
+                }
+                b->h_blk[i] = (int)b->h_blk_cart.size() - 1;
+            }
+            b->nblk = (int)b->h_blk_cart.size();
+            std::vector<double> x(nprim), y(nprim), z(nprim);
+            for (int i = 0; i < nprim; i++) {
+                x[i] = xyz[3 * i];
+                y[i] = xyz[3 * i + 1];
+                z[i] = xyz[3 * i + 2];
+            }
+            b->d_c = dev_upload(b->h_c);
+            b->d_e = dev_upload(b->h_e);
+            b->d_x = dev_upload(x);
+            b->d_y = dev_upload(y);
+            b->d_z = dev_upload(z);
+            b->d_L = dev_upload(b->h_L);
+            b->d_cart = dev_upload(b->h_cart);
+            b->d_blk = dev_upload(b->h_blk);
+            b->d_blk_cart = dev_upload(b->h_blk_cart);
+            b->d_blk_L = dev_upload(b->h_blk_L);
+        } catch (...) {
+            lsjk_basis_destroy(b);
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int lsjk_basis_destroy(lsjk_basis* b)
+{
+    return guarded([&] {
+        if (!b) return;
+        dev_free(b->d_c);
+        dev_free(b->d_e);
+        dev_free(b->d_x);
+        dev_free(b->d_y);
+        dev_free(b->d_z);
+        dev_free(b->d_L);
+        dev_free(b->d_cart);
+        dev_free(b->d_blk);
+        dev_free(b->d_blk_cart);
+        dev_free(b->d_blk_L);
+        delete b;
+    });
+}
+
+int lsjk_pairlist_create(const lsjk_basis* b1, const lsjk_basis* b2, int symmetric, int npair, const int* p1,
+                         const int* p2, const float* bound, lsjk_pairlist** out)
+{
+    return guarded([&] {
+        require_init();
+        if (!b1 || !b2 || !out || npair < 0 || (npair > 0 && (!p1 || !p2 || !bound)))
+            throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: null argument");
+        if (symmetric && (b1->nprim != b2->nprim || b1->ncart != b2->ncart))
+            throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: symmetric list over bases of different size");
+        for (int k = 0; k < npair; k++) {
+            if (p1[k] < 0 || p1[k] >= b1->nprim || p2[k] < 0 || p2[k] >= b2->nprim)
+                throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: primitive index out of range");
+            if (symmetric && p2[k] > p1[k])
+                throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: symmetric list must have p2 <= p1");
+        }
+        lsjk_pairlist* pl = new lsjk_pairlist();
+        try {
+            pl->b1 = b1;
+            pl->b2 = b2;
+            pl->symmetric = symmetric ? 1 : 0;
+            pl->npair = npair;
+            pl->h_p1.assign(p1, p1 + npair);
+            pl->h_p2.assign(p2, p2 + npair);
+            pl->h_bound.assign(bound, bound + npair);
+            build_pairlist(pl);
+        } catch (...) {
+            free_pairlist(pl);
+            delete pl;
+            throw;
+        }
+        *out = pl;
+    });
+}
+
+int lsjk_pairlist_destroy(lsjk_pairlist* pl)
+{
+    return guarded([&] {
+        if (!pl) return;
+        free_pairlist(pl);
+        delete pl;
+    });
+}
+
+int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
+                 const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev)
+{
+    (void)thredp;
+    return guarded([&] {
+        using namespace lsjk;
+        require_init();
+        if (!pl12 || !pl34 || !D34_dev || !J12_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_coulomb: null argument");
+        Ewald ew;
+        check_ewald(nterm, scales, omegas, ew);
+        hipStream_t st = S.stream;
+        const Basis *b1 = pl12->b1, *b2 = pl12->b2, *b3 = pl34->b1, *b4 = pl34->b2;
+        // workspace
+        float* kb = nullptr;
+        double* kd = nullptr;
+        double* jp = nullptr;
+        Workspace wsp;
+        wsp.add(&kb, (size_t)pl34->npair);
+        wsp.add(&kd, pl34->hoff[2 * kMaxL + 1]);
+        wsp.add(&jp, pl12->hoff[2 * kMaxL + 1]);
+        wsp.commit();
+        JSegments seg = get_jtasks(pl12, pl34);
+
+        HIPCHK(hipEventRecord(S.ev[0], st));
+        launch_fill(S.d_sc, sizeof(CallScalars), st);
+        launch_fill(jp, std::max<size_t>(1, pl12->hoff[2 * kMaxL + 1]) * sizeof(double), st);
+        const float max12 = pl12->maxbound;
+        const float thre_ket = thresp / max12;
+
+        JPrepArgs pa{};
+        pa.p1 = pl34->d_p1;
+        pa.p2 = pl34->d_p2;
+        pa.bound = pl34->d_bound;
+        pa.hpos = pl34->d_hpos;
+        pa.K = pl34->d_K;
+        pa.eA = b3->d_e; pa.xA = b3->d_x; pa.yA = b3->d_y; pa.zA = b3->d_z; pa.cartA = b3->d_cart;
+        pa.eB = b4->d_e; pa.xB = b4->d_x; pa.yB = b4->d_y; pa.zB = b4->d_z; pa.cartB = b4->d_cart;
+        pa.D = D34_dev;
+        pa.ldD = b4->ncart;
+        pa.symmetric = pl34->symmetric;
+        pa.thre_ket = thre_ket;
+        pa.kb = kb;
+        pa.kd = kd;
+        pa.sc = S.d_sc;
+        for (size_t c = 0; c + 1 < pl34->cls_start.size(); c++) {
+            int L = pl34->cls_L1[c] + pl34->cls_L2[c];
+            launch_j_ket_prep(pl34->cls_L1[c], pl34->cls_L2[c], pa, pl34->cls_start[c],
+                              pl34->cls_start[c + 1] - pl34->cls_start[c], pl34->hstart[L], pl34->hoff[L], st);
+        }
+        HIPCHK(hipGetLastError());
+
+        HIPCHK(hipEventRecord(S.ev[1], st));
+        if (ew.n > 0) {
+            for (int L12 = 0; L12 <= 4; L12++)
+                for (int L34 = 0; L34 <= 4; L34++) {
+                    if (seg.cnt[L12][L34] == 0) continue;
+                    JQuartetArgs qa{};
+                    qa.rys = S.rys;
+                    qa.ew = ew;
+                    qa.thresp = thresp;
+                    qa.count = S.counting ? 1 : 0;
+                    qa.count_cls = jcount_index(L12, L34);
+                    qa.bra_b = pl12->d_hb;
+                    qa.bra_geom = pl12->d_hgeom;
+                    qa.bra_n = pl12->npair;
+                    qa.bra_hstart = pl12->hstart[L12];
+                    qa.jp = jp + pl12->hoff[L12];
+                    qa.ket_b = kb;
+                    qa.ket_static_b = pl34->d_hb;
+                    qa.ket_geom = pl34->d_hgeom;
+                    qa.ket_n = pl34->npair;
+                    qa.ket_hstart = pl34->hstart[L34];
+                    qa.ket_d = kd + pl34->hoff[L34];
+                    qa.sc = S.d_sc;
+                    launch_j_quartets(L12, L34, qa, (const JTask*)seg.d_tasks + seg.off[L12][L34], seg.cnt[L12][L34], st);
+                }
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(S.ev[2], st));
+
+        JScatterArgs sa{};
+        sa.p1 = pl12->d_p1;
+        sa.p2 = pl12->d_p2;
+        sa.bound = pl12->d_bound;
+        sa.hpos = pl12->d_hpos;
+        sa.K = pl12->d_K;
+        sa.eA = b1->d_e; sa.xA = b1->d_x; sa.yA = b1->d_y; sa.zA = b1->d_z; sa.cartA = b1->d_cart;
+        sa.eB = b2->d_e; sa.xB = b2->d_x; sa.yB = b2->d_y; sa.zB = b2->d_z; sa.cartB = b2->d_cart;
+        sa.jp = jp;
+        sa.J = J12_dev;
+        sa.ldJ = b2->ncart;
+        sa.symmetric = pl12->symmetric;
+        sa.thresp = thresp;
+        sa.sc = S.d_sc;
+        if (ew.n > 0) {
+            for (size_t c = 0; c + 1 < pl12->cls_start.size(); c++) {
+                int L = pl12->cls_L1[c] + pl12->cls_L2[c];
+                launch_j_bra_scatter(pl12->cls_L1[c], pl12->cls_L2[c], sa, pl12->cls_start[c],
+                                     pl12->cls_start[c + 1] - pl12->cls_start[c], pl12->hstart[L], pl12->hoff[L], st);
+            }
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(S.ev[3], st));
+        finish_call(S.counting);
+    });
+}
+
+int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
+                  int nterm, const double* scales, const double* omegas, float thresp, float thredp, double* K13_dev)
+{
+    (void)thredp;
+    return guarded([&] {
+        using namespace lsjk;
+        require_init();
+        if (!pl12 || !pl34 || !D24_dev || !K13_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_exchange: null argument");
+        Ewald ew;
+        check_ewald(nterm, scales, omegas, ew);
+        hipStream_t st = S.stream;
+        const Basis *b1 = pl12->b1, *b2 = pl12->b2, *b3 = pl34->b1, *b4 = pl34->b2;
+        const bool symmode = D24symm && (pl12 == pl34);
+        double* Dsh = nullptr;
+        Workspace wsp;
+        wsp.add(&Dsh, (size_t)b2->nblk * b4->nblk);
+        wsp.commit();
+
+        HIPCHK(hipEventRecord(S.ev[0], st));
+        launch_fill(S.d_sc, sizeof(CallScalars), st);
+        launch_k_blockmax(D24_dev, b4->ncart, b2->d_blk_cart, b2->d_blk_L, b2->nblk, b4->d_blk_cart, b4->d_blk_L,
+                          b4->nblk, Dsh, S.d_sc, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(S.ev[1], st));
+        if (ew.n > 0) {
+            for (int L1 = 0; L1 <= b1->maxL; L1++)
+                for (int L3 = 0; L3 <= b3->maxL; L3++) {
+                    if (symmode && L3 < L1) continue;
+                    const auto& rows1 = pl12->h_rows_by_L[L1];
+                    const auto& rows3 = pl34->h_rows_by_L[L3];
+                    if (rows1.empty() || rows3.empty()) continue;
+                    const bool tri = symmode && (L1 == L3);
+                    long long nunits = tri ? (long long)rows1.size() * (rows1.size() + 1) / 2
+                                           : (long long)rows1.size() * rows3.size();
+                    KArgs ka{};
+                    ka.rys = S.rys;
+                    ka.ew = ew;
+                    ka.thresp = thresp;
+                    ka.count = S.counting ? 1 : 0;
+                    ka.symmetric = symmode ? 1 : 0;
+                    ka.same_L = (L1 == L3) ? 1 : 0;
+                    const lsjk_pairlist* sides[2] = {pl12, pl34};
+                    KSide* ks[2] = {&ka.s12, &ka.s34};
+                    int Ls[2] = {L1, L3};
+                    for (int q = 0; q < 2; q++) {
+                        const lsjk_pairlist* pl = sides[q];
+                        KSide& s = *ks[q];
+                        s.row_start = pl->d_row_start;
+                        s.maxLp = pl->kmaxL2;
+                        s.eb = pl->d_ebound;
+                        s.ecart = pl->d_ecart;
+                        s.eblk = pl->d_eblk;
+                        s.eg = pl->d_egeom;
+                        s.nent = pl->nent;
+                        s.rows = pl->d_rows_by_L[Ls[q]];
+                        s.nrows = (int)pl->h_rows_by_L[Ls[q]].size();
+                        s.x = pl->b1->d_x;
+                        s.y = pl->b1->d_y;
+                        s.z = pl->b1->d_z;
+                        s.cart = pl->b1->d_cart;
+                    }
+                    ka.D = D24_dev;
+                    ka.ldD = b4->ncart;
+                    ka.Dsh = Dsh;
+                    ka.ldDsh = b4->nblk;
+                    ka.K = K13_dev;
+                    ka.ldK = b3->ncart;
+                    ka.maxb12 = pl12->maxbound;
+                    ka.maxb34 = pl34->maxbound;
+                    ka.sc = S.d_sc;
+                    for (int L2 = 0; L2 <= b2->maxL; L2++)
+                        for (int L4 = 0; L4 <= b4->maxL; L4++) launch_k_quartets(L1, L2, L3, L4, ka, nunits, st);
+                }
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(S.ev[2], st));
+        HIPCHK(hipEventRecord(S.ev[3], st));
+        finish_call(S.counting);
+    });
+}
+
+int lsjk_malloc(void** ptr, size_t bytes)
+{
+    return guarded([&] {
+        require_init();
+        if (!ptr) throw LsjkError(LSJK_E_INVALID, "lsjk_malloc: null pointer");
+        HIPCHK(hipMalloc(ptr, std::max<size_t>(bytes, 1)));
+    });
+}
+int lsjk_free(void* ptr)
+{
+    return guarded([&] {
+        if (ptr) HIPCHK(hipFree(ptr));
+    });
+}
+int lsjk_memcpy_htod(void* dst, const void* src, size_t bytes)
+{
+    return guarded([&] {
+        require_init();
+        if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S.stream));
+        HIPCHK(hipStreamSynchronize(S.stream));
+    });
+}
+int lsjk_memcpy_dtoh(void* dst, const void* src, size_t bytes)
+{
+    return guarded([&] {
+        require_init();
+        if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S.stream));
+        HIPCHK(hipStreamSynchronize(S.stream));
+    });
+}
+int lsjk_memset_zero(void* dst, size_t bytes)
+{
+    return guarded([&] {
+        require_init();
+        lsjk::launch_fill(dst, bytes, S.stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(S.stream));
+    });
+}
+int lsjk_synchronize(void)
+{
+    return guarded([&] {
+        require_init();
+        HIPCHK(hipStreamSynchronize(S.stream));
+    });
+}
+int lsjk_set_counting(int enable)
+{
+    return guarded([&] { S.counting = enable != 0; });
+}
+int lsjk_last_counts(uint64_t* counts)
+{
+    return guarded([&] {
+        if (!counts) throw LsjkError(LSJK_E_INVALID, "lsjk_last_counts: null pointer");
+        std::memcpy(counts, S.counts, sizeof(S.counts));
+    });
+}
+int lsjk_last_timing(double* ms)
+{
+    return guarded([&] {
+        if (!ms) throw LsjkError(LSJK_E_INVALID, "lsjk_last_timing: null pointer");
+        ms[0] = S.ms[0];
+        ms[1] = S.ms[1];
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,319 @@
+// Direct Coulomb (J) build on gfx950: Hermite (McMurchie-Davidson) J-engine.
+//
+// Behaviour follows IntBox::coulombCPU (reference src/cpp/intbox/cpu/coulomb.cpp:14-196):
+//   * ket truncation  B34' = float(B34 * max|D block|) kept if > thresp/max12
+//     (PairListUtil::truncate_ket, src/cpp/core/pair_list_util.cpp:60-123),
+//   * bra truncation  B12 >= thresp/max34 (truncate_bra, pair_list_util.cpp:32-58),
+//   * quartet sieve   float(B12 * B34') >= thresp (coulomb.cpp:115),
+//   * Hermite densities with (-1)^|t| phases (HermiteL::cart_to_herm, hermite.cpp:9-206) and
+//     the transposed bra transform (herm_to_cart, hermite.cpp:207-411).
+// The same set of primitive quartets is evaluated; only the order of evaluation differs.
+//
+// Kernels (one launch each per call, per class):
+//   j_ket_prep<LA,LB>    one thread per ket pair: truncated bound, Hermite density d_u.
+//   j_quartets<L12,L34>  one wave per (64 bra pairs x ket chunk) task: each lane owns one bra
+//                        pair and sweeps the chunk's kets, staged through LDS; Hermite integrals
+//                        R_{t+u} by Rys quadrature; jp_t += sum_u R_{t+u} d_u in registers.
+//   j_bra_scatter<LA,LB> one thread per bra pair: J_ab += K sum_t E E E jp_t (symmetrised).
+#include "internal.hpp"
+#include "kernels.hpp"
+
+namespace lsjk {
+
+// ------------------------------------------------------------------ ket preparation
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void j_ket_prep(JPrepArgs a, int k0, int nk, int hstart, size_t hoff)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), L = LA + LB, H = nherm(L);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nk) return;
+    const int k = k0 + i;
+    const int ia = a.p1[k], ib = a.p2[k];
+    const int ca = a.cartA[ia], cb = a.cartB[ib];
+    const int pos = a.hpos[k];
+    double Dt[NA][NB];
+    double val = 0.0;
+#pragma unroll
+    for (int p = 0; p < NA; p++)
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            double v = a.D[(size_t)(ca + p) * a.ldD + (cb + q)];
+            Dt[p][q] = v;
+            val = fmax(val, fabs(v));
+        }
+    if (a.symmetric) {
+        const double scale = (ia == ib) ? 0.5 : 1.0;
+#pragma unroll
+        for (int p = 0; p < NA; p++)
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                double v = a.D[(size_t)(cb + q) * a.ldD + (ca + p)];
+                val = fmax(val, fabs(v));
+                Dt[p][q] = (Dt[p][q] + v) * scale;
+            }
+    }
+    atomicMax(&a.sc->dmax_bits, (unsigned long long)__double_as_longlong(val));
+    const float B2 = (float)(val * (double)a.bound[k]);
+    const bool keep = B2 > a.thre_ket;
+    a.kb[pos] = keep ? B2 : -1.0f;
+    double* dst = a.kd + hoff + (size_t)(pos - hstart) * H;
+    if (!keep) {
+#pragma unroll
+        for (int t = 0; t < H; t++) dst[t] = 0.0;
+        return;
+    }
+    atomicMax(&a.sc->max34_bits, __float_as_uint(B2));
+    const double eA = a.eA[ia], eB = a.eB[ib];
+    const double p = eA + eB, ip = 1.0 / p;
+    const double Ax = a.xA[ia], Ay = a.yA[ia], Az = a.zA[ia];
+    const double Bx = a.xB[ib], By = a.yB[ib], Bz = a.zB[ib];
+    const double Px = (eA * Ax + eB * Bx) * ip, Py = (eA * Ay + eB * By) * ip, Pz = (eA * Az + eB * Bz) * ip;
+    HermE<LA, LB> Ex, Ey, Ez;
+    hermite_E<LA, LB>(0.5 * ip, Px - Ax, Px - Bx, Ex);
+    hermite_E<LA, LB>(0.5 * ip, Py - Ay, Py - By, Ey);
+    hermite_E<LA, LB>(0.5 * ip, Pz - Az, Pz - Bz, Ez);
+    const double Kab = a.K[k];
+#pragma unroll
+    for (int t = 0; t < H; t++) {
+        const int tx = herm_tx(t), ty = herm_ty(t), tz = herm_tz(t);
+        double s = 0.0;
+#pragma unroll
+        for (int p = 0; p < NA; p++)
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                const int ax = cart_lx(LA, p), ay = cart_ly(LA, p), az = cart_lz(LA, p);
+                const int bx = cart_lx(LB, q), by = cart_ly(LB, q), bz = cart_lz(LB, q);
+                if (tx > ax + bx || ty > ay + by || tz > az + bz) continue;
+                s += Ex.v[tx][ax][bx] * Ey.v[ty][ay][by] * Ez.v[tz][az][bz] * Dt[p][q];
+            }
+        dst[t] = (((tx + ty + tz) & 1) ? -Kab : Kab) * s;
+    }
+}
+
+// ------------------------------------------------------------------ quartets
+// Hermite Coulomb integrals R_{tuv} (t+u+v <= L) for one bra-ket pair by Rys quadrature:
+// R_{tuv} = pref * sum_i w_i Hx_t Hy_u Hz_v with H_0 = 1, H_1 = U X, H_{n+1} = U (X H_n + n H_{n-1}),
+// U = -2 rho x_i (coulomb.cpp:138-158).
+template <int L>
+__device__ __forceinline__ void hermite_R(const RysTable& tab, const Ewald& ew, double p12, double p34,
+                                          double X, double Y, double Z, double* R)
+{
+    constexpr int NR = L / 2 + 1;
+    constexpr int HL = nherm(L);
+#pragma unroll
+    for (int t = 0; t < HL; t++) R[t] = 0.0;
+    const double psum = p12 + p34;
+    const double rho = p12 * p34 / psum;
+    const double base = 2.0 * 17.493418327624862 / (p12 * p34 * sqrt(psum));  // 2 pi^{5/2} / (p q sqrt(p+q))
+    const double r2 = X * X + Y * Y + Z * Z;
+    for (int it = 0; it < ew.n; it++) {
+        const double om = ew.omega[it];
+        double d2 = 1.0;
+        double pref = ew.alpha[it] * base;
+        if (om != -1.0) {
+            d2 = om * om / (rho + om * om);
+            pref *= sqrt(d2);
+        }
+        const double T = rho * d2 * r2;
+        double x[NR], w[NR];
+        rys_roots<NR>(tab, T, x, w);
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const double U = -2.0 * rho * d2 * x[i];
+            double Hx[L + 1], Hy[L + 1], Hz[L + 1];
+            Hx[0] = pref * w[i];
+            Hy[0] = 1.0;
+            Hz[0] = 1.0;
+            if (L > 0) {
+                Hx[1] = U * X * Hx[0];
+                Hy[1] = U * Y;
+                Hz[1] = U * Z;
+            }
+#pragma unroll
+            for (int u = 1; u < L; u++) {
+                Hx[u + 1] = U * (X * Hx[u] + u * Hx[u - 1]);
+                Hy[u + 1] = U * (Y * Hy[u] + u * Hy[u - 1]);
+                Hz[u + 1] = U * (Z * Hz[u] + u * Hz[u - 1]);
+            }
+#pragma unroll
+            for (int t = 0; t < HL; t++) R[t] += Hx[herm_tx(t)] * Hy[herm_ty(t)] * Hz[herm_tz(t)];
+        }
+    }
+}
+
+constexpr int kJTile = 64;  // kets staged per LDS round
+
+template <int L12, int L34>
+__global__ __launch_bounds__(64) void j_quartets(JQuartetArgs a, const JTask* __restrict__ tasks)
+{
+    constexpr int H12 = nherm(L12), H34 = nherm(L34), L = L12 + L34, HL = nherm(L);
+    __shared__ double s_geom[4][kJTile];
+    __shared__ float s_b[kJTile];
+    __shared__ double s_d[kJTile * H34];
+
+    const JTask task = tasks[blockIdx.x];
+    const int lane = threadIdx.x;
+    const float thresp = a.thresp;
+    const float max34 = __uint_as_float(a.sc->max34_bits);
+    const float thre_bra = thresp / max34;
+    const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
+
+    const int bra = task.bra0 + lane;
+    const bool active = lane < task.nbra;
+    float B12 = 0.0f;
+    double p12 = 1.0, Px = 0.0, Py = 0.0, Pz = 0.0;
+    if (active) {
+        B12 = a.bra_b[bra];
+        p12 = a.bra_geom[bra];
+        Px = a.bra_geom[a.bra_n + bra];
+        Py = a.bra_geom[2 * a.bra_n + bra];
+        Pz = a.bra_geom[3 * a.bra_n + bra];
+    }
+    const bool bra_ok = active && (B12 >= thre_bra);
+    // largest bra bound of the tile (sorted, descending): lane 0
+    const float B12max = a.bra_b[task.bra0];
+    if (!(a.bra_b[task.bra0] >= thre_bra)) return;  // no bra of the tile survives truncation
+
+    double jp[H12];
+#pragma unroll
+    for (int t = 0; t < H12; t++) jp[t] = 0.0;
+    unsigned long long nq = 0;
+
+    for (int k0 = task.ket0; k0 < task.ket1; k0 += kJTile) {
+        // Exact early exit: B34' <= float(dmax * B34) and the static ket bounds are sorted.
+        const float bk = (float)(dmax * (double)a.ket_static_b[k0]);
+        if (B12max * bk < thresp) break;
+        const int nk = min(kJTile, task.ket1 - k0);
+        __syncthreads();
+        if (lane < nk) {
+            const int k = k0 + lane;
+            s_b[lane] = a.ket_b[k];
+            s_geom[0][lane] = a.ket_geom[k];
+            s_geom[1][lane] = a.ket_geom[a.ket_n + k];
+            s_geom[2][lane] = a.ket_geom[2 * a.ket_n + k];
+            s_geom[3][lane] = a.ket_geom[3 * a.ket_n + k];
+        }
+        const double* src = a.ket_d + (size_t)(k0 - a.ket_hstart) * H34;
+        for (int q = lane; q < nk * H34; q += 64) s_d[q] = src[q];
+        __syncthreads();
+        if (!bra_ok) continue;
+        for (int j = 0; j < nk; j++) {
+            const float B34 = s_b[j];
+            if (B12 * B34 < thresp) continue;
+            nq++;
+            double R[HL];
+            hermite_R<L>(a.rys, a.ew, p12, s_geom[0][j], Px - s_geom[1][j], Py - s_geom[2][j], Pz - s_geom[3][j], R);
+            const double* d = s_d + j * H34;
+#pragma unroll
+            for (int u = 0; u < H34; u++) {
+                const double du = d[u];
+                const int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
+#pragma unroll
+                for (int t = 0; t < H12; t++)
+                    jp[t] = fma(R[herm_index(herm_tx(t) + ux, herm_ty(t) + uy, herm_tz(t) + uz)], du, jp[t]);
+            }
+        }
+    }
+    if (bra_ok) {
+        double* dst = a.jp + (size_t)(bra - a.bra_hstart) * H12;
+#pragma unroll
+        for (int t = 0; t < H12; t++)
+            if (jp[t] != 0.0) atomic_add(dst + t, jp[t]);
+    }
+    if (a.count) {
+        // per-wave reduction of the evaluated-quartet count
+        for (int o = 32; o > 0; o >>= 1) nq += __shfl_xor(nq, o);
+        if (lane == 0 && nq) atomicAdd(&a.sc->counts[a.count_cls], nq);
+    }
+}
+
+// ------------------------------------------------------------------ bra scatter
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void j_bra_scatter(JScatterArgs a, int k0, int nk, int hstart, size_t hoff)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), L = LA + LB, H = nherm(L);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nk) return;
+    const int k = k0 + i;
+    const float max34 = __uint_as_float(a.sc->max34_bits);
+    const float thre_bra = a.thresp / max34;
+    if (!(a.bound[k] >= thre_bra)) return;
+    const int ia = a.p1[k], ib = a.p2[k];
+    const int pos = a.hpos[k];
+    const double* jp = a.jp + hoff + (size_t)(pos - hstart) * H;
+    double jt[H];
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < H; t++) {
+        jt[t] = jp[t];
+        any |= (jt[t] != 0.0);
+    }
+    if (!any) return;
+    const double eA = a.eA[ia], eB = a.eB[ib];
+    const double p = eA + eB, ip = 1.0 / p;
+    const double Ax = a.xA[ia], Ay = a.yA[ia], Az = a.zA[ia];
+    const double Bx = a.xB[ib], By = a.yB[ib], Bz = a.zB[ib];
+    const double Px = (eA * Ax + eB * Bx) * ip, Py = (eA * Ay + eB * By) * ip, Pz = (eA * Az + eB * Bz) * ip;
+    HermE<LA, LB> Ex, Ey, Ez;
+    hermite_E<LA, LB>(0.5 * ip, Px - Ax, Px - Bx, Ex);
+    hermite_E<LA, LB>(0.5 * ip, Py - Ay, Py - By, Ey);
+    hermite_E<LA, LB>(0.5 * ip, Pz - Az, Pz - Bz, Ez);
+    double Kab = a.K[k];
+    if (a.symmetric && ia == ib) Kab *= 0.5;
+    const int ca = a.cartA[ia], cb = a.cartB[ib];
+#pragma unroll
+    for (int p = 0; p < NA; p++) {
+        const int ax = cart_lx(LA, p), ay = cart_ly(LA, p), az = cart_lz(LA, p);
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int bx = cart_lx(LB, q), by = cart_ly(LB, q), bz = cart_lz(LB, q);
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t < H; t++) {
+                const int tx = herm_tx(t), ty = herm_ty(t), tz = herm_tz(t);
+                if (tx > ax + bx || ty > ay + by || tz > az + bz) continue;
+                s += Ex.v[tx][ax][bx] * Ey.v[ty][ay][by] * Ez.v[tz][az][bz] * jt[t];
+            }
+            s *= Kab;
+            atomic_add(a.J + (size_t)(ca + p) * a.ldJ + (cb + q), s);
+            if (a.symmetric) atomic_add(a.J + (size_t)(cb + q) * a.ldJ + (ca + p), s);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+#define LSJK_L2_CASES(F) F(0, 0) F(0, 1) F(0, 2) F(1, 0) F(1, 1) F(1, 2) F(2, 0) F(2, 1) F(2, 2)
+
+void launch_j_ket_prep(int LA, int LB, const JPrepArgs& a, int k0, int nk, int hstart, size_t hoff, hipStream_t s)
+{
+    if (nk <= 0) return;
+    dim3 grid((nk + 255) / 256), block(256);
+#define CASE(X, Y) \
+    if (LA == X && LB == Y) { hipLaunchKernelGGL((j_ket_prep<X, Y>), grid, block, 0, s, a, k0, nk, hstart, hoff); return; }
+    LSJK_L2_CASES(CASE)
+#undef CASE
+}
+
+void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk, int hstart, size_t hoff, hipStream_t s)
+{
+    if (nk <= 0) return;
+    dim3 grid((nk + 255) / 256), block(256);
+#define CASE(X, Y) \
+    if (LA == X && LB == Y) { hipLaunchKernelGGL((j_bra_scatter<X, Y>), grid, block, 0, s, a, k0, nk, hstart, hoff); return; }
+    LSJK_L2_CASES(CASE)
+#undef CASE
+}
+
+void launch_j_quartets(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask, hipStream_t s)
+{
+    if (ntask <= 0) return;
+    dim3 grid(ntask), block(64);
+#define CASE(X, Y) \
+    if (L12 == X && L34 == Y) { hipLaunchKernelGGL((j_quartets<X, Y>), grid, block, 0, s, a, tasks); return; }
+#define ROW(X) CASE(X, 0) CASE(X, 1) CASE(X, 2) CASE(X, 3) CASE(X, 4)
+    ROW(0) ROW(1) ROW(2) ROW(3) ROW(4)
+#undef ROW
+#undef CASE
+}
+
+}  // namespace lsjk

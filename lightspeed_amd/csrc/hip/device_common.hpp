@@ -1,0 +1,181 @@
+// Shared device-side definitions for the J/K kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lsjk {
+
+constexpr int kMaxL = 2;            // per-primitive angular momentum supported by the kernels
+constexpr int kMaxHerm = 35;        // Hermite functions of a d-d pair (L = 4)
+constexpr int kMaxEwald = 8;        // operator terms per call
+constexpr double kPi = 3.14159265358979323846;
+
+__host__ __device__ constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
+__host__ __device__ constexpr int nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+
+// Cartesian component k of shell L in lexical order (xx, xy, xz, yy, yz, zz ...):
+// exponent of x, y, z.
+__host__ __device__ constexpr int cart_lx(int L, int k)
+{
+    int idx = 0;
+    for (int i = 0; i <= L; i++)
+        for (int j = 0; j <= i; j++, idx++)
+            if (idx == k) return L - i;
+    return 0;
+}
+__host__ __device__ constexpr int cart_ly(int L, int k)
+{
+    int idx = 0;
+    for (int i = 0; i <= L; i++)
+        for (int j = 0; j <= i; j++, idx++)
+            if (idx == k) return i - j;
+    return 0;
+}
+__host__ __device__ constexpr int cart_lz(int L, int k)
+{
+    int idx = 0;
+    for (int i = 0; i <= L; i++)
+        for (int j = 0; j <= i; j++, idx++)
+            if (idx == k) return j;
+    return 0;
+}
+
+// Hermite index t of total order <= L in the nested (L, lexical) order: components.
+__host__ __device__ constexpr int herm_tx(int t)
+{
+    int idx = 0;
+    for (int L = 0; L <= 12; L++)
+        for (int i = 0; i <= L; i++)
+            for (int j = 0; j <= i; j++, idx++)
+                if (idx == t) return L - i;
+    return 0;
+}
+__host__ __device__ constexpr int herm_ty(int t)
+{
+    int idx = 0;
+    for (int L = 0; L <= 12; L++)
+        for (int i = 0; i <= L; i++)
+            for (int j = 0; j <= i; j++, idx++)
+                if (idx == t) return i - j;
+    return 0;
+}
+__host__ __device__ constexpr int herm_tz(int t)
+{
+    int idx = 0;
+    for (int L = 0; L <= 12; L++)
+        for (int i = 0; i <= L; i++)
+            for (int j = 0; j <= i; j++, idx++)
+                if (idx == t) return j;
+    return 0;
+}
+// Index of Hermite tuple (tx, ty, tz) in the same order.
+__host__ __device__ constexpr int herm_index(int tx, int ty, int tz)
+{
+    int L = tx + ty + tz;
+    int base = nherm(L - 1);
+    if (L == 0) base = 0;
+    // within shell L: i = L - tx, j = tz
+    int i = L - tx;
+    int off = i * (i + 1) / 2 + tz;
+    return base + off;
+}
+
+// Rys table in device memory: pointers set up by the library at init.
+struct RysTable {
+    const double* coef;      // [offset[n] + bin*n*2*NCOEF + root*2*NCOEF + {x|w}*NCOEF + k]
+    const double* asym_r2;   // [n(n-1)/2 + i]
+    const double* asym_w;
+    int offset[10];
+    int nbin[10];
+    double h_inv[10];
+    double Tc[10];
+};
+
+// Evaluate the nrys-root rule: x_i = t_i^2, w_i.  Same arithmetic as the oracle's rys_eval.
+// A non-finite T takes the asymptotic branch (no table access), and the bin is clamped, so the
+// table is never read out of bounds.
+template <int NRYS>
+__device__ __forceinline__ void rys_roots(const RysTable& tab, double T, double* x, double* w)
+{
+    constexpr int NC = 12;  // LSJK_RYS_NCOEF
+    if (!(T < tab.Tc[NRYS])) {
+        const double iT = 1.0 / T;
+        const double s = sqrt(iT);
+#pragma unroll
+        for (int i = 0; i < NRYS; i++) {
+            x[i] = tab.asym_r2[NRYS * (NRYS - 1) / 2 + i] * iT;
+            w[i] = tab.asym_w[NRYS * (NRYS - 1) / 2 + i] * s;
+        }
+        return;
+    }
+    const double sT = fmax(T, 0.0) * tab.h_inv[NRYS];
+    const int b = min((int)sT, tab.nbin[NRYS] - 1);
+    const double u = 2.0 * (sT - b) - 1.0;
+    const double* c = tab.coef + tab.offset[NRYS] + (size_t)b * NRYS * 2 * NC;
+#pragma unroll
+    for (int i = 0; i < NRYS; i++) {
+        const double* cx = c + i * 2 * NC;
+        const double* cw = cx + NC;
+        double px = cx[NC - 1];
+        double pw = cw[NC - 1];
+#pragma unroll
+        for (int k = NC - 2; k >= 0; k--) {
+            px = fma(px, u, cx[k]);
+            pw = fma(pw, u, cw[k]);
+        }
+        x[i] = px;
+        w[i] = pw;
+    }
+}
+
+// McMurchie-Davidson Hermite expansion coefficients for one axis:
+// E[t][i][j] for i <= LA, j <= LB, t <= i + j (Helgaker, Jorgensen & Olsen eqs. 9.5.6-9.5.7):
+//   E^{i+1,j}_t = E^{ij}_{t-1}/(2p) + X_PA E^{ij}_t + (t+1) E^{ij}_{t+1}
+//   E^{i,j+1}_t = E^{ij}_{t-1}/(2p) + X_PB E^{ij}_t + (t+1) E^{ij}_{t+1}
+template <int LA, int LB>
+struct HermE {
+    double v[LA + LB + 1][LA + 1][LB + 1];
+};
+
+template <int LA, int LB>
+__device__ __forceinline__ void hermite_E(double oo2p, double XPA, double XPB, HermE<LA, LB>& E)
+{
+#pragma unroll
+    for (int t = 0; t <= LA + LB; t++)
+#pragma unroll
+        for (int i = 0; i <= LA; i++)
+#pragma unroll
+            for (int j = 0; j <= LB; j++) E.v[t][i][j] = 0.0;
+    E.v[0][0][0] = 1.0;
+#pragma unroll
+    for (int i = 0; i < LA; i++) {
+#pragma unroll
+        for (int t = 0; t <= i + 1; t++) {
+            double a = (t > 0) ? E.v[t - 1][i][0] * oo2p : 0.0;
+            double b = (t <= i) ? XPA * E.v[t][i][0] : 0.0;
+            double c = (t + 1 <= i) ? (t + 1) * E.v[t + 1][i][0] : 0.0;
+            E.v[t][i + 1][0] = a + b + c;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i <= LA; i++) {
+#pragma unroll
+        for (int j = 0; j < LB; j++) {
+#pragma unroll
+            for (int t = 0; t <= i + j + 1; t++) {
+                double a = (t > 0) ? E.v[t - 1][i][j] * oo2p : 0.0;
+                double b = (t <= i + j) ? XPB * E.v[t][i][j] : 0.0;
+                double c = (t + 1 <= i + j) ? (t + 1) * E.v[t + 1][i][j] : 0.0;
+                E.v[t][i][j + 1] = a + b + c;
+            }
+        }
+    }
+}
+
+// Double-precision atomic add that maps to the hardware global FP64 atomic.
+__device__ __forceinline__ void atomic_add(double* p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace lsjk

@@ -1,0 +1,339 @@
+// Direct exchange (K) build on gfx950.
+//
+// K13 += sum_{2,4} (12|34) D24 over the quartets the reference evaluates
+// (IntBox::exchangeCPU, reference src/cpp/intbox/cpu/exchange.cpp:17-355):
+//   * pairs (1,2) and (3,4) taken from the K-invert partner lists with b >= thresp/(maxb*D24max)
+//     (KBlock::build21 / build_symmetric, src/cpp/core/kvec.cpp:128-232; exchange.cpp:54-71),
+//   * quartet sieve float(b12*b34) * max|D24 block| >= thresp (exchange.cpp:145-152, 186).
+// The reference loops pivot pairs (2,4) and scatters each quartet's (1,3) block with atomics.  Here
+// the loop is output-stationary: one wave owns one (1,3) primitive pair, its lanes sweep the
+// (2,4) partner combinations, accumulate the N1 x N3 block in registers, and the wave reduces it
+// before one atomic add per element.  With a symmetric D24 and one symmetric list on both sides
+// only row1 <= row3 is computed and the transposed block is added as well (K = K^T), the same 2x
+// the reference's D24symm half-sieve (exchange.cpp:138-139, 326-336) obtains.
+//
+// Integrals: Rys quadrature with the 2-D integrals built by the vertical recurrence on centres
+// A and C and the horizontal transfer to B and D (Rys, Dupuis & King, J. Comput. Chem. 4, 154
+// (1983)), in registers, for each root x_i = t_i^2 (attenuated as t^2 d^2 for erf operators).
+#include "internal.hpp"
+#include "kernels.hpp"
+
+namespace lsjk {
+
+template <int L1, int L2, int L3, int L4>
+struct Axis4 {
+    double v[L1 + 1][L2 + 1][L3 + 1][L4 + 1];
+};
+
+template <int L1, int L2, int L3, int L4>
+__device__ __forceinline__ void rys_axis(double g00, double C00, double D00, double B10, double B01, double B00,
+                                         double AB, double CD, Axis4<L1, L2, L3, L4>& I)
+{
+    constexpr int NX = L1 + L2 + 1, MX = L3 + L4 + 1;
+    double G[NX][MX];
+    G[0][0] = g00;
+    if (NX > 1) G[1][0] = C00 * g00;
+#pragma unroll
+    for (int n = 1; n < NX - 1; n++) G[n + 1][0] = C00 * G[n][0] + n * B10 * G[n - 1][0];
+#pragma unroll
+    for (int m = 0; m < MX - 1; m++) {
+#pragma unroll
+        for (int n = 0; n < NX; n++) {
+            double v = D00 * G[n][m];
+            if (m > 0) v += m * B01 * G[n][m - 1];
+            if (n > 0) v += n * B00 * G[n - 1][m];
+            G[n][m + 1] = v;
+        }
+    }
+    // ket transfer: T[n][c][d] = (n 0 | c d)
+    double T[NX][MX][L4 + 1];
+#pragma unroll
+    for (int n = 0; n < NX; n++)
+#pragma unroll
+        for (int c = 0; c < MX; c++) T[n][c][0] = G[n][c];
+#pragma unroll
+    for (int d = 0; d < L4; d++)
+#pragma unroll
+        for (int n = 0; n < NX; n++)
+#pragma unroll
+            for (int c = 0; c < MX - 1 - d; c++) T[n][c][d + 1] = T[n][c + 1][d] + CD * T[n][c][d];
+    // bra transfer
+#pragma unroll
+    for (int c = 0; c <= L3; c++)
+#pragma unroll
+        for (int d = 0; d <= L4; d++) {
+            double U[NX][L2 + 1];
+#pragma unroll
+            for (int a = 0; a < NX; a++) U[a][0] = T[a][c][d];
+#pragma unroll
+            for (int b = 0; b < L2; b++)
+#pragma unroll
+                for (int a = 0; a < NX - 1 - b; a++) U[a][b + 1] = U[a + 1][b] + AB * U[a][b];
+#pragma unroll
+            for (int a = 0; a <= L1; a++)
+#pragma unroll
+                for (int b = 0; b <= L2; b++) I.v[a][b][c][d] = U[a][b];
+        }
+}
+
+struct KEnt {
+    double p, Px, Py, Pz, K, Bx, By, Bz;
+};
+
+__device__ __forceinline__ KEnt load_ent(const double* eg, int n, int i)
+{
+    KEnt e;
+    e.p = eg[i];
+    e.Px = eg[n + i];
+    e.Py = eg[2 * n + i];
+    e.Pz = eg[3 * n + i];
+    e.K = eg[4 * n + i];
+    e.Bx = eg[5 * n + i];
+    e.By = eg[6 * n + i];
+    e.Bz = eg[7 * n + i];
+    return e;
+}
+
+// One primitive quartet: acc[a][c] += sum_{b,d} (ab|cd) Db[b][d].
+template <int L1, int L2, int L3, int L4>
+__device__ __forceinline__ void k_quartet(const RysTable& tab, const Ewald& ew, double Ax, double Ay, double Az,
+                                          double Cx, double Cy, double Cz, const KEnt& e12, const KEnt& e34,
+                                          const double (&Db)[ncart(L2)][ncart(L4)],
+                                          double (&acc)[ncart(L1)][ncart(L3)])
+{
+    constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4);
+    constexpr int NR = (L1 + L2 + L3 + L4) / 2 + 1;
+    const double p = e12.p, q = e34.p;
+    const double pq = p + q;
+    const double ipq = 1.0 / pq;
+    const double rho = p * q * ipq;
+    const double PQx = e12.Px - e34.Px, PQy = e12.Py - e34.Py, PQz = e12.Pz - e34.Pz;
+    const double r2 = PQx * PQx + PQy * PQy + PQz * PQz;
+    const double base = 2.0 * 17.493418327624862 / (p * q * sqrt(pq)) * e12.K * e34.K;
+    const double PAx = e12.Px - Ax, PAy = e12.Py - Ay, PAz = e12.Pz - Az;
+    const double QCx = e34.Px - Cx, QCy = e34.Py - Cy, QCz = e34.Pz - Cz;
+    const double ABx = Ax - e12.Bx, ABy = Ay - e12.By, ABz = Az - e12.Bz;
+    const double CDx = Cx - e34.Bx, CDy = Cy - e34.By, CDz = Cz - e34.Bz;
+    const double qf = q * ipq, pf = p * ipq;  // rho/p, rho/q
+    const double i2p = 0.5 / p, i2q = 0.5 / q;
+    for (int it = 0; it < ew.n; it++) {
+        const double om = ew.omega[it];
+        double d2 = 1.0;
+        double pref = ew.alpha[it] * base;
+        if (om != -1.0) {
+            d2 = om * om / (rho + om * om);
+            pref *= sqrt(d2);
+        }
+        double x[NR], w[NR];
+        rys_roots<NR>(tab, rho * d2 * r2, x, w);
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const double u = x[i] * d2;
+            const double B00 = 0.5 * u * ipq;
+            const double B10 = i2p * (1.0 - u * qf);
+            const double B01 = i2q * (1.0 - u * pf);
+            const double cq = u * qf, cp = u * pf;
+            Axis4<L1, L2, L3, L4> Ix, Iy, Iz;
+            rys_axis<L1, L2, L3, L4>(pref * w[i], PAx - cq * PQx, QCx + cp * PQx, B10, B01, B00, ABx, CDx, Ix);
+            rys_axis<L1, L2, L3, L4>(1.0, PAy - cq * PQy, QCy + cp * PQy, B10, B01, B00, ABy, CDy, Iy);
+            rys_axis<L1, L2, L3, L4>(1.0, PAz - cq * PQz, QCz + cp * PQz, B10, B01, B00, ABz, CDz, Iz);
+#pragma unroll
+            for (int a = 0; a < N1; a++) {
+                constexpr int dummy = 0;
+                (void)dummy;
+                const int ax = cart_lx(L1, a), ay = cart_ly(L1, a), az = cart_lz(L1, a);
+#pragma unroll
+                for (int c = 0; c < N3; c++) {
+                    const int cx = cart_lx(L3, c), cy = cart_ly(L3, c), cz = cart_lz(L3, c);
+                    double s = 0.0;
+#pragma unroll
+                    for (int b = 0; b < N2; b++) {
+                        const int bx = cart_lx(L2, b), by = cart_ly(L2, b), bz = cart_lz(L2, b);
+#pragma unroll
+                        for (int d = 0; d < N4; d++) {
+                            const int dx = cart_lx(L4, d), dy = cart_ly(L4, d), dz = cart_lz(L4, d);
+                            s = fma(Ix.v[ax][bx][cx][dx] * Iy.v[ay][by][cy][dy] * Iz.v[az][bz][cz][dz], Db[b][d], s);
+                        }
+                    }
+                    acc[a][c] += s;
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Map a unit index to (i1, i3): rectangle, or upper triangle (i1 <= i3) of an n x n grid.
+__device__ __forceinline__ void unit_rows(long long u, int n1, int n3, int tri, int& i1, int& i3)
+{
+    if (!tri) {
+        i1 = (int)(u / n3);
+        i3 = (int)(u - (long long)i1 * n3);
+        return;
+    }
+    // row i1 holds n - i1 units; start(i1) = i1*n - i1*(i1-1)/2
+    const double n = (double)n1;
+    double disc = (2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * (double)u;
+    int r = (int)floor(((2.0 * n + 1.0) - sqrt(disc)) * 0.5);
+    if (r < 0) r = 0;
+    if (r > n1 - 1) r = n1 - 1;
+    auto start = [&](long long k) { return k * (long long)n1 - k * (k - 1) / 2; };
+    while (r > 0 && start(r) > u) r--;
+    while (r + 1 < n1 && start(r + 1) <= u) r++;
+    i1 = r;
+    i3 = r + (int)(u - start(r));
+}
+
+template <int L1, int L2, int L3, int L4>
+__global__ __launch_bounds__(256) void k_quartets(KArgs a, long long nunits)
+{
+    constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4);
+    const int lane = threadIdx.x & 63;
+    const long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (unit >= nunits) return;
+    int i1, i3;
+    unit_rows(unit, a.s12.nrows, a.s34.nrows, a.symmetric && a.same_L, i1, i3);
+    const int r1 = a.s12.rows[i1];
+    const int r3 = a.s34.rows[i3];
+    const int a0 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2];
+    const int a1 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2 + 1];
+    const int c0 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4];
+    const int c1 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4 + 1];
+    if (a1 <= a0 || c1 <= c0) return;
+
+    const float thresp = a.thresp;
+    const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
+    const float thre12 = (float)((double)thresp / ((double)a.maxb34 * dmax));
+    const float thre34 = (float)((double)thresp / ((double)a.maxb12 * dmax));
+    const float b12top = a.s12.eb[a0], b34top = a.s34.eb[c0];
+    // Effective extents: partners that can pass the sieve with the global D maximum (the lists are
+    // sorted by bound, so the passing entries are a prefix).
+    int n2 = 0, n4 = 0;
+    for (int k = lane; k < a1 - a0; k += 64) {
+        const float b = a.s12.eb[a0 + k];
+        n2 += (b >= thre12 && b34top >= thre34 && (double)(b * b34top) * dmax >= (double)thresp) ? 1 : 0;
+    }
+    for (int k = lane; k < c1 - c0; k += 64) {
+        const float b = a.s34.eb[c0 + k];
+        n4 += (b >= thre34 && b12top >= thre12 && (double)(b12top * b) * dmax >= (double)thresp) ? 1 : 0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        n2 += __shfl_xor(n2, o);
+        n4 += __shfl_xor(n4, o);
+    }
+    if (n2 == 0 || n4 == 0) return;
+
+    const double Ax = a.s12.x[r1], Ay = a.s12.y[r1], Az = a.s12.z[r1];
+    const double Cx = a.s34.x[r3], Cy = a.s34.y[r3], Cz = a.s34.z[r3];
+    double acc[N1][N3];
+#pragma unroll
+    for (int i = 0; i < N1; i++)
+#pragma unroll
+        for (int j = 0; j < N3; j++) acc[i][j] = 0.0;
+    unsigned long long nq = 0;
+    const long long ntot = (long long)n2 * n4;
+    for (long long f = lane; f < ntot; f += 64) {
+        const int j2 = (int)(f / n4);
+        const int j4 = (int)(f - (long long)j2 * n4);
+        const int e2 = a0 + j2, e4 = c0 + j4;
+        const float b12 = a.s12.eb[e2], b34 = a.s34.eb[e4];
+        if (!(b12 >= thre12 && b34 >= thre34)) continue;
+        const double dblk = a.Dsh[(size_t)a.s12.eblk[e2] * a.ldDsh + a.s34.eblk[e4]];
+        if ((double)(b12 * b34) * dblk < (double)thresp) continue;
+        nq++;
+        const int o2 = a.s12.ecart[e2], o4 = a.s34.ecart[e4];
+        double Db[N2][N4];
+#pragma unroll
+        for (int b = 0; b < N2; b++)
+#pragma unroll
+            for (int d = 0; d < N4; d++) Db[b][d] = a.D[(size_t)(o2 + b) * a.ldD + (o4 + d)];
+        const KEnt E12 = load_ent(a.s12.eg, a.s12.nent, e2);
+        const KEnt E34 = load_ent(a.s34.eg, a.s34.nent, e4);
+        k_quartet<L1, L2, L3, L4>(a.rys, a.ew, Ax, Ay, Az, Cx, Cy, Cz, E12, E34, Db, acc);
+    }
+    // wave reduction and scatter
+    const int o1 = a.s12.cart[r1], o3 = a.s34.cart[r3];
+    const bool mirror = a.symmetric && (r1 != r3);
+#pragma unroll
+    for (int i = 0; i < N1; i++)
+#pragma unroll
+        for (int j = 0; j < N3; j++) {
+            const double v = wave_sum(acc[i][j]);
+            if (lane == ((i * N3 + j) & 63) && v != 0.0) {
+                atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
+                if (mirror) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
+            }
+        }
+    if (a.count) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) nq += __shfl_xor(nq, o);
+        if (lane == 0 && nq) atomicAdd(&a.sc->counts[kcount_index(L1, L2, L3, L4)], nq);
+    }
+}
+
+// Block maxima of |D24| per (block2, block4) and the global maximum.
+__global__ void k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2, int nb2, const int* bc4,
+                           const int* bL4, int nb4, double* Dsh, CallScalars* sc)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nb2 * nb4) return;
+    const int i2 = (int)(i / nb4), i4 = (int)(i - (long long)i2 * nb4);
+    const int o2 = bc2[i2], o4 = bc4[i4];
+    const int n2 = ncart(bL2[i2]), n4 = ncart(bL4[i4]);
+    double m = 0.0;
+    for (int b = 0; b < n2; b++)
+        for (int d = 0; d < n4; d++) m = fmax(m, fabs(D[(size_t)(o2 + b) * ldD + (o4 + d)]));
+    Dsh[i] = m;
+    atomicMax(&sc->dmax_bits, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void fill_zero(unsigned int* p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+void launch_fill(void* p, size_t bytes, hipStream_t s)
+{
+    size_t n = bytes / 4;
+    if (n == 0) return;
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, s, (unsigned int*)p, n);
+}
+
+void launch_k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2, int nb2, const int* bc4,
+                       const int* bL4, int nb4, double* Dsh, CallScalars* sc, hipStream_t s)
+{
+    long long n = (long long)nb2 * nb4;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_blockmax, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, ldD, bc2, bL2, nb2, bc4,
+                       bL4, nb4, Dsh, sc);
+}
+
+void launch_k_quartets(int L1, int L2, int L3, int L4, const KArgs& a, long long nunits, hipStream_t s)
+{
+    if (nunits <= 0) return;
+    dim3 grid((unsigned)((nunits + 3) / 4)), block(256);
+#define CASE(A, B, C, D)                                                                            \
+    if (L1 == A && L2 == B && L3 == C && L4 == D) {                                                 \
+        hipLaunchKernelGGL((k_quartets<A, B, C, D>), grid, block, 0, s, a, nunits);                 \
+        return;                                                                                     \
+    }
+#define C4(A, B, C) CASE(A, B, C, 0) CASE(A, B, C, 1) CASE(A, B, C, 2)
+#define C3(A, B) C4(A, B, 0) C4(A, B, 1) C4(A, B, 2)
+#define C2(A) C3(A, 0) C3(A, 1) C3(A, 2)
+    C2(0) C2(1) C2(2)
+#undef C2
+#undef C3
+#undef C4
+#undef CASE
+}
+
+}  // namespace lsjk

@@ -1,0 +1,90 @@
+// Library-internal data structures of liblsjk (device-resident basis / pair-list data and the
+// per-call kernel argument blocks).  Not part of the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+
+namespace lsjk {
+
+// ---------------------------------------------------------------- basis
+// Primitive-level SoA copy of a basis.  "Blocks" are the distinct Cartesian function blocks
+// (one per contracted shell): primitives of one shell share a block.
+struct Basis {
+    int nprim = 0, ncart = 0, nblk = 0, maxL = 0;
+    std::vector<double> h_c, h_e, h_xyz;
+    std::vector<int> h_L, h_cart, h_blk, h_blk_cart, h_blk_L;
+    double *d_c = nullptr, *d_e = nullptr, *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
+    int *d_L = nullptr, *d_cart = nullptr, *d_blk = nullptr, *d_blk_cart = nullptr, *d_blk_L = nullptr;
+};
+
+// ---------------------------------------------------------------- pair list
+// J view ("Hermite order"): pairs grouped by total angular momentum L = L1 + L2 and sorted by
+// Schwarz bound (descending) inside each group.  The per-call ket Hermite densities and bra
+// accumulators live in this order.
+// K view ("partner lists"): for each row primitive r (a basis-1 primitive of the list) and each
+// partner angular momentum, the partner entries (basis-2 primitives) sorted by bound, descending;
+// a symmetric list contributes both orientations (the diagonal once).
+struct PairList {
+    const Basis* b1 = nullptr;
+    const Basis* b2 = nullptr;
+    int symmetric = 0;
+    int npair = 0;
+    float maxbound = 0.0f;  // PairListUtil::max_bound
+    // original list order: classes (L1, L2)
+    std::vector<int> cls_L1, cls_L2, cls_start;
+    std::vector<int> h_p1, h_p2;
+    std::vector<float> h_bound;
+    int *d_p1 = nullptr, *d_p2 = nullptr, *d_hpos = nullptr;
+    float* d_bound = nullptr;
+    double* d_K = nullptr;  // c1 c2 exp(-e1 e2/p R^2) per pair (list order)
+    // J view
+    int hstart[2 * kMaxL + 2] = {0};  // Hermite-order start of total-L group
+    size_t hoff[2 * kMaxL + 2] = {0}; // offset (in doubles) of group in the H-strided buffers
+    float* d_hb = nullptr;             // static bound, Hermite order
+    double* d_hgeom = nullptr;         // p, Px, Py, Pz (SoA, 4 * npair), Hermite order
+    std::vector<float> h_hb;
+    // K view
+    int nrow = 0;                      // = b1->nprim
+    int kmaxL2 = 0;                    // max partner L
+    std::vector<int> h_row_start;      // [nrow * (kmaxL2+1) + 1]
+    int* d_row_start = nullptr;
+    int nent = 0;
+    float* d_ebound = nullptr;
+    int* d_eprim = nullptr;   // partner primitive
+    int* d_ecart = nullptr;   // partner Cartesian offset
+    int* d_eblk = nullptr;    // partner block id
+    double* d_egeom = nullptr; // SoA: p, Px, Py, Pz, K, Bx, By, Bz  (8 * nent)
+    // active rows per angular momentum (rows with at least one partner), in primitive order
+    std::vector<std::vector<int>> h_rows_by_L;
+    std::vector<int*> d_rows_by_L;
+    // cached J task lists keyed by the ket list
+    std::map<const PairList*, std::pair<int*, int>> jtasks;
+};
+
+// ---------------------------------------------------------------- per-call argument blocks
+
+struct Ewald {
+    int n = 0;
+    double alpha[kMaxEwald];
+    double omega[kMaxEwald];
+};
+
+// Device scalars produced and consumed on the stream within one call.
+struct CallScalars {
+    unsigned int max34_bits;     // max truncated ket bound (float bits)
+    unsigned int pad;
+    unsigned long long dmax_bits;  // max |D| (double bits)
+    unsigned long long counts[1024]; // evaluated primitive quartets (see jcount_index / kcount_index)
+};
+
+struct JTask {
+    int bra0, nbra, ket0, ket1;
+};
+
+}  // namespace lsjk

@@ -1,0 +1,117 @@
+// Kernel argument blocks and launcher declarations (library-internal).
+#pragma once
+#include "internal.hpp"
+
+namespace lsjk {
+
+// Index of the quartet counters in CallScalars::counts.
+__host__ __device__ constexpr int jcount_index(int L12, int L34) { return L12 * 5 + L34; }
+__host__ __device__ constexpr int kcount_index(int L1, int L2, int L3, int L4)
+{
+    return 256 + ((L1 * 5 + L2) * 5 + L3) * 5 + L4;
+}
+
+struct JPrepArgs {
+    const int* p1;
+    const int* p2;
+    const float* bound;
+    const int* hpos;
+    const double* K;
+    const double *eA, *xA, *yA, *zA;
+    const int* cartA;
+    const double *eB, *xB, *yB, *zB;
+    const int* cartB;
+    const double* D;
+    int ldD;
+    int symmetric;
+    float thre_ket;
+    float* kb;
+    double* kd;
+    CallScalars* sc;
+};
+
+struct JQuartetArgs {
+    RysTable rys;
+    Ewald ew;
+    float thresp;
+    int count;
+    int count_cls;
+    // bra (Hermite order of the bra list)
+    const float* bra_b;
+    const double* bra_geom;  // SoA p, Px, Py, Pz, stride bra_n
+    int bra_n;
+    int bra_hstart;
+    double* jp;  // accumulators of the bra group
+    // ket (Hermite order of the ket list)
+    const float* ket_b;         // truncated bounds (per call)
+    const float* ket_static_b;  // Schwarz bounds
+    const double* ket_geom;
+    int ket_n;
+    int ket_hstart;
+    const double* ket_d;  // Hermite densities of the ket group
+    CallScalars* sc;
+};
+
+struct JScatterArgs {
+    const int* p1;
+    const int* p2;
+    const float* bound;
+    const int* hpos;
+    const double* K;
+    const double *eA, *xA, *yA, *zA;
+    const int* cartA;
+    const double *eB, *xB, *yB, *zB;
+    const int* cartB;
+    const double* jp;
+    double* J;
+    int ldJ;
+    int symmetric;
+    float thresp;
+    CallScalars* sc;
+};
+
+// K: one wave per (row1, row3) output primitive pair of fixed angular momenta.
+struct KSide {
+    const int* row_start;  // [nrow * (maxLp+1) + 1]
+    int maxLp;
+    const float* eb;       // entry bound
+    const int* ecart;      // partner Cartesian offset
+    const int* eblk;       // partner block id
+    const double* eg;      // SoA p, Px, Py, Pz, K, Bx, By, Bz (stride nent)
+    int nent;
+    const int* rows;       // active rows of this angular momentum (primitive indices)
+    int nrows;
+    const double *x, *y, *z;  // row basis coordinates
+    const int* cart;          // row basis Cartesian offsets
+};
+
+struct KArgs {
+    RysTable rys;
+    Ewald ew;
+    float thresp;
+    int count;
+    int symmetric;  // compute only row1 <= row3 and add the transposed block
+    int same_L;     // L1 == L3 (triangular unit enumeration in symmetric mode)
+    KSide s12, s34;
+    const double* D;     // D24, ncart2 x ncart4
+    int ldD;
+    const double* Dsh;   // block maxima, nblk2 x nblk4
+    int ldDsh;
+    double* K;           // K13, ncart1 x ncart3
+    int ldK;
+    float maxb12, maxb34;  // max_bound of the two lists
+    CallScalars* sc;
+};
+
+void launch_j_ket_prep(int LA, int LB, const JPrepArgs& a, int k0, int nk, int hstart, size_t hoff, hipStream_t s);
+void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk, int hstart, size_t hoff,
+                          hipStream_t s);
+void launch_j_quartets(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask, hipStream_t s);
+
+void launch_k_blockmax(const double* D, int ldD, const int* blk_cart2, const int* blk_L2, int nblk2,
+                       const int* blk_cart4, const int* blk_L4, int nblk4, double* Dsh, CallScalars* sc,
+                       hipStream_t s);
+void launch_k_quartets(int L1, int L2, int L3, int L4, const KArgs& a, long long nunits, hipStream_t s);
+void launch_fill(void* p, size_t bytes, hipStream_t s);
+
+}  // namespace lsjk

@@ -1,0 +1,143 @@
+// IntBox::coulomb / IntBox::exchange: the reference's J/K entry points (reference
+// src/lightspeed/intbox.hpp:721-828, dispatch src/cpp/intbox/intbox.cpp:213-329) routed to the HIP
+// library.  Pure <-> Cartesian transforms happen here on the host, as reference coulombCPU does
+// around its kernels (src/cpp/intbox/cpu/coulomb.cpp:26-35, 186-193).
+#include <cstring>
+
+#include "../../../include/lsjk.h"
+#include "lightspeed.hpp"
+
+namespace lightspeed {
+
+namespace {
+
+void check(int rc)
+{
+    if (rc != LSJK_OK) throw std::runtime_error(std::string("lightspeed_amd HIP backend: ") + lsjk_last_error());
+}
+
+// RAII device buffer
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) { check(lsjk_malloc(&p, bytes)); }
+    ~DevBuf()
+    {
+        if (p) lsjk_free(p);
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    double* d() const { return (double*)p; }
+};
+
+void ewald_arrays(const std::shared_ptr<Ewald>& ewald, std::vector<double>& s, std::vector<double>& o)
+{
+    s = ewald->scales();
+    o = ewald->omegas();
+}
+
+void require_ready(const char* who, const std::shared_ptr<ResourceList>& res, const std::shared_ptr<PairList>& pl12,
+                   const std::shared_ptr<PairList>& pl34)
+{
+    std::string why;
+    if (!IntBox::is_hip_ready(res, pl12, pl34, &why))
+        throw std::runtime_error(std::string(who) + ": HIP backend unavailable (" + why + ")");
+    hip_session_init(res->gpus()[0]);
+}
+
+// accumulate a freshly computed Cartesian result into the caller's tensor (reference semantics:
+// allocCart2 aliases J12 for Cartesian bases, cartToPure2 accumulates for pure ones)
+std::shared_ptr<Tensor> finish(const std::shared_ptr<Basis>& b1, const std::shared_ptr<Basis>& b2,
+                               const std::shared_ptr<Tensor>& Tcart, const std::shared_ptr<Tensor>& Tout)
+{
+    if (!b1->has_pure() && !b2->has_pure()) {
+        if (!Tout) return Tcart;
+        Tout->axpby(Tcart, 1.0, 1.0);
+        return Tout;
+    }
+    return PureTransform::cartToPure2(b1, b2, Tcart, Tout);
+}
+
+}  // namespace
+
+bool IntBox::is_hip_ready(const std::shared_ptr<ResourceList>& res, const std::shared_ptr<PairList>& pl12,
+                          const std::shared_ptr<PairList>& pl34, std::string* why)
+{
+    auto fail = [&](const char* m) {
+        if (why) *why = m;
+        return false;
+    };
+    if (!res || res->ngpu() == 0) return fail("the ResourceList has no GPU");
+    if (pl12->basis1()->max_L() > 2 || pl12->basis2()->max_L() > 2 || pl34->basis1()->max_L() > 2 ||
+        pl34->basis2()->max_L() > 2)
+        return fail("angular momentum above d is not supported");
+    return true;
+}
+
+std::shared_ptr<Tensor> IntBox::coulomb(const std::shared_ptr<ResourceList>& resources,
+                                        const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl12,
+                                        const std::shared_ptr<PairList>& pl34, const std::shared_ptr<Tensor>& D34,
+                                        float thresp, float thredp, const std::shared_ptr<Tensor>& J12)
+{
+    auto b1 = pl12->basis1(), b2 = pl12->basis2(), b3 = pl34->basis1(), b4 = pl34->basis2();
+    D34->shape_error({b3->nao(), b4->nao()});
+    if (J12) J12->shape_error({b1->nao(), b2->nao()});
+    require_ready("IntBox::coulomb", resources, pl12, pl34);
+    auto Dc = PureTransform::pureToCart2(b3, b4, D34);
+    const size_t nD = b3->ncart() * b4->ncart(), nJ = b1->ncart() * b2->ncart();
+    DevBuf dD(nD * sizeof(double)), dJ(nJ * sizeof(double));
+    check(lsjk_memcpy_htod(dD.p, Dc->data().data(), nD * sizeof(double)));
+    check(lsjk_memset_zero(dJ.p, nJ * sizeof(double)));
+    coulomb_device(ewald, pl12, pl34, dD.d(), thresp, thredp, dJ.d());
+    auto Jc = std::make_shared<Tensor>(std::vector<size_t>{b1->ncart(), b2->ncart()}, "J");
+    check(lsjk_memcpy_dtoh(Jc->data().data(), dJ.p, nJ * sizeof(double)));
+    return finish(b1, b2, Jc, J12);
+}
+
+std::shared_ptr<Tensor> IntBox::exchange(const std::shared_ptr<ResourceList>& resources,
+                                         const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl12,
+                                         const std::shared_ptr<PairList>& pl34, const std::shared_ptr<Tensor>& D24,
+                                         bool D24symm, float thresp, float thredp, const std::shared_ptr<Tensor>& K13)
+{
+    auto b1 = pl12->basis1(), b2 = pl12->basis2(), b3 = pl34->basis1(), b4 = pl34->basis2();
+    D24->shape_error({b2->nao(), b4->nao()});
+    if (K13) K13->shape_error({b1->nao(), b3->nao()});
+    // reference exchange.cpp:73-81
+    if (D24symm && !Basis::equivalent(b2, b4))
+        throw std::runtime_error("IntBox::exchange: D24symm implied, but basis sets 2 and 4 not the same");
+    if (D24symm) D24symm = Basis::equivalent(b1, b3);
+    require_ready("IntBox::exchange", resources, pl12, pl34);
+    auto Dc = PureTransform::pureToCart2(b2, b4, D24);
+    const size_t nD = b2->ncart() * b4->ncart(), nK = b1->ncart() * b3->ncart();
+    DevBuf dD(nD * sizeof(double)), dK(nK * sizeof(double));
+    check(lsjk_memcpy_htod(dD.p, Dc->data().data(), nD * sizeof(double)));
+    check(lsjk_memset_zero(dK.p, nK * sizeof(double)));
+    exchange_device(ewald, pl12, pl34, dD.d(), D24symm, thresp, thredp, dK.d());
+    auto Kc = std::make_shared<Tensor>(std::vector<size_t>{b1->ncart(), b3->ncart()}, "K");
+    check(lsjk_memcpy_dtoh(Kc->data().data(), dK.p, nK * sizeof(double)));
+    return finish(b1, b3, Kc, K13);
+}
+
+void IntBox::coulomb_device(const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl12,
+                            const std::shared_ptr<PairList>& pl34, const double* D34_dev, float thresp, float thredp,
+                            double* J12_dev)
+{
+    std::vector<double> s, o;
+    ewald_arrays(ewald, s, o);
+    lsjk_pairlist* d12 = pl12->device_handle();
+    lsjk_pairlist* d34 = (pl34 == pl12) ? d12 : pl34->device_handle();
+    check(lsjk_coulomb(d12, d34, D34_dev, (int)s.size(), s.data(), o.data(), thresp, thredp, J12_dev));
+}
+
+void IntBox::exchange_device(const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl12,
+                             const std::shared_ptr<PairList>& pl34, const double* D24_dev, bool D24symm, float thresp,
+                             float thredp, double* K13_dev)
+{
+    std::vector<double> s, o;
+    ewald_arrays(ewald, s, o);
+    lsjk_pairlist* d12 = pl12->device_handle();
+    lsjk_pairlist* d34 = (pl34 == pl12) ? d12 : pl34->device_handle();
+    check(lsjk_exchange(d12, d34, D24_dev, D24symm ? 1 : 0, (int)s.size(), s.data(), o.data(), thresp, thredp,
+                        K13_dev));
+}
+
+}  // namespace lightspeed

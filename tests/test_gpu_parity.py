@@ -1,0 +1,122 @@
+"""HIP path parity (MI355X): IntBox::coulomb / IntBox::exchange through liblsjk against the
+reference's stored results and against the CPU oracle.
+
+Tolerances: the reference's own (1e-9 prop, 1e-12 ewald/mix) for the fixtures; 1e-10 absolute
+against the FP64 oracle at the (H2O)10/6-31G workload.  Quartet counts (the screening decisions)
+must agree exactly with the oracle's.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import fixture_cases as FC
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("basname", FC.BASES)
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_prop_fixtures_gpu(product_backend, basname, symmetric):
+    _say("prop %s sym=%s" % (basname, symmetric))
+    for key, val, ref, tol in FC.prop_cases(product_backend, basname, symmetric):
+        assert np.abs(val - ref).max() < tol, key
+
+
+@pytest.mark.parametrize("basname", FC.BASES)
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_ewald_fixtures_gpu(product_backend, basname, symmetric):
+    _say("ewald %s sym=%s" % (basname, symmetric))
+    for key, val, ref, tol in FC.ewald_cases(product_backend, basname, symmetric):
+        assert np.abs(val - ref).max() < tol, key
+
+
+@pytest.mark.parametrize("combo", FC.MIX2_COMBOS)
+def test_mix2_fixtures_gpu(product_backend, combo):
+    _say("mix2 %s" % (combo,))
+    for key, val, ref, tol in FC.mix2_cases(product_backend, *combo):
+        assert np.abs(val - ref).max() < tol, key
+
+
+@pytest.mark.parametrize("rot", range(4))
+def test_mix4_fixtures_gpu(product_backend, rot):
+    _say("mix4 rot=%d" % rot)
+    for key, val, ref, tol in FC.mix4_cases(product_backend, rot):
+        assert np.abs(val - ref).max() < tol, key
+
+
+def _water10(ls):
+    from lightspeed_amd import workloads as W
+    mol, bname, nocc = W.config("h2o10_631g")
+    bas = ls.Basis.from_gbs_file(mol, bname)
+    D = W.random_density(bas.nao, nocc, 1)
+    return mol, bas, D
+
+
+def test_h2o10_631g_against_oracle(product_backend, oracle_backend):
+    """(H2O)10 / 6-31G (130 functions), Schwarz 1e-12, random orthonormal C (nocc 50)."""
+    ls = product_backend.ls
+    _say("h2o10 6-31G vs oracle")
+    mol, bas, D = _water10(ls)
+    pl = ls.PairList.build_schwarz(bas, bas, True, 1.0e-12)
+    ob = oracle_backend.O.OBasis(bas)
+    opl = oracle_backend.O.OPairList(ob, ob, True, 1.0e-12)
+    res = ls.ResourceList.build()
+    ew = ls.Ewald([1.0], [-1.0])
+    oj_counts = np.zeros(625, np.int64)
+    ok_counts = np.zeros(625, np.int64)
+    Jo = oracle_backend.O.coulomb(FC.COULOMB, opl, opl, D, 1.0e-12, counts=oj_counts)
+    Ko = oracle_backend.O.exchange(FC.COULOMB, opl, opl, D, True, 1.0e-12)
+    Ko_ns = oracle_backend.O.exchange(FC.COULOMB, opl, opl, D, False, 1.0e-12, counts=ok_counts)
+    ls.set_counting(True)
+    try:
+        J = ls.IntBox.coulomb(res, ew, pl, pl, ls.Tensor.array(D), 1.0e-12, 1.0e-12).np
+        jc = ls.last_counts()
+        K = ls.IntBox.exchange(res, ew, pl, pl, ls.Tensor.array(D), True, 1.0e-12, 1.0e-12).np
+        K_ns = ls.IntBox.exchange(res, ew, pl, pl, ls.Tensor.array(D), False, 1.0e-12, 1.0e-12).np
+        kc = ls.last_counts()
+    finally:
+        ls.set_counting(False)
+    assert np.abs(J - Jo).max() < 1.0e-10
+    assert np.abs(K - Ko).max() < 1.0e-10
+    assert np.abs(K_ns - Ko_ns).max() < 1.0e-10
+    # screening parity: J counts per (L12, L34), K counts per (L1, L2, L3, L4) class
+    oj = np.zeros(25, np.int64)
+    for L1 in range(5):
+        for L2 in range(5):
+            for L3 in range(5):
+                for L4 in range(5):
+                    n = oj_counts[((L1 * 5 + L2) * 5 + L3) * 5 + L4]
+                    if n:
+                        oj[(L1 + L2) * 5 + (L3 + L4)] += n
+    assert np.array_equal(jc[:25].astype(np.int64), oj)
+    assert np.array_equal(kc[256:256 + 625].astype(np.int64), ok_counts)
+
+
+def test_device_pointer_path(product_backend):
+    """coulomb_device / exchange_device on HBM-resident buffers equal the host-API results and
+    accumulate into their outputs."""
+    ls = product_backend.ls
+    _say("device pointer path")
+    mol, bas, D = _water10(ls)
+    pl = ls.PairList.build_schwarz(bas, bas, True, 1.0e-12)
+    res = ls.ResourceList.build()
+    ew = ls.Ewald([1.0], [-1.0])
+    J = ls.IntBox.coulomb(res, ew, pl, pl, ls.Tensor.array(D), 1.0e-12, 1.0e-12).np
+    K = ls.IntBox.exchange(res, ew, pl, pl, ls.Tensor.array(D), True, 1.0e-12, 1.0e-12).np
+    n = bas.ncart
+    dD = ls.DeviceArray([n, n])
+    dD.upload(D)
+    dJ = ls.DeviceArray([n, n])
+    dJ.upload(np.ones((n, n)))
+    dK = ls.DeviceArray([n, n])
+    ls.IntBox.coulomb_device(ew, pl, pl, dD.ptr, 1.0e-12, 1.0e-12, dJ.ptr)
+    ls.IntBox.exchange_device(ew, pl, pl, dD.ptr, True, 1.0e-12, 1.0e-12, dK.ptr)
+    ls.IntBox.exchange_device(ew, pl, pl, dD.ptr, True, 1.0e-12, 1.0e-12, dK.ptr)
+    assert np.abs(dJ.download() - (J + 1.0)).max() < 1.0e-12
+    assert np.abs(dK.download() - 2.0 * K).max() < 1.0e-12
