@@ -108,6 +108,19 @@ int lsjk_set_counting(int enable);
 int lsjk_last_counts(uint64_t* counts1024);
 int lsjk_last_timing(double* ms2);
 
+/* Quartet counts under the reference's screening conventions, no integrals evaluated (the
+ * denominator of the quartets/s metric; reference coulomb.cpp:38-52, 115 for J and
+ * exchange.cpp:54-71, 124-139, 186 for K, D24symm half-sieve included).  countsJ / countsK:
+ * 625 entries each, indexed ((L1*5+L2)*5+L3)*5+L4.  Either density may be NULL to skip it. */
+int lsjk_count_quartets(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev,
+                        const double* D24_dev, int D24symm, float thresp, uint64_t* countsJ, uint64_t* countsK);
+
+/* Per-launch timing of the quartet kernels of the last build call (HIP events around each
+ * launch, enabled by lsjk_set_profiling(1)): ids[i] is the launch's counter index (as in
+ * lsjk_last_counts), ms[i] its duration.  *n receives the number of launches. */
+int lsjk_set_profiling(int enable);
+int lsjk_last_launches(int capacity, int* n, int* ids, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,9 @@ set_counting = _native.set_counting
 last_counts = _native.last_counts
 last_timing = _native.last_timing
 synchronize = _native.synchronize
+set_profiling = _native.set_profiling
+last_launches = _native.last_launches
+count_quartets = _native.count_quartets
 
 NATIVE_LIBRARY = _os.path.join(_os.path.dirname(__file__), "liblsjk.so")
 

@@ -52,6 +52,10 @@ struct Session {
     bool counting = false;
     uint64_t counts[1024] = {0};
     double ms[2] = {0.0, 0.0};
+    bool profiling = false;
+    std::vector<hipEvent_t> kev;      // event pool for per-launch timing
+    std::vector<int> kid;             // launch ids of the last call (count index)
+    std::vector<double> kms;          // their durations (ms)
 };
 Session S;
 
@@ -281,6 +285,30 @@ void build_pairlist(lsjk_pairlist* pl)
     pl->d_egeom = dev_upload(eg);
     pl->d_rows_by_L.assign(kMaxL + 1, nullptr);
     for (int L = 0; L <= kMaxL; L++) pl->d_rows_by_L[L] = dev_upload(pl->h_rows_by_L[L]);
+    pl->d_cls_start = dev_upload(pl->cls_start);
+    pl->d_cls_L1 = dev_upload(pl->cls_L1);
+    pl->d_cls_L2 = dev_upload(pl->cls_L2);
+    if (!pl->symmetric) {
+        // pivot view: rows = basis-2 primitives, partners = basis-1 primitives (KBlock::build21)
+        std::vector<Ent> pv(n);
+        for (int k = 0; k < n; k++) pv[k] = {pl->h_p2[k], pl->h_p1[k], pl->h_bound[k]};
+        const int maxLa = b1->maxL;
+        std::sort(pv.begin(), pv.end(), [&](const Ent& a, const Ent& b) {
+            if (a.row != b.row) return a.row < b.row;
+            int la = b1->h_L[a.part], lb = b1->h_L[b.part];
+            if (la != lb) return la < lb;
+            if (a.b != b.b) return a.b > b.b;
+            return a.part < b.part;
+        });
+        std::vector<int> rs((size_t)b2->nprim * (maxLa + 1) + 1, 0);
+        std::vector<int> cnt((size_t)b2->nprim * (maxLa + 1), 0);
+        for (auto& e : pv) cnt[(size_t)e.row * (maxLa + 1) + b1->h_L[e.part]]++;
+        for (size_t q = 0; q < cnt.size(); q++) rs[q + 1] = rs[q] + cnt[q];
+        std::vector<float> pb(n);
+        for (int k = 0; k < n; k++) pb[k] = pv[k].b;
+        pl->d_piv_row_start = dev_upload(rs);
+        pl->d_piv_b = dev_upload(pb);
+    }
 }
 
 void free_pairlist(lsjk_pairlist* pl)
@@ -299,6 +327,11 @@ void free_pairlist(lsjk_pairlist* pl)
     dev_free(pl->d_eblk);
     dev_free(pl->d_egeom);
     for (auto& p : pl->d_rows_by_L) dev_free(p);
+    dev_free(pl->d_piv_row_start);
+    dev_free(pl->d_piv_b);
+    dev_free(pl->d_cls_start);
+    dev_free(pl->d_cls_L1);
+    dev_free(pl->d_cls_L2);
     for (auto& kv : pl->jtasks) dev_free(kv.second.first);
     pl->jtasks.clear();
 }
@@ -341,9 +374,37 @@ JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
     return seg;
 }
 
+// Per-launch timing: bracket a launch with an event pair from the pool when profiling.
+struct LaunchTimer {
+    int n = 0;
+    void begin(int id)
+    {
+        if (!S.profiling) return;
+        if ((size_t)(2 * n + 2) > S.kev.size()) {
+            size_t old = S.kev.size();
+            S.kev.resize(old + 64);
+            for (size_t i = old; i < S.kev.size(); i++) HIPCHK(hipEventCreate(&S.kev[i]));
+        }
+        S.kid.push_back(id);
+        HIPCHK(hipEventRecord(S.kev[2 * n], S.stream));
+    }
+    void end()
+    {
+        if (!S.profiling) return;
+        HIPCHK(hipEventRecord(S.kev[2 * n + 1], S.stream));
+        n++;
+    }
+};
+
 void finish_call(bool count)
 {
     HIPCHK(hipStreamSynchronize(S.stream));
+    S.kms.assign(S.kid.size(), 0.0);
+    for (size_t i = 0; i < S.kid.size(); i++) {
+        float m = 0.0f;
+        HIPCHK(hipEventElapsedTime(&m, S.kev[2 * i], S.kev[2 * i + 1]));
+        S.kms[i] = m;
+    }
     float m0 = 0.0f, m1 = 0.0f;
     HIPCHK(hipEventElapsedTime(&m0, S.ev[0], S.ev[3]));
     HIPCHK(hipEventElapsedTime(&m1, S.ev[1], S.ev[2]));
@@ -424,6 +485,7 @@ int lsjk_finalize(void)
         S.ws_bytes = 0;
         for (auto& e : S.ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : S.kev) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(S.stream);
         S = Session();
     });
@@ -602,6 +664,8 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
         HIPCHK(hipGetLastError());
 
         HIPCHK(hipEventRecord(S.ev[1], st));
+        S.kid.clear();
+        LaunchTimer lt;
         if (ew.n > 0) {
             for (int L12 = 0; L12 <= 4; L12++)
                 for (int L34 = 0; L34 <= 4; L34++) {
@@ -624,7 +688,9 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
                     qa.ket_hstart = pl34->hstart[L34];
                     qa.ket_d = kd + pl34->hoff[L34];
                     qa.sc = S.d_sc;
+                    lt.begin(jcount_index(L12, L34));
                     launch_j_quartets(L12, L34, qa, (const JTask*)seg.d_tasks + seg.off[L12][L34], seg.cnt[L12][L34], st);
+                    lt.end();
                 }
             HIPCHK(hipGetLastError());
         }
@@ -681,6 +747,8 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                           b4->nblk, Dsh, S.d_sc, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(S.ev[1], st));
+        S.kid.clear();
+        LaunchTimer lt;
         if (ew.n > 0) {
             for (int L1 = 0; L1 <= b1->maxL; L1++)
                 for (int L3 = 0; L3 <= b3->maxL; L3++) {
@@ -728,13 +796,125 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                     ka.maxb34 = pl34->maxbound;
                     ka.sc = S.d_sc;
                     for (int L2 = 0; L2 <= b2->maxL; L2++)
-                        for (int L4 = 0; L4 <= b4->maxL; L4++) launch_k_quartets(L1, L2, L3, L4, ka, nunits, st);
+                        for (int L4 = 0; L4 <= b4->maxL; L4++) {
+                            lt.begin(kcount_index(L1, L2, L3, L4));
+                            launch_k_quartets(L1, L2, L3, L4, ka, nunits, st);
+                            lt.end();
+                        }
                 }
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(S.ev[2], st));
         HIPCHK(hipEventRecord(S.ev[3], st));
         finish_call(S.counting);
+    });
+}
+
+int lsjk_count_quartets(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev,
+                        const double* D24_dev, int D24symm, float thresp, uint64_t* countsJ, uint64_t* countsK)
+{
+    return guarded([&] {
+        using namespace lsjk;
+        require_init();
+        if (!pl12 || !pl34 || !countsJ || !countsK || (!D34_dev && !D24_dev))
+            throw LsjkError(LSJK_E_INVALID, "lsjk_count_quartets: null argument");
+        hipStream_t st = S.stream;
+        const Basis *b2 = pl12->b2, *b3 = pl34->b1, *b4 = pl34->b2;
+        float* kb = nullptr;
+        double* kd = nullptr;
+        double* Dsh = nullptr;
+        unsigned long long* out = nullptr;
+        Workspace wsp;
+        wsp.add(&kb, (size_t)pl34->npair);
+        wsp.add(&kd, pl34->hoff[2 * kMaxL + 1]);
+        wsp.add(&Dsh, (size_t)b2->nblk * b4->nblk);
+        wsp.add(&out, 2 * 625);
+        wsp.commit();
+        launch_fill(out, 2 * 625 * sizeof(unsigned long long), st);
+        std::memset(countsJ, 0, 625 * sizeof(uint64_t));
+        std::memset(countsK, 0, 625 * sizeof(uint64_t));
+        if (D34_dev) {
+            launch_fill(S.d_sc, sizeof(CallScalars), st);
+            JPrepArgs pa{};
+            pa.p1 = pl34->d_p1; pa.p2 = pl34->d_p2; pa.bound = pl34->d_bound; pa.hpos = pl34->d_hpos; pa.K = pl34->d_K;
+            pa.eA = b3->d_e; pa.xA = b3->d_x; pa.yA = b3->d_y; pa.zA = b3->d_z; pa.cartA = b3->d_cart;
+            pa.eB = b4->d_e; pa.xB = b4->d_x; pa.yB = b4->d_y; pa.zB = b4->d_z; pa.cartB = b4->d_cart;
+            pa.D = D34_dev; pa.ldD = b4->ncart; pa.symmetric = pl34->symmetric;
+            pa.thre_ket = thresp / pl12->maxbound;
+            pa.kb = kb; pa.kd = kd; pa.sc = S.d_sc;
+            for (size_t c = 0; c + 1 < pl34->cls_start.size(); c++) {
+                int L = pl34->cls_L1[c] + pl34->cls_L2[c];
+                launch_j_ket_prep(pl34->cls_L1[c], pl34->cls_L2[c], pa, pl34->cls_start[c],
+                                  pl34->cls_start[c + 1] - pl34->cls_start[c], pl34->hstart[L], pl34->hoff[L], st);
+            }
+            CountJArgs ca{};
+            ca.thresp = thresp;
+            ca.bra_bound = pl12->d_bound;
+            ca.ket_b = kb;
+            ca.ket_hpos = pl34->d_hpos;
+            ca.ket_cls_start = pl34->d_cls_start;
+            ca.ket_cls_L1 = pl34->d_cls_L1;
+            ca.ket_cls_L2 = pl34->d_cls_L2;
+            ca.ket_ncls = (int)pl34->cls_L1.size();
+            ca.out = out;
+            ca.sc = S.d_sc;
+            for (size_t c = 0; c + 1 < pl12->cls_start.size(); c++)
+                launch_count_j(ca, pl12->cls_start[c], pl12->cls_start[c + 1] - pl12->cls_start[c], pl12->cls_L1[c],
+                               pl12->cls_L2[c], st);
+            HIPCHK(hipGetLastError());
+        }
+        if (D24_dev) {
+            launch_fill(S.d_sc, sizeof(CallScalars), st);
+            launch_k_blockmax(D24_dev, b4->ncart, b2->d_blk_cart, b2->d_blk_L, b2->nblk, b4->d_blk_cart, b4->d_blk_L,
+                              b4->nblk, Dsh, S.d_sc, st);
+            CountKArgs ka{};
+            ka.thresp = thresp;
+            ka.half = (D24symm && pl12 == pl34) ? 1 : 0;
+            ka.n2 = b2->nprim;
+            ka.n4 = b4->nprim;
+            ka.rs12 = pl12->symmetric ? pl12->d_row_start : pl12->d_piv_row_start;
+            ka.b12 = pl12->symmetric ? pl12->d_ebound : pl12->d_piv_b;
+            ka.maxLa = pl12->b1->maxL;
+            ka.rs34 = pl34->symmetric ? pl34->d_row_start : pl34->d_piv_row_start;
+            ka.b34 = pl34->symmetric ? pl34->d_ebound : pl34->d_piv_b;
+            ka.maxLc = pl34->b1->maxL;
+            ka.L2 = b2->d_L;
+            ka.L4 = b4->d_L;
+            ka.blk2 = b2->d_blk;
+            ka.blk4 = b4->d_blk;
+            ka.Dsh = Dsh;
+            ka.ldDsh = b4->nblk;
+            ka.maxb12 = pl12->maxbound;
+            ka.maxb34 = pl34->maxbound;
+            ka.out = out + 625;
+            ka.sc = S.d_sc;
+            launch_count_k(ka, st);
+            HIPCHK(hipGetLastError());
+        }
+        std::vector<unsigned long long> h(2 * 625);
+        HIPCHK(hipMemcpyAsync(h.data(), out, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int i = 0; i < 625; i++) {
+            countsJ[i] = h[i];
+            countsK[i] = h[625 + i];
+        }
+    });
+}
+
+int lsjk_set_profiling(int enable)
+{
+    return guarded([&] { S.profiling = enable != 0; });
+}
+
+int lsjk_last_launches(int capacity, int* n, int* ids, double* ms)
+{
+    return guarded([&] {
+        if (!n) throw LsjkError(LSJK_E_INVALID, "lsjk_last_launches: null pointer");
+        *n = (int)S.kms.size();
+        for (int i = 0; i < *n && i < capacity; i++) {
+            if (ids) ids[i] = S.kid[i];
+            if (ms) ms[i] = S.kms[i];
+        }
     });
 }
 

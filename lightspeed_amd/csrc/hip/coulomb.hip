@@ -73,21 +73,22 @@ __global__ __launch_bounds__(256) void j_ket_prep(JPrepArgs a, int k0, int nk, i
     hermite_E<LA, LB>(0.5 * ip, Py - Ay, Py - By, Ey);
     hermite_E<LA, LB>(0.5 * ip, Pz - Az, Pz - Bz, Ez);
     const double Kab = a.K[k];
-#pragma unroll
-    for (int t = 0; t < H; t++) {
-        const int tx = herm_tx(t), ty = herm_ty(t), tz = herm_tz(t);
+    static_for<H>([&](auto T_) {
+        constexpr int t = decltype(T_)::value;
+        constexpr int tx = herm_tx(t), ty = herm_ty(t), tz = herm_tz(t);
         double s = 0.0;
-#pragma unroll
-        for (int p = 0; p < NA; p++)
-#pragma unroll
-            for (int q = 0; q < NB; q++) {
-                const int ax = cart_lx(LA, p), ay = cart_ly(LA, p), az = cart_lz(LA, p);
-                const int bx = cart_lx(LB, q), by = cart_ly(LB, q), bz = cart_lz(LB, q);
-                if (tx > ax + bx || ty > ay + by || tz > az + bz) continue;
-                s += Ex.v[tx][ax][bx] * Ey.v[ty][ay][by] * Ez.v[tz][az][bz] * Dt[p][q];
-            }
+        static_for<NA>([&](auto P_) {
+            constexpr int p = decltype(P_)::value;
+            constexpr int ax = cart_lx(LA, p), ay = cart_ly(LA, p), az = cart_lz(LA, p);
+            static_for<NB>([&](auto Q_) {
+                constexpr int q = decltype(Q_)::value;
+                constexpr int bx = cart_lx(LB, q), by = cart_ly(LB, q), bz = cart_lz(LB, q);
+                if constexpr (tx <= ax + bx && ty <= ay + by && tz <= az + bz)
+                    s += Ex.v[tx][ax][bx] * Ey.v[ty][ay][by] * Ez.v[tz][az][bz] * Dt[p][q];
+            });
+        });
         dst[t] = (((tx + ty + tz) & 1) ? -Kab : Kab) * s;
-    }
+    });
 }
 
 // ------------------------------------------------------------------ quartets
@@ -135,8 +136,10 @@ __device__ __forceinline__ void hermite_R(const RysTable& tab, const Ewald& ew, 
                 Hy[u + 1] = U * (Y * Hy[u] + u * Hy[u - 1]);
                 Hz[u + 1] = U * (Z * Hz[u] + u * Hz[u - 1]);
             }
-#pragma unroll
-            for (int t = 0; t < HL; t++) R[t] += Hx[herm_tx(t)] * Hy[herm_ty(t)] * Hz[herm_tz(t)];
+            static_for<HL>([&](auto T_) {
+                constexpr int t = decltype(T_)::value;
+                R[t] += Hx[herm_tx(t)] * Hy[herm_ty(t)] * Hz[herm_tz(t)];
+            });
         }
     }
 }
@@ -204,14 +207,16 @@ __global__ __launch_bounds__(64) void j_quartets(JQuartetArgs a, const JTask* __
             double R[HL];
             hermite_R<L>(a.rys, a.ew, p12, s_geom[0][j], Px - s_geom[1][j], Py - s_geom[2][j], Pz - s_geom[3][j], R);
             const double* d = s_d + j * H34;
-#pragma unroll
-            for (int u = 0; u < H34; u++) {
+            static_for<H34>([&](auto U_) {
+                constexpr int u = decltype(U_)::value;
+                constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
                 const double du = d[u];
-                const int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
-#pragma unroll
-                for (int t = 0; t < H12; t++)
-                    jp[t] = fma(R[herm_index(herm_tx(t) + ux, herm_ty(t) + uy, herm_tz(t) + uz)], du, jp[t]);
-            }
+                static_for<H12>([&](auto T_) {
+                    constexpr int t = decltype(T_)::value;
+                    constexpr int idx = herm_index(herm_tx(t) + ux, herm_ty(t) + uy, herm_tz(t) + uz);
+                    jp[t] = fma(R[idx], du, jp[t]);
+                });
+            });
         }
     }
     if (bra_ok) {
@@ -261,24 +266,24 @@ __global__ __launch_bounds__(256) void j_bra_scatter(JScatterArgs a, int k0, int
     double Kab = a.K[k];
     if (a.symmetric && ia == ib) Kab *= 0.5;
     const int ca = a.cartA[ia], cb = a.cartB[ib];
-#pragma unroll
-    for (int p = 0; p < NA; p++) {
-        const int ax = cart_lx(LA, p), ay = cart_ly(LA, p), az = cart_lz(LA, p);
-#pragma unroll
-        for (int q = 0; q < NB; q++) {
-            const int bx = cart_lx(LB, q), by = cart_ly(LB, q), bz = cart_lz(LB, q);
+    static_for<NA>([&](auto P_) {
+        constexpr int p = decltype(P_)::value;
+        constexpr int ax = cart_lx(LA, p), ay = cart_ly(LA, p), az = cart_lz(LA, p);
+        static_for<NB>([&](auto Q_) {
+            constexpr int q = decltype(Q_)::value;
+            constexpr int bx = cart_lx(LB, q), by = cart_ly(LB, q), bz = cart_lz(LB, q);
             double s = 0.0;
-#pragma unroll
-            for (int t = 0; t < H; t++) {
-                const int tx = herm_tx(t), ty = herm_ty(t), tz = herm_tz(t);
-                if (tx > ax + bx || ty > ay + by || tz > az + bz) continue;
-                s += Ex.v[tx][ax][bx] * Ey.v[ty][ay][by] * Ez.v[tz][az][bz] * jt[t];
-            }
+            static_for<H>([&](auto T_) {
+                constexpr int t = decltype(T_)::value;
+                constexpr int tx = herm_tx(t), ty = herm_ty(t), tz = herm_tz(t);
+                if constexpr (tx <= ax + bx && ty <= ay + by && tz <= az + bz)
+                    s += Ex.v[tx][ax][bx] * Ey.v[ty][ay][by] * Ez.v[tz][az][bz] * jt[t];
+            });
             s *= Kab;
             atomic_add(a.J + (size_t)(ca + p) * a.ldJ + (cb + q), s);
             if (a.symmetric) atomic_add(a.J + (size_t)(cb + q) * a.ldJ + (ca + p), s);
-        }
-    }
+        });
+    });
 }
 
 // ------------------------------------------------------------------ launchers

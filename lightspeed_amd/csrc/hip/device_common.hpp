@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace lsjk {
 
 constexpr int kMaxL = 2;            // per-primitive angular momentum supported by the kernels
@@ -78,6 +80,19 @@ __host__ __device__ constexpr int herm_index(int tx, int ty, int tz)
     int i = L - tx;
     int off = i * (i + 1) / 2 + tz;
     return base + off;
+}
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1, fully expanded, so every
+// index derived from I (Hermite / Cartesian tuples, table offsets) is a constant expression.
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 // Rys table in device memory: pointers set up by the library at init.

@@ -137,27 +137,25 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const Ewald& ew, 
             rys_axis<L1, L2, L3, L4>(pref * w[i], PAx - cq * PQx, QCx + cp * PQx, B10, B01, B00, ABx, CDx, Ix);
             rys_axis<L1, L2, L3, L4>(1.0, PAy - cq * PQy, QCy + cp * PQy, B10, B01, B00, ABy, CDy, Iy);
             rys_axis<L1, L2, L3, L4>(1.0, PAz - cq * PQz, QCz + cp * PQz, B10, B01, B00, ABz, CDz, Iz);
-#pragma unroll
-            for (int a = 0; a < N1; a++) {
-                constexpr int dummy = 0;
-                (void)dummy;
-                const int ax = cart_lx(L1, a), ay = cart_ly(L1, a), az = cart_lz(L1, a);
-#pragma unroll
-                for (int c = 0; c < N3; c++) {
-                    const int cx = cart_lx(L3, c), cy = cart_ly(L3, c), cz = cart_lz(L3, c);
+            static_for<N1>([&](auto A_) {
+                constexpr int a = decltype(A_)::value;
+                constexpr int ax = cart_lx(L1, a), ay = cart_ly(L1, a), az = cart_lz(L1, a);
+                static_for<N3>([&](auto C_) {
+                    constexpr int c = decltype(C_)::value;
+                    constexpr int cx = cart_lx(L3, c), cy = cart_ly(L3, c), cz = cart_lz(L3, c);
                     double s = 0.0;
-#pragma unroll
-                    for (int b = 0; b < N2; b++) {
-                        const int bx = cart_lx(L2, b), by = cart_ly(L2, b), bz = cart_lz(L2, b);
-#pragma unroll
-                        for (int d = 0; d < N4; d++) {
-                            const int dx = cart_lx(L4, d), dy = cart_ly(L4, d), dz = cart_lz(L4, d);
+                    static_for<N2>([&](auto B_) {
+                        constexpr int b = decltype(B_)::value;
+                        constexpr int bx = cart_lx(L2, b), by = cart_ly(L2, b), bz = cart_lz(L2, b);
+                        static_for<N4>([&](auto D_) {
+                            constexpr int d = decltype(D_)::value;
+                            constexpr int dx = cart_lx(L4, d), dy = cart_ly(L4, d), dz = cart_lz(L4, d);
                             s = fma(Ix.v[ax][bx][cx][dx] * Iy.v[ay][by][cy][dy] * Iz.v[az][bz][cz][dz], Db[b][d], s);
-                        }
-                    }
+                        });
+                    });
                     acc[a][c] += s;
-                }
-            }
+                });
+            });
         }
     }
 }

@@ -60,6 +60,13 @@ struct PairList {
     int* d_ecart = nullptr;   // partner Cartesian offset
     int* d_eblk = nullptr;    // partner block id
     double* d_egeom = nullptr; // SoA: p, Px, Py, Pz, K, Bx, By, Bz  (8 * nent)
+    // pivot view for counting (rows = basis-2 primitives, partners = basis-1 primitives, bounds
+    // only); a symmetric list uses the row view above
+    int* d_piv_row_start = nullptr;
+    float* d_piv_b = nullptr;
+    int* d_cls_start = nullptr;
+    int* d_cls_L1 = nullptr;
+    int* d_cls_L2 = nullptr;
     // active rows per angular momentum (rows with at least one partner), in primitive order
     std::vector<std::vector<int>> h_rows_by_L;
     std::vector<int*> d_rows_by_L;

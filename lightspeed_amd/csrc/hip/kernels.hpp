@@ -103,6 +103,40 @@ struct KArgs {
     CallScalars* sc;
 };
 
+struct CountJArgs {
+    float thresp;
+    const float* bra_bound;   // bra list bounds (list order)
+    const float* ket_b;       // truncated ket bounds (Hermite order)
+    const int* ket_hpos;      // list order -> Hermite order
+    const int* ket_cls_start; // ket classes (list order)
+    const int* ket_cls_L1;
+    const int* ket_cls_L2;
+    int ket_ncls;
+    unsigned long long* out;
+    CallScalars* sc;
+};
+
+struct CountKArgs {
+    float thresp;
+    int half;                 // D24symm half-sieve (same list): pivots i2 <= i4 only
+    int n2, n4;               // pivot primitives of the two lists
+    const int* rs12;          // pivot-view segment starts [n2 * (maxLa+1) + 1]
+    const float* b12;         // pivot-view partner bounds
+    int maxLa;
+    const int* rs34;
+    const float* b34;
+    int maxLc;
+    const int *L2, *L4, *blk2, *blk4;  // pivot angular momenta and block ids
+    const double* Dsh;
+    int ldDsh;
+    float maxb12, maxb34;
+    unsigned long long* out;
+    CallScalars* sc;
+};
+
+void launch_count_j(const CountJArgs& a, int k0, int nk, int L1, int L2, hipStream_t s);
+void launch_count_k(const CountKArgs& a, hipStream_t s);
+
 void launch_j_ket_prep(int LA, int LB, const JPrepArgs& a, int k0, int nk, int hstart, size_t hoff, hipStream_t s);
 void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk, int hstart, size_t hoff,
                           hipStream_t s);

@@ -250,4 +250,28 @@ PYBIND11_MODULE(_lightspeed, m)
         return py::make_tuple(ms[0], ms[1]);
     });
     m.def("synchronize", []() { check(lsjk_synchronize()); });
+    m.def("set_profiling", [](bool on) { check(lsjk_set_profiling(on ? 1 : 0)); });
+    m.def("last_launches", []() {
+        int n = 0;
+        check(lsjk_last_launches(0, &n, nullptr, nullptr));
+        std::vector<int> ids(std::max(n, 1));
+        std::vector<double> ms(std::max(n, 1));
+        check(lsjk_last_launches(n, &n, ids.data(), ms.data()));
+        py::list out;
+        for (int i = 0; i < n; i++) out.append(py::make_tuple(ids[i], ms[i]));
+        return out;
+    });
+    m.def(
+        "count_quartets",
+        [](std::shared_ptr<PairList> pl12, std::shared_ptr<PairList> pl34, uintptr_t D34, uintptr_t D24,
+           bool D24symm, float thresp) {
+            lsjk_pairlist* d12 = pl12->device_handle();
+            lsjk_pairlist* d34 = (pl34 == pl12) ? d12 : pl34->device_handle();
+            py::array_t<uint64_t> cj(625), ck(625);
+            check(lsjk_count_quartets(d12, d34, (const double*)D34, (const double*)D24, D24symm ? 1 : 0, thresp,
+                                      (uint64_t*)cj.mutable_data(), (uint64_t*)ck.mutable_data()));
+            return py::make_tuple(cj, ck);
+        },
+        py::arg("pairlist12"), py::arg("pairlist34"), py::arg("D34_ptr"), py::arg("D24_ptr"), py::arg("D24symm"),
+        py::arg("thresp"));
 }

@@ -189,10 +189,44 @@ class OPairList:
             raise RuntimeError("PairList::build_schwarz: basis sets are not symmetric")
 
     def __del__(self):
+        if getattr(self, "_keep", None) is not None:
+            return  # arrays owned by numpy (a subsample)
         try:
             lib().oracle_pairlist_free(ctypes.byref(self.c))
         except Exception:
             pass
+
+    def subsample(self, stride):
+        """A pair list holding every `stride`-th pair of each class (order and bounds kept), used
+        to time the oracle on a bounded sample of a workload."""
+        out = OPairList.__new__(OPairList)
+        out.b1, out.b2, out.symmetric, out.threpq = self.b1, self.b2, self.symmetric, self.threpq
+        p1s, p2s, bds, starts, L1s, L2s = [], [], [], [0], [], []
+        for (L1, L2, p1, p2, b) in self.classes():
+            p1s.append(p1[::stride])
+            p2s.append(p2[::stride])
+            bds.append(b[::stride])
+            starts.append(starts[-1] + len(p1s[-1]))
+            L1s.append(L1)
+            L2s.append(L2)
+        p1 = np.ascontiguousarray(np.concatenate(p1s) if p1s else np.zeros(0), dtype=np.int32)
+        p2 = np.ascontiguousarray(np.concatenate(p2s) if p2s else np.zeros(0), dtype=np.int32)
+        bd = np.ascontiguousarray(np.concatenate(bds) if bds else np.zeros(0), dtype=np.float32)
+        out._keep = (p1, p2, bd)
+        c = _CPairList()
+        c.symmetric = int(self.symmetric)
+        c.npair = int(p1.size)
+        c.p1 = p1.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        c.p2 = p2.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        c.bound = bd.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        c.ncls = len(L1s)
+        for k in range(len(L1s)):
+            c.cls_L1[k] = L1s[k]
+            c.cls_L2[k] = L2s[k]
+            c.cls_start[k] = starts[k]
+        c.cls_start[len(L1s)] = starts[-1]
+        out.c = c
+        return out
 
     def classes(self):
         """[(L1, L2, p1[], p2[], bound[]), ...] in list order."""

@@ -120,3 +120,27 @@ def test_device_pointer_path(product_backend):
     ls.IntBox.exchange_device(ew, pl, pl, dD.ptr, True, 1.0e-12, 1.0e-12, dK.ptr)
     assert np.abs(dJ.download() - (J + 1.0)).max() < 1.0e-12
     assert np.abs(dK.download() - 2.0 * K).max() < 1.0e-12
+
+
+def test_reference_convention_counts(product_backend, oracle_backend):
+    """lsjk_count_quartets (the metric's denominator) equals the oracle's evaluated-quartet counts
+    per class, for J and for K with and without the D24symm half-sieve."""
+    ls = product_backend.ls
+    _say("reference-convention counts")
+    mol, bas, D = _water10(ls)
+    pl = ls.PairList.build_schwarz(bas, bas, True, 1.0e-12)
+    ob = oracle_backend.O.OBasis(bas)
+    opl = oracle_backend.O.OPairList(ob, ob, True, 1.0e-12)
+    cj = np.zeros(625, np.int64)
+    ck = np.zeros(625, np.int64)
+    ck_ns = np.zeros(625, np.int64)
+    oracle_backend.O.coulomb(FC.COULOMB, opl, opl, D, 1.0e-12, counts=cj)
+    oracle_backend.O.exchange(FC.COULOMB, opl, opl, D, True, 1.0e-12, counts=ck)
+    oracle_backend.O.exchange(FC.COULOMB, opl, opl, D, False, 1.0e-12, counts=ck_ns)
+    dD = ls.DeviceArray([bas.ncart, bas.ncart])
+    dD.upload(D)
+    nj, nk = ls.count_quartets(pl, pl, dD.ptr, dD.ptr, True, 1.0e-12)
+    _, nk_ns = ls.count_quartets(pl, pl, 0, dD.ptr, False, 1.0e-12)
+    assert np.array_equal(nj.astype(np.int64), cj)
+    assert np.array_equal(nk.astype(np.int64), ck)
+    assert np.array_equal(nk_ns.astype(np.int64), ck_ns)
