@@ -189,6 +189,8 @@ def main():
     achieved = dom_fl / (means[dom] * 1e-3)
     total_fl = sum(NJ[c] * flops_per_quartet(*cls_of(c)) + NK[c] * flops_per_quartet(*cls_of(c)) for c in range(625))
     quartets = int(NJ.sum() + NK.sum())
+    total_fl = float(total_fl)
+    dom_fl = float(dom_fl)
     dom_name = ("j_quartets<%d,%d>" % (dom[1] // 5, dom[1] % 5)) if dom[0] == "J" else \
         ("k_quartets<%d,%d,%d,%d>" % cls_of(dom[1] - 256))
     roofline = {"bound": "mfma", "pipe": "fp64 (vector ALU; the MI355X FP64 peak)", "kernel": dom_name,

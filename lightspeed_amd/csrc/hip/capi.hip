@@ -2,6 +2,7 @@
 // the J and K kernels on the library stream.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -52,12 +53,17 @@ struct Session {
     bool counting = false;
     uint64_t counts[1024] = {0};
     double ms[2] = {0.0, 0.0};
+    hipStream_t side[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
+    int nside = 4;  // side streams in use (environment LSJK_STREAMS = 1..4; 1 serialises the launches)
     bool profiling = false;
     std::vector<hipEvent_t> kev;      // event pool for per-launch timing
     std::vector<int> kid;             // launch ids of the last call (count index)
     std::vector<double> kms;          // their durations (ms)
 };
 Session S;
+constexpr int kSide = 4;
 
 template <typename T>
 T* dev_upload(const std::vector<T>& v)
@@ -263,10 +269,31 @@ void build_pairlist(lsjk_pairlist* pl)
         eg[(size_t)4 * ne + q] = b1->h_c[r] * b2->h_c[s] * std::exp(-er * es * ip * R2);
         for (int d = 0; d < 3; d++) eg[(size_t)(5 + d) * ne + q] = b2->h_xyz[3 * s + d];
     }
-    pl->h_rows_by_L.assign(kMaxL + 1, {});
-    for (int r = 0; r < pl->nrow; r++) {
-        if (pl->h_row_start[(size_t)(r + 1) * (maxLp + 1)] > pl->h_row_start[(size_t)r * (maxLp + 1)])
-            pl->h_rows_by_L[b1->h_L[r]].push_back(r);
+    for (int L = 0; L <= kMaxL; L++)
+        for (int Lp = 0; Lp <= kMaxL; Lp++) pl->h_rows[L][Lp].clear();
+    for (int r = 0; r < pl->nrow; r++)
+        for (int Lp = 0; Lp <= maxLp; Lp++)
+            if (pl->h_row_start[(size_t)r * (maxLp + 1) + Lp + 1] > pl->h_row_start[(size_t)r * (maxLp + 1) + Lp])
+                pl->h_rows[b1->h_L[r]][Lp].push_back(r);
+    // triangle enumerations (a <= b) for the symmetric-mode launches with equal row momenta; the
+    // symmetric mode depends on the list being used on both sides with a symmetric D24, not on the
+    // list itself being symmetric, so every list gets them (for every Lp <= kMaxL)
+    std::vector<int> tri_start[kMaxL + 1][kMaxL + 1][kMaxL + 1];
+    {
+        for (int L = 0; L <= kMaxL; L++)
+            for (int La = 0; La <= kMaxL; La++)
+                for (int Lb = 0; Lb <= kMaxL; Lb++) {
+                    const auto& A = pl->h_rows[L][La];
+                    const auto& B = pl->h_rows[L][Lb];
+                    auto& pre = pl->h_tri_pre[L][La][Lb];
+                    auto& st = tri_start[L][La][Lb];
+                    pre.assign(A.size() + 1, 0);
+                    st.assign(A.size(), 0);
+                    for (size_t i = 0; i < A.size(); i++) {
+                        st[i] = (int)(std::lower_bound(B.begin(), B.end(), A[i]) - B.begin());
+                        pre[i + 1] = pre[i] + (long long)(B.size() - st[i]);
+                    }
+                }
     }
 
     // uploads
@@ -283,8 +310,14 @@ void build_pairlist(lsjk_pairlist* pl)
     pl->d_ecart = dev_upload(ecart);
     pl->d_eblk = dev_upload(eblk);
     pl->d_egeom = dev_upload(eg);
-    pl->d_rows_by_L.assign(kMaxL + 1, nullptr);
-    for (int L = 0; L <= kMaxL; L++) pl->d_rows_by_L[L] = dev_upload(pl->h_rows_by_L[L]);
+    for (int L = 0; L <= kMaxL; L++)
+        for (int Lp = 0; Lp <= kMaxL; Lp++) {
+            pl->d_rows[L][Lp] = dev_upload(pl->h_rows[L][Lp]);
+            for (int Lb = 0; Lb <= kMaxL; Lb++) {
+                    pl->d_tri_pre[L][Lp][Lb] = dev_upload(pl->h_tri_pre[L][Lp][Lb]);
+                    pl->d_tri_start[L][Lp][Lb] = dev_upload(tri_start[L][Lp][Lb]);
+                }
+        }
     pl->d_cls_start = dev_upload(pl->cls_start);
     pl->d_cls_L1 = dev_upload(pl->cls_L1);
     pl->d_cls_L2 = dev_upload(pl->cls_L2);
@@ -326,7 +359,14 @@ void free_pairlist(lsjk_pairlist* pl)
     dev_free(pl->d_ecart);
     dev_free(pl->d_eblk);
     dev_free(pl->d_egeom);
-    for (auto& p : pl->d_rows_by_L) dev_free(p);
+    for (int L = 0; L <= lsjk::kMaxL; L++)
+        for (int Lp = 0; Lp <= lsjk::kMaxL; Lp++) {
+            dev_free(pl->d_rows[L][Lp]);
+            for (int Lb = 0; Lb <= lsjk::kMaxL; Lb++) {
+                dev_free(pl->d_tri_pre[L][Lp][Lb]);
+                dev_free(pl->d_tri_start[L][Lp][Lb]);
+            }
+        }
     dev_free(pl->d_piv_row_start);
     dev_free(pl->d_piv_b);
     dev_free(pl->d_cls_start);
@@ -337,8 +377,10 @@ void free_pairlist(lsjk_pairlist* pl)
 }
 
 // J task list for (bra list, ket list): per (L12, L34) group pair, tiles of 64 bra pairs x chunks of
-// kJChunk ket pairs, heaviest (largest bounds) first.
-constexpr int kJChunk = 1024;
+// ket pairs (between 64 and 1024, sized so a group pair yields about 2048 tasks), heaviest (largest
+// bounds) first.
+constexpr int kJChunkMax = 1024;
+constexpr int kJTasksTarget = 2048;
 
 struct JSegments {
     int* d_tasks = nullptr;
@@ -357,9 +399,12 @@ JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
             int b0 = pl12->hstart[L12], b1 = pl12->hstart[L12 + 1];
             int k0 = pl34->hstart[L34], k1 = pl34->hstart[L34 + 1];
             if (b1 <= b0 || k1 <= k0) continue;
+            const long long ntile = (b1 - b0 + 63) / 64;
+            long long chunk = ((long long)(k1 - k0) * ntile + kJTasksTarget - 1) / kJTasksTarget;
+            chunk = std::min<long long>(kJChunkMax, std::max<long long>(64, (chunk + 63) / 64 * 64));
             for (int b = b0; b < b1; b += 64)
-                for (int k = k0; k < k1; k += kJChunk)
-                    tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + kJChunk, k1)});
+                for (int k = k0; k < k1; k += (int)chunk)
+                    tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + (int)chunk, k1)});
             seg.cnt[L12][L34] = (int)tasks.size() - seg.off[L12][L34];
         }
     auto it = pl12->jtasks.find(pl34);
@@ -374,11 +419,32 @@ JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
     return seg;
 }
 
+// Fork/join of the class launches over the side streams: launches are dispatched heaviest first,
+// round-robin, so small classes run concurrently instead of leaving the device idle.
+struct Fanout {
+    int next = 0;
+    void begin()
+    {
+        HIPCHK(hipEventRecord(S.fork, S.stream));
+        for (int i = 0; i < kSide; i++) HIPCHK(hipStreamWaitEvent(S.side[i], S.fork, 0));
+    }
+    hipStream_t pick() { return S.side[(next++) % S.nside]; }
+    void end()
+    {
+        for (int i = 0; i < kSide; i++) {
+            HIPCHK(hipEventRecord(S.join[i], S.side[i]));
+            HIPCHK(hipStreamWaitEvent(S.stream, S.join[i], 0));
+        }
+    }
+};
+
 // Per-launch timing: bracket a launch with an event pair from the pool when profiling.
 struct LaunchTimer {
     int n = 0;
-    void begin(int id)
+    hipStream_t cur = nullptr;
+    void begin(int id, hipStream_t st)
     {
+        cur = st;
         if (!S.profiling) return;
         if ((size_t)(2 * n + 2) > S.kev.size()) {
             size_t old = S.kev.size();
@@ -386,12 +452,12 @@ struct LaunchTimer {
             for (size_t i = old; i < S.kev.size(); i++) HIPCHK(hipEventCreate(&S.kev[i]));
         }
         S.kid.push_back(id);
-        HIPCHK(hipEventRecord(S.kev[2 * n], S.stream));
+        HIPCHK(hipEventRecord(S.kev[2 * n], cur));
     }
     void end()
     {
         if (!S.profiling) return;
-        HIPCHK(hipEventRecord(S.kev[2 * n + 1], S.stream));
+        HIPCHK(hipEventRecord(S.kev[2 * n + 1], cur));
         n++;
     }
 };
@@ -449,6 +515,15 @@ int lsjk_init(int device)
         if (device < 0 || device >= ndev) throw LsjkError(LSJK_E_INVALID, "lsjk_init: no such device");
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+        for (int i = 0; i < kSide; i++) {
+            HIPCHK(hipStreamCreateWithFlags(&S.side[i], hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&S.join[i], hipEventDisableTiming));
+        }
+        HIPCHK(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
+        if (const char* env = std::getenv("LSJK_STREAMS")) {
+            int v = std::atoi(env);
+            S.nside = std::min(kSide, std::max(1, v));
+        }
         std::vector<double> coef(lsjk_rys_coef, lsjk_rys_coef + sizeof(lsjk_rys_coef) / sizeof(double));
         std::vector<double> r2(lsjk_rys_asym_r2, lsjk_rys_asym_r2 + sizeof(lsjk_rys_asym_r2) / sizeof(double));
         std::vector<double> w(lsjk_rys_asym_w, lsjk_rys_asym_w + sizeof(lsjk_rys_asym_w) / sizeof(double));
@@ -486,6 +561,11 @@ int lsjk_finalize(void)
         for (auto& e : S.ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : S.kev) (void)hipEventDestroy(e);
+        for (int i = 0; i < kSide; i++) {
+            (void)hipEventDestroy(S.join[i]);
+            (void)hipStreamDestroy(S.side[i]);
+        }
+        (void)hipEventDestroy(S.fork);
         (void)hipStreamDestroy(S.stream);
         S = Session();
     });
@@ -667,8 +747,17 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
         S.kid.clear();
         LaunchTimer lt;
         if (ew.n > 0) {
-            for (int L12 = 0; L12 <= 4; L12++)
-                for (int L34 = 0; L34 <= 4; L34++) {
+            Fanout fo;
+            fo.begin();
+            std::vector<std::pair<int, int>> order;
+            for (int L12 = 4; L12 >= 0; L12--)
+                for (int L34 = 4; L34 >= 0; L34--) order.push_back({L12, L34});
+            std::stable_sort(order.begin(), order.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                return a.first + a.second > b.first + b.second;
+            });
+            for (auto& o : order) {
+                {
+                    const int L12 = o.first, L34 = o.second;
                     if (seg.cnt[L12][L34] == 0) continue;
                     JQuartetArgs qa{};
                     qa.rys = S.rys;
@@ -688,10 +777,13 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
                     qa.ket_hstart = pl34->hstart[L34];
                     qa.ket_d = kd + pl34->hoff[L34];
                     qa.sc = S.d_sc;
-                    lt.begin(jcount_index(L12, L34));
-                    launch_j_quartets(L12, L34, qa, (const JTask*)seg.d_tasks + seg.off[L12][L34], seg.cnt[L12][L34], st);
+                    hipStream_t ss = fo.pick();
+                    lt.begin(jcount_index(L12, L34), ss);
+                    launch_j_quartets(L12, L34, qa, (const JTask*)seg.d_tasks + seg.off[L12][L34], seg.cnt[L12][L34], ss);
                     lt.end();
                 }
+            }
+            fo.end();
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(S.ev[2], st));
@@ -750,58 +842,81 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
         S.kid.clear();
         LaunchTimer lt;
         if (ew.n > 0) {
+            Fanout fo;
+            fo.begin();
+            struct KL {
+                int L1, L2, L3, L4;
+                long long nunits;
+                KArgs ka;
+            };
+            std::vector<KL> kls;
             for (int L1 = 0; L1 <= b1->maxL; L1++)
-                for (int L3 = 0; L3 <= b3->maxL; L3++) {
-                    if (symmode && L3 < L1) continue;
-                    const auto& rows1 = pl12->h_rows_by_L[L1];
-                    const auto& rows3 = pl34->h_rows_by_L[L3];
-                    if (rows1.empty() || rows3.empty()) continue;
-                    const bool tri = symmode && (L1 == L3);
-                    long long nunits = tri ? (long long)rows1.size() * (rows1.size() + 1) / 2
-                                           : (long long)rows1.size() * rows3.size();
-                    KArgs ka{};
-                    ka.rys = S.rys;
-                    ka.ew = ew;
-                    ka.thresp = thresp;
-                    ka.count = S.counting ? 1 : 0;
-                    ka.symmetric = symmode ? 1 : 0;
-                    ka.same_L = (L1 == L3) ? 1 : 0;
-                    const lsjk_pairlist* sides[2] = {pl12, pl34};
-                    KSide* ks[2] = {&ka.s12, &ka.s34};
-                    int Ls[2] = {L1, L3};
-                    for (int q = 0; q < 2; q++) {
-                        const lsjk_pairlist* pl = sides[q];
-                        KSide& s = *ks[q];
-                        s.row_start = pl->d_row_start;
-                        s.maxLp = pl->kmaxL2;
-                        s.eb = pl->d_ebound;
-                        s.ecart = pl->d_ecart;
-                        s.eblk = pl->d_eblk;
-                        s.eg = pl->d_egeom;
-                        s.nent = pl->nent;
-                        s.rows = pl->d_rows_by_L[Ls[q]];
-                        s.nrows = (int)pl->h_rows_by_L[Ls[q]].size();
-                        s.x = pl->b1->d_x;
-                        s.y = pl->b1->d_y;
-                        s.z = pl->b1->d_z;
-                        s.cart = pl->b1->d_cart;
-                    }
-                    ka.D = D24_dev;
-                    ka.ldD = b4->ncart;
-                    ka.Dsh = Dsh;
-                    ka.ldDsh = b4->nblk;
-                    ka.K = K13_dev;
-                    ka.ldK = b3->ncart;
-                    ka.maxb12 = pl12->maxbound;
-                    ka.maxb34 = pl34->maxbound;
-                    ka.sc = S.d_sc;
+                for (int L3 = 0; L3 <= b3->maxL; L3++)
                     for (int L2 = 0; L2 <= b2->maxL; L2++)
                         for (int L4 = 0; L4 <= b4->maxL; L4++) {
-                            lt.begin(kcount_index(L1, L2, L3, L4));
-                            launch_k_quartets(L1, L2, L3, L4, ka, nunits, st);
-                            lt.end();
+                            if (symmode && L3 < L1) continue;
+                            const auto& rows1 = pl12->h_rows[L1][L2];
+                            const auto& rows3 = pl34->h_rows[L3][L4];
+                            if (rows1.empty() || rows3.empty()) continue;
+                            // in the symmetric mode with equal row momenta only row1 <= row3 units
+                            // are evaluated (tiles below the diagonal exit at once)
+                            const bool tri = symmode && (L1 == L3);
+                            const long long nunits =
+                                (long long)rows1.size() * (long long)((rows3.size() + kKTile - 1) / kKTile);
+                            if (nunits == 0) continue;
+                            KArgs ka{};
+                            ka.rys = S.rys;
+                            ka.ew = ew;
+                            ka.thresp = thresp;
+                            ka.count = S.counting ? 1 : 0;
+                            ka.symmetric = symmode ? 1 : 0;
+                            ka.tri = tri ? 1 : 0;
+                            ka.tri_pre = tri ? pl12->d_tri_pre[L1][L2][L4] : nullptr;
+                            ka.tri_start = tri ? pl12->d_tri_start[L1][L2][L4] : nullptr;
+                            const lsjk_pairlist* sides[2] = {pl12, pl34};
+                            KSide* ks[2] = {&ka.s12, &ka.s34};
+                            const int Ls[2] = {L1, L3}, Lps[2] = {L2, L4};
+                            for (int q = 0; q < 2; q++) {
+                                const lsjk_pairlist* pl = sides[q];
+                                KSide& s = *ks[q];
+                                s.row_start = pl->d_row_start;
+                                s.maxLp = pl->kmaxL2;
+                                s.eb = pl->d_ebound;
+                                s.ecart = pl->d_ecart;
+                                s.eblk = pl->d_eblk;
+                                s.eg = pl->d_egeom;
+                                s.nent = pl->nent;
+                                s.rows = pl->d_rows[Ls[q]][Lps[q]];
+                                s.nrows = (int)pl->h_rows[Ls[q]][Lps[q]].size();
+                                s.x = pl->b1->d_x;
+                                s.y = pl->b1->d_y;
+                                s.z = pl->b1->d_z;
+                                s.cart = pl->b1->d_cart;
+                            }
+                            ka.D = D24_dev;
+                            ka.ldD = b4->ncart;
+                            ka.Dsh = Dsh;
+                            ka.ldDsh = b4->nblk;
+                            ka.K = K13_dev;
+                            ka.ldK = b3->ncart;
+                            ka.maxb12 = pl12->maxbound;
+                            ka.maxb34 = pl34->maxbound;
+                            ka.sc = S.d_sc;
+                            kls.push_back({L1, L2, L3, L4, nunits, ka});
                         }
-                }
+            // heaviest classes first (angular momentum, then number of output pairs)
+            std::stable_sort(kls.begin(), kls.end(), [](const KL& a, const KL& b) {
+                int la = a.L1 + a.L2 + a.L3 + a.L4, lb = b.L1 + b.L2 + b.L3 + b.L4;
+                if (la != lb) return la > lb;
+                return a.nunits > b.nunits;
+            });
+            for (auto& k : kls) {
+                hipStream_t ss = fo.pick();
+                lt.begin(kcount_index(k.L1, k.L2, k.L3, k.L4), ss);
+                launch_k_quartets(k.L1, k.L2, k.L3, k.L4, k.ka, k.nunits, ss);
+                lt.end();
+            }
+            fo.end();
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(S.ev[2], st));

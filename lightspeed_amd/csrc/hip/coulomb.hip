@@ -144,6 +144,81 @@ __device__ __forceinline__ void hermite_R(const RysTable& tab, const Ewald& ew, 
     }
 }
 
+// High angular momentum: the same contraction factorised per Rys root, so the (L+1)(L+2)(L+3)/6
+// Hermite integrals are never held at once:
+//   A[uy][tz] = sum_uz Hz[tz+uz] d[ux,uy,uz];  B[ty][tz] = sum_uy Hy[ty+uy] A[uy][tz];
+//   jp[tx,ty,tz] += Hx[tx+ux] B[ty][tz]   (for each ux).
+template <int L12, int L34>
+__device__ __forceinline__ void hermite_contract_factorized(const RysTable& tab, const Ewald& ew, double p12, double p34,
+                                                            double X, double Y, double Z, const double* d, double* jp)
+{
+    constexpr int L = L12 + L34;
+    constexpr int NR = L / 2 + 1;
+    const double psum = p12 + p34;
+    const double rho = p12 * p34 / psum;
+    const double base = 2.0 * 17.493418327624862 / (p12 * p34 * sqrt(psum));
+    const double r2 = X * X + Y * Y + Z * Z;
+    for (int it = 0; it < ew.n; it++) {
+        const double om = ew.omega[it];
+        double d2 = 1.0;
+        double pref = ew.alpha[it] * base;
+        if (om != -1.0) {
+            d2 = om * om / (rho + om * om);
+            pref *= sqrt(d2);
+        }
+        double x[NR], w[NR];
+        rys_roots<NR>(tab, rho * d2 * r2, x, w);
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const double U = -2.0 * rho * d2 * x[i];
+            double Hx[L + 1], Hy[L + 1], Hz[L + 1];
+            Hx[0] = pref * w[i];
+            Hy[0] = 1.0;
+            Hz[0] = 1.0;
+            Hx[1] = U * X * Hx[0];
+            Hy[1] = U * Y;
+            Hz[1] = U * Z;
+#pragma unroll
+            for (int u = 1; u < L; u++) {
+                Hx[u + 1] = U * (X * Hx[u] + u * Hx[u - 1]);
+                Hy[u + 1] = U * (Y * Hy[u] + u * Hy[u - 1]);
+                Hz[u + 1] = U * (Z * Hz[u] + u * Hz[u - 1]);
+            }
+            static_for<L34 + 1>([&](auto UX_) {
+                constexpr int ux = decltype(UX_)::value;
+                double A[L34 + 1 - ux][L12 + 1];
+                static_for<L34 + 1 - ux>([&](auto UY_) {
+                    constexpr int uy = decltype(UY_)::value;
+                    static_for<L12 + 1>([&](auto TZ_) {
+                        constexpr int tz = decltype(TZ_)::value;
+                        double s = 0.0;
+                        static_for<L34 + 1 - ux - uy>([&](auto UZ_) {
+                            constexpr int uz = decltype(UZ_)::value;
+                            s = fma(Hz[tz + uz], d[herm_index(ux, uy, uz)], s);
+                        });
+                        A[uy][tz] = s;
+                    });
+                });
+                static_for<L12 + 1>([&](auto TY_) {
+                    constexpr int ty = decltype(TY_)::value;
+                    static_for<L12 + 1 - ty>([&](auto TZ_) {
+                        constexpr int tz = decltype(TZ_)::value;
+                        double s = 0.0;
+                        static_for<L34 + 1 - ux>([&](auto UY_) {
+                            constexpr int uy = decltype(UY_)::value;
+                            s = fma(Hy[ty + uy], A[uy][tz], s);
+                        });
+                        static_for<L12 + 1 - ty - tz>([&](auto TX_) {
+                            constexpr int tx = decltype(TX_)::value;
+                            jp[herm_index(tx, ty, tz)] = fma(Hx[tx + ux], s, jp[herm_index(tx, ty, tz)]);
+                        });
+                    });
+                });
+            });
+        }
+    }
+}
+
 constexpr int kJTile = 64;  // kets staged per LDS round
 
 template <int L12, int L34>
@@ -204,19 +279,28 @@ __global__ __launch_bounds__(64) void j_quartets(JQuartetArgs a, const JTask* __
             const float B34 = s_b[j];
             if (B12 * B34 < thresp) continue;
             nq++;
-            double R[HL];
-            hermite_R<L>(a.rys, a.ew, p12, s_geom[0][j], Px - s_geom[1][j], Py - s_geom[2][j], Pz - s_geom[3][j], R);
-            const double* d = s_d + j * H34;
-            static_for<H34>([&](auto U_) {
-                constexpr int u = decltype(U_)::value;
-                constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
-                const double du = d[u];
-                static_for<H12>([&](auto T_) {
-                    constexpr int t = decltype(T_)::value;
-                    constexpr int idx = herm_index(herm_tx(t) + ux, herm_ty(t) + uy, herm_tz(t) + uz);
-                    jp[t] = fma(R[idx], du, jp[t]);
+            if constexpr (L >= 5) {
+                double dreg[H34];
+#pragma unroll
+                for (int u = 0; u < H34; u++) dreg[u] = s_d[j * H34 + u];
+                hermite_contract_factorized<L12, L34>(a.rys, a.ew, p12, s_geom[0][j], Px - s_geom[1][j],
+                                                      Py - s_geom[2][j], Pz - s_geom[3][j], dreg, jp);
+            } else {
+                double R[HL];
+                hermite_R<L>(a.rys, a.ew, p12, s_geom[0][j], Px - s_geom[1][j], Py - s_geom[2][j], Pz - s_geom[3][j],
+                             R);
+                const double* d = s_d + j * H34;
+                static_for<H34>([&](auto U_) {
+                    constexpr int u = decltype(U_)::value;
+                    constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
+                    const double du = d[u];
+                    static_for<H12>([&](auto T_) {
+                        constexpr int t = decltype(T_)::value;
+                        constexpr int idx = herm_index(herm_tx(t) + ux, herm_ty(t) + uy, herm_tz(t) + uz);
+                        jp[t] = fma(R[idx], du, jp[t]);
+                    });
                 });
-            });
+            }
         }
     }
     if (bra_ok) {

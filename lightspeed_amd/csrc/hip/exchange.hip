@@ -167,85 +167,48 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Map a unit index to (i1, i3): rectangle, or upper triangle (i1 <= i3) of an n x n grid.
-__device__ __forceinline__ void unit_rows(long long u, int n1, int n3, int tri, int& i1, int& i3)
-{
-    if (!tri) {
-        i1 = (int)(u / n3);
-        i3 = (int)(u - (long long)i1 * n3);
-        return;
-    }
-    // row i1 holds n - i1 units; start(i1) = i1*n - i1*(i1-1)/2
-    const double n = (double)n1;
-    double disc = (2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * (double)u;
-    int r = (int)floor(((2.0 * n + 1.0) - sqrt(disc)) * 0.5);
-    if (r < 0) r = 0;
-    if (r > n1 - 1) r = n1 - 1;
-    auto start = [&](long long k) { return k * (long long)n1 - k * (k - 1) / 2; };
-    while (r > 0 && start(r) > u) r--;
-    while (r + 1 < n1 && start(r + 1) <= u) r++;
-    i1 = r;
-    i3 = r + (int)(u - start(r));
-}
+constexpr int kKQueue = 128;
 
+// One wave per tile = (row1, block of kKTile row3s) of a class (L1 L2 | L3 L4).  For each unit
+// (row1, row3) of the tile the lanes test the sieve over the candidate rectangle (partners that can
+// pass with the global D maximum), compact the passing (2,4) combinations into an LDS queue shared
+// by all units of the tile, and evaluate the quartets 64 at a time; each quartet's N1 x N3 block is
+// added to the tile's LDS accumulator, which is flushed to K once at the end (plus the transposed
+// block in the symmetric mode).
 template <int L1, int L2, int L3, int L4>
-__global__ __launch_bounds__(256) void k_quartets(KArgs a, long long nunits)
+__global__ __launch_bounds__(64) void k_tiles(KArgs a, int nb3)
 {
-    constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4);
-    const int lane = threadIdx.x & 63;
-    const long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (unit >= nunits) return;
-    int i1, i3;
-    unit_rows(unit, a.s12.nrows, a.s34.nrows, a.symmetric && a.same_L, i1, i3);
+    constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4), N13 = N1 * N3;
+    __shared__ double s_acc[kKTile * N13];
+    __shared__ int s_e2[kKQueue], s_e4[kKQueue], s_t[kKQueue];
+    const int lane = threadIdx.x;
+    const int i1 = (int)(blockIdx.x / nb3);
+    const int ib = (int)(blockIdx.x - (unsigned)i1 * nb3);
     const int r1 = a.s12.rows[i1];
-    const int r3 = a.s34.rows[i3];
-    const int a0 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2];
-    const int a1 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2 + 1];
-    const int c0 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4];
-    const int c1 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4 + 1];
-    if (a1 <= a0 || c1 <= c0) return;
+    const int j3a = ib * kKTile;
+    const int j3b = min(j3a + kKTile, a.s34.nrows);
+    if (a.tri && a.s34.rows[j3b - 1] < r1) return;  // rows are in primitive order
+    for (int q = lane; q < kKTile * N13; q += 64) s_acc[q] = 0.0;
 
     const float thresp = a.thresp;
     const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
     const float thre12 = (float)((double)thresp / ((double)a.maxb34 * dmax));
     const float thre34 = (float)((double)thresp / ((double)a.maxb12 * dmax));
-    const float b12top = a.s12.eb[a0], b34top = a.s34.eb[c0];
-    // Effective extents: partners that can pass the sieve with the global D maximum (the lists are
-    // sorted by bound, so the passing entries are a prefix).
-    int n2 = 0, n4 = 0;
-    for (int k = lane; k < a1 - a0; k += 64) {
-        const float b = a.s12.eb[a0 + k];
-        n2 += (b >= thre12 && b34top >= thre34 && (double)(b * b34top) * dmax >= (double)thresp) ? 1 : 0;
-    }
-    for (int k = lane; k < c1 - c0; k += 64) {
-        const float b = a.s34.eb[c0 + k];
-        n4 += (b >= thre34 && b12top >= thre12 && (double)(b12top * b) * dmax >= (double)thresp) ? 1 : 0;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        n2 += __shfl_xor(n2, o);
-        n4 += __shfl_xor(n4, o);
-    }
-    if (n2 == 0 || n4 == 0) return;
-
+    const int a0 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2];
+    const int a1 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2 + 1];
+    const float b12top = a.s12.eb[a0];
     const double Ax = a.s12.x[r1], Ay = a.s12.y[r1], Az = a.s12.z[r1];
-    const double Cx = a.s34.x[r3], Cy = a.s34.y[r3], Cz = a.s34.z[r3];
-    double acc[N1][N3];
+    // partners of row1 that can pass with the largest possible ket bound are a prefix
+    int n2max = 0;
+    for (int k = lane; k < a1 - a0; k += 64) n2max += (a.s12.eb[a0 + k] >= thre12) ? 1 : 0;
 #pragma unroll
-    for (int i = 0; i < N1; i++)
-#pragma unroll
-        for (int j = 0; j < N3; j++) acc[i][j] = 0.0;
+    for (int o = 32; o > 0; o >>= 1) n2max += __shfl_xor(n2max, o);
+
     unsigned long long nq = 0;
-    const long long ntot = (long long)n2 * n4;
-    for (long long f = lane; f < ntot; f += 64) {
-        const int j2 = (int)(f / n4);
-        const int j4 = (int)(f - (long long)j2 * n4);
-        const int e2 = a0 + j2, e4 = c0 + j4;
-        const float b12 = a.s12.eb[e2], b34 = a.s34.eb[e4];
-        if (!(b12 >= thre12 && b34 >= thre34)) continue;
-        const double dblk = a.Dsh[(size_t)a.s12.eblk[e2] * a.ldDsh + a.s34.eblk[e4]];
-        if ((double)(b12 * b34) * dblk < (double)thresp) continue;
-        nq++;
+    int qn = 0;  // queue length (wave-uniform)
+    auto evaluate = [&](int slot) {
+        const int e2 = s_e2[slot], e4 = s_e4[slot], tt = s_t[slot];
+        const int r3 = a.s34.rows[j3a + tt];
         const int o2 = a.s12.ecart[e2], o4 = a.s34.ecart[e4];
         double Db[N2][N4];
 #pragma unroll
@@ -254,21 +217,96 @@ __global__ __launch_bounds__(256) void k_quartets(KArgs a, long long nunits)
             for (int d = 0; d < N4; d++) Db[b][d] = a.D[(size_t)(o2 + b) * a.ldD + (o4 + d)];
         const KEnt E12 = load_ent(a.s12.eg, a.s12.nent, e2);
         const KEnt E34 = load_ent(a.s34.eg, a.s34.nent, e4);
-        k_quartet<L1, L2, L3, L4>(a.rys, a.ew, Ax, Ay, Az, Cx, Cy, Cz, E12, E34, Db, acc);
-    }
-    // wave reduction and scatter
-    const int o1 = a.s12.cart[r1], o3 = a.s34.cart[r3];
-    const bool mirror = a.symmetric && (r1 != r3);
+        double acc[N1][N3];
 #pragma unroll
-    for (int i = 0; i < N1; i++)
+        for (int i = 0; i < N1; i++)
 #pragma unroll
-        for (int j = 0; j < N3; j++) {
-            const double v = wave_sum(acc[i][j]);
-            if (lane == ((i * N3 + j) & 63) && v != 0.0) {
-                atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
-                if (mirror) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
+            for (int j = 0; j < N3; j++) acc[i][j] = 0.0;
+        k_quartet<L1, L2, L3, L4>(a.rys, a.ew, Ax, Ay, Az, a.s34.x[r3], a.s34.y[r3], a.s34.z[r3], E12, E34, Db, acc);
+#pragma unroll
+        for (int i = 0; i < N1; i++)
+#pragma unroll
+            for (int j = 0; j < N3; j++) atomicAdd(&s_acc[tt * N13 + i * N3 + j], acc[i][j]);
+        nq++;
+    };
+
+    for (int t = j3a; t < j3b && n2max > 0; t++) {
+        const int r3 = a.s34.rows[t];
+        if (a.tri && r3 < r1) continue;
+        const int c0 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4];
+        const int c1 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4 + 1];
+        const float b34top = a.s34.eb[c0];
+        if (!(b34top >= thre34)) continue;
+        int n2 = 0, n4 = 0;
+        for (int k = lane; k < n2max; k += 64)
+            n2 += ((double)(a.s12.eb[a0 + k] * b34top) * dmax >= (double)thresp) ? 1 : 0;
+        for (int k = lane; k < c1 - c0; k += 64) {
+            const float b = a.s34.eb[c0 + k];
+            n4 += (b >= thre34 && (double)(b12top * b) * dmax >= (double)thresp) ? 1 : 0;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            n2 += __shfl_xor(n2, o);
+            n4 += __shfl_xor(n4, o);
+        }
+        const int ntot = n2 * n4;
+        for (int base = 0; base < ntot; base += 64) {
+            const int f = base + lane;
+            bool pass = false;
+            int e2 = 0, e4 = 0;
+            if (f < ntot) {
+                const int j2 = f / n4;
+                e2 = a0 + j2;
+                e4 = c0 + (f - j2 * n4);
+                const float b12 = a.s12.eb[e2], b34 = a.s34.eb[e4];
+                if (b12 >= thre12 && b34 >= thre34) {
+                    const double dblk = a.Dsh[(size_t)a.s12.eblk[e2] * a.ldDsh + a.s34.eblk[e4]];
+                    pass = (double)(b12 * b34) * dblk >= (double)thresp;
+                }
+            }
+            const unsigned long long m = __ballot(pass);
+            if (pass) {
+                const int pos = qn + __popcll(m & ((1ull << lane) - 1ull));
+                s_e2[pos] = e2;
+                s_e4[pos] = e4;
+                s_t[pos] = t - j3a;
+            }
+            qn += __popcll(m);
+            if (qn >= 64) {
+                __syncthreads();
+                evaluate(lane);
+                __syncthreads();
+                const int rest = qn - 64;
+                int m2 = 0, m4 = 0, mt = 0;
+                if (lane < rest) {
+                    m2 = s_e2[64 + lane];
+                    m4 = s_e4[64 + lane];
+                    mt = s_t[64 + lane];
+                }
+                __syncthreads();
+                if (lane < rest) {
+                    s_e2[lane] = m2;
+                    s_e4[lane] = m4;
+                    s_t[lane] = mt;
+                }
+                qn = rest;
             }
         }
+    }
+    __syncthreads();
+    if (lane < qn) evaluate(lane);
+    __syncthreads();
+    // flush the tile accumulator
+    const int o1 = a.s12.cart[r1];
+    for (int q = lane; q < (j3b - j3a) * N13; q += 64) {
+        const double v = s_acc[q];
+        if (v == 0.0) continue;
+        const int tt = q / N13, ij = q - tt * N13, i = ij / N3, j = ij - i * N3;
+        const int r3 = a.s34.rows[j3a + tt];
+        const int o3 = a.s34.cart[r3];
+        atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
+        if (a.symmetric && r1 != r3) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
+    }
     if (a.count) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) nq += __shfl_xor(nq, o);
@@ -315,13 +353,14 @@ void launch_k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2,
                        bL4, nb4, Dsh, sc);
 }
 
-void launch_k_quartets(int L1, int L2, int L3, int L4, const KArgs& a, long long nunits, hipStream_t s)
+void launch_k_quartets(int L1, int L2, int L3, int L4, const KArgs& a, long long ntiles, hipStream_t s)
 {
-    if (nunits <= 0) return;
-    dim3 grid((unsigned)((nunits + 3) / 4)), block(256);
+    if (ntiles <= 0) return;
+    const int nb3 = (a.s34.nrows + kKTile - 1) / kKTile;
+    dim3 grid((unsigned)ntiles), block(64);
 #define CASE(A, B, C, D)                                                                            \
     if (L1 == A && L2 == B && L3 == C && L4 == D) {                                                 \
-        hipLaunchKernelGGL((k_quartets<A, B, C, D>), grid, block, 0, s, a, nunits);                 \
+        hipLaunchKernelGGL((k_tiles<A, B, C, D>), grid, block, 0, s, a, nb3);                       \
         return;                                                                                     \
     }
 #define C4(A, B, C) CASE(A, B, C, 0) CASE(A, B, C, 1) CASE(A, B, C, 2)

@@ -68,8 +68,14 @@ struct PairList {
     int* d_cls_L1 = nullptr;
     int* d_cls_L2 = nullptr;
     // active rows per angular momentum (rows with at least one partner), in primitive order
-    std::vector<std::vector<int>> h_rows_by_L;
-    std::vector<int*> d_rows_by_L;
+    // rows with at least one partner of angular momentum Lp, per (row L, Lp), in primitive order
+    std::vector<int> h_rows[kMaxL + 1][kMaxL + 1];
+    int* d_rows[kMaxL + 1][kMaxL + 1] = {};
+    // triangle enumeration for a symmetric list: units (a in rows[L][La], b in rows[L][Lb], a <= b);
+    // tri_start[i] = first index into rows[L][Lb] with b >= a_i, tri_pre[i] = units before row i
+    std::vector<long long> h_tri_pre[kMaxL + 1][kMaxL + 1][kMaxL + 1];
+    long long* d_tri_pre[kMaxL + 1][kMaxL + 1][kMaxL + 1] = {};
+    int* d_tri_start[kMaxL + 1][kMaxL + 1][kMaxL + 1] = {};
     // cached J task lists keyed by the ket list
     std::map<const PairList*, std::pair<int*, int>> jtasks;
 };

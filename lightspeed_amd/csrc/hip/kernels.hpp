@@ -70,7 +70,8 @@ struct JScatterArgs {
     CallScalars* sc;
 };
 
-// K: one wave per (row1, row3) output primitive pair of fixed angular momenta.
+// K: one wave per tile = (row1, kKTile consecutive row3s) of fixed angular momenta.
+constexpr int kKTile = 16;
 struct KSide {
     const int* row_start;  // [nrow * (maxLp+1) + 1]
     int maxLp;
@@ -79,7 +80,7 @@ struct KSide {
     const int* eblk;       // partner block id
     const double* eg;      // SoA p, Px, Py, Pz, K, Bx, By, Bz (stride nent)
     int nent;
-    const int* rows;       // active rows of this angular momentum (primitive indices)
+    const int* rows;       // rows of this class with partners of the class's partner L (primitive indices)
     int nrows;
     const double *x, *y, *z;  // row basis coordinates
     const int* cart;          // row basis Cartesian offsets
@@ -91,7 +92,9 @@ struct KArgs {
     float thresp;
     int count;
     int symmetric;  // compute only row1 <= row3 and add the transposed block
-    int same_L;     // L1 == L3 (triangular unit enumeration in symmetric mode)
+    int tri;        // units are the (row1 <= row3) pairs given by tri_pre / tri_start
+    const long long* tri_pre;
+    const int* tri_start;
     KSide s12, s34;
     const double* D;     // D24, ncart2 x ncart4
     int ldD;
