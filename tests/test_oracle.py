@@ -34,3 +34,74 @@ def test_mix2_fixtures(oracle_backend, combo):
 def test_mix4_fixtures(oracle_backend, rot):
     for key, val, ref, tol in FC.mix4_cases(oracle_backend, rot):
         assert np.abs(val - ref).max() < tol, key
+
+
+# ---------------------------------------------------------------- closed forms and invariants
+
+import math  # noqa: E402
+
+import lightspeed_amd as ls  # noqa: E402
+from lightspeed_amd import workloads as W  # noqa: E402
+
+
+def _s_basis(centers, exps):
+    P = ls.Primitive
+    prims = [P(1.0, e, x, y, z, 1.0, 0, False, i, i, i, i, i) for i, ((x, y, z), e) in enumerate(zip(centers, exps))]
+    return ls.Basis("s", prims)
+
+
+def _ssss(A, a, B, b, C, c, D, d):
+    """Textbook (ss|ss) for unnormalised primitives: 2 pi^{5/2}/(pq sqrt(p+q)) K_AB K_CD F0(T)."""
+    A, B, C, D = map(np.asarray, (A, B, C, D))
+    p, q = a + b, c + d
+    P, Q = (a * A + b * B) / p, (c * C + d * D) / q
+    Kab = math.exp(-a * b / p * np.sum((A - B) ** 2))
+    Kcd = math.exp(-c * d / q * np.sum((C - D) ** 2))
+    T = p * q / (p + q) * np.sum((P - Q) ** 2)
+    F0 = 1.0 if T < 1e-14 else 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+    return 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * Kab * Kcd * F0
+
+
+def test_ssss_closed_form(oracle_backend):
+    O = oracle_backend.O
+    rng = np.random.default_rng(4)
+    cen = rng.uniform(-2, 2, (4, 3))
+    ex = rng.uniform(0.3, 3.0, 4)
+    bas = O.OBasis(_s_basis(cen, ex))
+    pl = O.OPairList(bas, bas, False, 1e-30)
+    for (i, j, k, l) in [(0, 1, 2, 3), (0, 0, 1, 1), (3, 2, 0, 1), (1, 1, 1, 1)]:
+        D = np.zeros((4, 4))
+        D[k, l] = 1.0
+        J = O.coulomb(FC.COULOMB, pl, pl, D, 1e-30)
+        ref = _ssss(cen[i], ex[i], cen[j], ex[j], cen[k], ex[k], cen[l], ex[l])
+        assert abs(J[i, j] - ref) < 1e-13 * max(1, abs(ref))
+        D = np.zeros((4, 4))
+        D[j, l] = 1.0
+        K = O.exchange(FC.COULOMB, pl, pl, D, False, 1e-30)
+        assert abs(K[i, k] - ref) < 1e-13 * max(1, abs(ref))
+
+
+def test_linearity_symmetry_translation(oracle_backend):
+    O = oracle_backend.O
+    mol = W.water_cluster(3, 6.0, 9)
+    bas = ls.Basis.from_gbs_file(mol, "6-31gss")
+    ob = O.OBasis(bas)
+    pl = O.OPairList(ob, ob, True, 1e-14)
+    rng = np.random.default_rng(1)
+    D1 = rng.standard_normal((bas.nao, bas.nao))
+    D1 = D1 + D1.T
+    D2 = W.random_density(bas.nao, 7, 2)
+    J1, J2 = O.coulomb(FC.COULOMB, pl, pl, D1, 1e-20), O.coulomb(FC.COULOMB, pl, pl, D2, 1e-20)
+    J12 = O.coulomb(FC.COULOMB, pl, pl, D1 + 2 * D2, 1e-20)
+    assert np.abs(J12 - (J1 + 2 * J2)).max() < 1e-11
+    assert np.abs(J1 - J1.T).max() < 1e-12
+    K2 = O.exchange(FC.COULOMB, pl, pl, D2, True, 1e-20)
+    assert np.abs(K2 - K2.T).max() < 1e-12
+    # rigid translation leaves J and K unchanged
+    shift = np.array([1.7, -0.4, 2.9])
+    prims = [ls.Primitive(p.c, p.e, p.x + shift[0], p.y + shift[1], p.z + shift[2], p.c0, p.L, p.is_pure, p.aoIdx,
+                          p.cartIdx, p.primIdx, p.shellIdx, p.atomIdx) for p in bas.primitives]
+    obs = O.OBasis(ls.Basis("shifted", prims))
+    pls = O.OPairList(obs, obs, True, 1e-14)
+    assert np.abs(O.coulomb(FC.COULOMB, pls, pls, D2, 1e-20) - J2).max() < 1e-11
+    assert np.abs(O.exchange(FC.COULOMB, pls, pls, D2, True, 1e-20) - K2).max() < 1e-11
