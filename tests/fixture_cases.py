@@ -9,6 +9,10 @@ and carries its stored result:
   * tests/coverage/mix_basis.py:253-420 (mix_h2o_nh3_ch4_c2h4.npz, 4 fragments, tol 1e-12).
 The data files are copies of the reference's fixtures under tests/golden/reference/.
 
+ref_golden_cases adds J and K of the reference's CPU path built from its own sources
+(oracle/ref/Makefile) on seeded water clusters, written by tests/golden/make_ref_golden.py
+(ref_jk_*.npz: (H2O)10/6-31G, (H2O)4/cc-pVDZ spherical, (H2O)3/6-31G** Cartesian d).
+
 A backend supplies make_basis(mol, name), make_pairlist(b1, b2, symmetric, thre),
 coulomb(ewald, pl12, pl34, D, thresp, J=None) and exchange(ewald, pl12, pl34, D, symm, thresp, K=None).
 """
@@ -148,6 +152,20 @@ def mix4_cases(backend, rot):
 
 MIX2_COMBOS = [(a, b) for a in ["sto-3g", "6-31gs", "cc-pvdz"] for b in ["sto-3g", "6-31gs", "cc-pvdz"] if a != b]
 BASES = ["sto-3g", "6-31gs", "cc-pvdz"]
+
+REF_GOLDEN = ["h2o10_631g", "h2o4_ccpvdz", "h2o3_631gss"]
+
+
+def ref_golden_cases(backend, name, tol):
+    """J and K (D24symm) of one ref_jk_<name>.npz problem, symmetric Schwarz list."""
+    from lightspeed_amd import workloads
+    z = np.load(os.path.join(HERE, "golden", "ref_jk_%s.npz" % name))
+    mol = workloads.water_cluster(int(z["nwater"]), float(z["box"]), int(z["seed"]))
+    b = backend.make_basis(mol, str(z["basis"]))
+    pl = backend.make_pairlist(b, b, True, float(z["threpq"]))
+    thresp = float(z["thresp"])
+    yield name + ":J", backend.coulomb(COULOMB, pl, pl, z["D"], thresp), z["J"], tol
+    yield name + ":K", backend.exchange(COULOMB, pl, pl, z["D"], True, thresp), z["K"], tol
 
 
 class OracleBackend:

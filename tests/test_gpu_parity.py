@@ -50,6 +50,14 @@ def test_mix4_fixtures_gpu(product_backend, rot):
         assert np.abs(val - ref).max() < tol, key
 
 
+@pytest.mark.parametrize("name", FC.REF_GOLDEN)
+def test_reference_build_golden_gpu(product_backend, name):
+    """J/K of the reference's CPU path compiled from its own sources, tolerance 1e-11."""
+    _say("reference-build golden %s" % name)
+    for key, val, ref, tol in FC.ref_golden_cases(product_backend, name, 1.0e-11):
+        assert np.abs(val - ref).max() < tol, key
+
+
 def _water10(ls):
     from lightspeed_amd import workloads as W
     mol, bname, nocc = W.config("h2o10_631g")

@@ -36,6 +36,42 @@ def test_mix4_fixtures(oracle_backend, rot):
         assert np.abs(val - ref).max() < tol, key
 
 
+@pytest.mark.parametrize("name", FC.REF_GOLDEN)
+def test_reference_build_golden(oracle_backend, name):
+    """J/K of the reference's CPU path compiled from its own sources (FP64 both sides)."""
+    for key, val, ref, tol in FC.ref_golden_cases(oracle_backend, name, 1.0e-12):
+        assert np.abs(val - ref).max() < tol, key
+
+
+def test_reference_build_live():
+    """Where the reference build exists (a container with the reference sources), the oracle
+    equals it on a fresh seeded problem with a non-symmetric K call as well."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "ref"))
+    import refjk
+    if not refjk.available():
+        pytest.skip("reference build (oracle/_ref/ref_jk) not present")
+    import oracle as O
+    mol = W.water_cluster(5, 7.5, 123)
+    bas = ls.Basis.from_gbs_file(mol, "6-31gs")
+    D = W.random_density(bas.nao, 25, 9)
+    for symm in (True, False):
+        J, K, _ = refjk.run(bas, D, 1e-13, 1e-12, D24symm=symm, J=symm)
+        ob = O.OBasis(bas)
+        pl = O.OPairList(ob, ob, True, 1e-13)
+        if symm:
+            assert np.abs(O.coulomb(FC.COULOMB, pl, pl, D, 1e-12) - J).max() < 1e-12
+        assert np.abs(O.exchange(FC.COULOMB, pl, pl, D, symm, 1e-12) - K).max() < 1e-12
+    # the bounded sample used for the CPU baseline: every 3rd bra pair of each class
+    J, K, _ = refjk.run(bas, D, 1e-13, 1e-12, stride=3)
+    ob = O.OBasis(bas)
+    pl = O.OPairList(ob, ob, True, 1e-13)
+    sub = pl.subsample(3)
+    assert np.abs(O.coulomb(FC.COULOMB, sub, pl, D, 1e-12) - J).max() < 1e-12
+    assert np.abs(O.exchange(FC.COULOMB, sub, pl, D, True, 1e-12) - K).max() < 1e-12
+
+
 # ---------------------------------------------------------------- closed forms and invariants
 
 import math  # noqa: E402
