@@ -102,6 +102,9 @@ def main():
     ap.add_argument("--workload", default="h2o64_631gss")
     ap.add_argument("--cpu-stride", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="short run for hardware-counter passes: one 256 MiB device copy (the byte-counter "
+                         "calibration kernel, copy_bytes) then one J+K step; prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the J/K path runs on one MI355X); "
@@ -130,6 +133,18 @@ def main():
     dD.upload(Dc)
     dJ = ls.DeviceArray([n, n])
     dK = ls.DeviceArray([n, n])
+
+    if args.pmc_pass:
+        nb = 256 * 1024 * 1024 // 8
+        src = ls.DeviceArray([nb])
+        dst = ls.DeviceArray([nb])
+        dst.copy_from(src)
+        log("pmc pass: copy_bytes of %d bytes done" % (nb * 8))
+        ls.IntBox.coulomb_device(ew, pl, pl, dD.ptr, thresp, thresp, dJ.ptr)
+        ls.IntBox.exchange_device(ew, pl, pl, dD.ptr, True, thresp, thresp, dK.ptr)
+        ls.synchronize()
+        log("pmc pass: one J+K step done")
+        return 0
 
     # quartet counts: reference convention (metric denominator) and per-launch evaluated counts
     NJ, NK = ls.count_quartets(pl, pl, dD.ptr, dD.ptr, True, thresp)

@@ -98,6 +98,9 @@ int lsjk_free(void* ptr);
 int lsjk_memcpy_htod(void* dst, const void* src, size_t bytes);
 int lsjk_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int lsjk_memset_zero(void* dst, size_t bytes);
+/* Device-to-device copy by a plain copy kernel (16-byte loads and stores); also the reference
+ * kernel for calibrating HBM byte counters. */
+int lsjk_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int lsjk_synchronize(void);
 
 /* Instrumentation of the last build call: quartets evaluated per class (1024 counters: J at

@@ -407,13 +407,13 @@ JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
                     tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + (int)chunk, k1)});
             seg.cnt[L12][L34] = (int)tasks.size() - seg.off[L12][L34];
         }
-    auto it = pl12->jtasks.find(pl34);
+    auto it = pl12->jtasks.find(pl34->uid);
     if (it == pl12->jtasks.end()) {
         lsjk::JTask* d = nullptr;
         size_t bytes = std::max<size_t>(1, tasks.size()) * sizeof(lsjk::JTask);
         HIPCHK(hipMalloc((void**)&d, bytes));
         if (!tasks.empty()) HIPCHK(hipMemcpy(d, tasks.data(), tasks.size() * sizeof(lsjk::JTask), hipMemcpyHostToDevice));
-        it = pl12->jtasks.insert({pl34, {(int*)d, (int)tasks.size()}}).first;
+        it = pl12->jtasks.insert({pl34->uid, {(int*)d, (int)tasks.size()}}).first;
     }
     seg.d_tasks = it->second.first;
     return seg;
@@ -665,6 +665,8 @@ int lsjk_pairlist_create(const lsjk_basis* b1, const lsjk_basis* b2, int symmetr
                 throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: symmetric list must have p2 <= p1");
         }
         lsjk_pairlist* pl = new lsjk_pairlist();
+        static unsigned long long next_uid = 0;
+        pl->uid = ++next_uid;
         try {
             pl->b1 = b1;
             pl->b2 = b2;
@@ -1068,6 +1070,16 @@ int lsjk_memset_zero(void* dst, size_t bytes)
     return guarded([&] {
         require_init();
         lsjk::launch_fill(dst, bytes, S.stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(S.stream));
+    });
+}
+int lsjk_memcpy_dtod(void* dst, const void* src, size_t bytes)
+{
+    return guarded([&] {
+        require_init();
+        if (bytes && (!dst || !src)) throw LsjkError(LSJK_E_INVALID, "lsjk_memcpy_dtod: null pointer");
+        lsjk::launch_copy(dst, src, bytes, S.stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(S.stream));
     });

@@ -344,6 +344,25 @@ void launch_fill(void* p, size_t bytes, hipStream_t s)
     hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, s, (unsigned int*)p, n);
 }
 
+// Plain copy, 16 bytes per lane per step (grid-stride); the tail below 16 bytes byte-wise.
+__global__ void copy_bytes(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16, unsigned char* dtail,
+                           const unsigned char* stail, int ntail)
+{
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t i = i0; i < n16; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+    if (i0 < (size_t)ntail) dtail[i0] = stail[i0];
+}
+
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return;
+    const size_t n16 = bytes / 16;
+    const int ntail = (int)(bytes % 16);
+    int blocks = (int)std::max<size_t>(1, std::min<size_t>((n16 + 255) / 256, 8192));
+    hipLaunchKernelGGL(copy_bytes, dim3(blocks), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n16,
+                       (unsigned char*)dst + n16 * 16, (const unsigned char*)src + n16 * 16, ntail);
+}
+
 void launch_k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2, int nb2, const int* bc4,
                        const int* bL4, int nb4, double* Dsh, CallScalars* sc, hipStream_t s)
 {

@@ -76,8 +76,10 @@ struct PairList {
     std::vector<long long> h_tri_pre[kMaxL + 1][kMaxL + 1][kMaxL + 1];
     long long* d_tri_pre[kMaxL + 1][kMaxL + 1][kMaxL + 1] = {};
     int* d_tri_start[kMaxL + 1][kMaxL + 1][kMaxL + 1] = {};
-    // cached J task lists keyed by the ket list
-    std::map<const PairList*, std::pair<int*, int>> jtasks;
+    // process-unique id (never reused, unlike the object's address)
+    unsigned long long uid = 0;
+    // cached J task lists keyed by the ket list's uid
+    std::map<unsigned long long, std::pair<int*, int>> jtasks;
 };
 
 // ---------------------------------------------------------------- per-call argument blocks

@@ -150,5 +150,6 @@ void launch_k_blockmax(const double* D, int ldD, const int* blk_cart2, const int
                        hipStream_t s);
 void launch_k_quartets(int L1, int L2, int L3, int L4, const KArgs& a, long long nunits, hipStream_t s);
 void launch_fill(void* p, size_t bytes, hipStream_t s);
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 }  // namespace lsjk

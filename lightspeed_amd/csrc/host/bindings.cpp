@@ -72,6 +72,11 @@ struct DeviceArray {
         return out;
     }
     void zero() { check(lsjk_memset_zero(p, std::max<size_t>(n, 1) * sizeof(double))); }
+    void copy_from(const DeviceArray& o)
+    {
+        if (o.n != n) throw std::runtime_error("DeviceArray.copy_from: size mismatch");
+        check(lsjk_memcpy_dtod(p, o.p, n * sizeof(double)));
+    }
 };
 
 PYBIND11_MODULE(_lightspeed, m)
@@ -84,7 +89,8 @@ PYBIND11_MODULE(_lightspeed, m)
         .def_property_readonly("ptr", [](const DeviceArray& d) { return (uintptr_t)d.p; })
         .def("upload", &DeviceArray::upload)
         .def("download", &DeviceArray::download)
-        .def("zero", &DeviceArray::zero);
+        .def("zero", &DeviceArray::zero)
+        .def("copy_from", &DeviceArray::copy_from);
 
     py::class_<Tensor, std::shared_ptr<Tensor>>(m, "Tensor", py::buffer_protocol())
         .def(py::init([](std::vector<size_t> shape, std::string name) { return std::make_shared<Tensor>(shape, name); }),
