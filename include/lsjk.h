@@ -103,6 +103,10 @@ int lsjk_memset_zero(void* dst, size_t bytes);
 int lsjk_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int lsjk_synchronize(void);
 
+/* Streams over which one build call spreads its per-class kernel launches (1..4, default 4; 1
+ * runs them one after another on one stream, so each launch's duration is its own). */
+int lsjk_set_streams(int n);
+
 /* Instrumentation of the last build call: quartets evaluated per class (1024 counters: J at
  * [L12*5 + L34], K at [256 + ((L1*5+L2)*5+L3)*5+L4]; enabled by lsjk_set_counting(1)) and the
  * device time of each phase in milliseconds (HIP events on the library stream):

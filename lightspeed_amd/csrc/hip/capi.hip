@@ -4,7 +4,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -56,7 +58,9 @@ struct Session {
     hipStream_t side[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t fork = nullptr;
     hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
-    int nside = 4;  // side streams in use (environment LSJK_STREAMS = 1..4; 1 serialises the launches)
+    int nside = 4;  // side streams in use (lsjk_set_streams / environment LSJK_STREAMS = 1..4; 1 serialises)
+    int num_cu = 1;         // compute units of the device
+    size_t max_lds = 65536; // LDS bytes per workgroup
     bool profiling = false;
     std::vector<hipEvent_t> kev;      // event pool for per-launch timing
     std::vector<int> kid;             // launch ids of the last call (count index)
@@ -81,6 +85,7 @@ void dev_free(T*& p)
     if (p) (void)hipFree((void*)p);
     p = nullptr;
 }
+
 
 void require_init()
 {
@@ -380,7 +385,7 @@ void free_pairlist(lsjk_pairlist* pl)
 // ket pairs (between 64 and 1024, sized so a group pair yields about 2048 tasks), heaviest (largest
 // bounds) first.
 constexpr int kJChunkMax = 1024;
-constexpr int kJTasksTarget = 2048;
+constexpr int kJTasksTarget = 16384;
 
 struct JSegments {
     int* d_tasks = nullptr;
@@ -487,6 +492,28 @@ void finish_call(bool count)
 
 }  // namespace
 
+int lsjk::persistent_grid(const void* kernel, int block, size_t lds, long long nwork_groups)
+{
+    // Dynamic LDS per workgroup stays within 64 KB whatever the device reports: a launch that
+    // requested more did not make progress on this stack.
+    if (lds > std::min(S.max_lds, lsjk::kMaxLdsPerGroup))
+        throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk: kernel needs more LDS per workgroup than allowed");
+    // called with the session lock held
+    static std::map<std::tuple<const void*, int, size_t>, int> cache;
+    const auto key = std::make_tuple(kernel, block, lds);
+    auto it = cache.find(key);
+    int per_cu = 0;
+    if (it == cache.end()) {
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds));
+        cache[key] = per_cu;
+    } else {
+        per_cu = it->second;
+    }
+    if (per_cu <= 0) throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk: kernel cannot be resident at this block size");
+    const long long g = (long long)per_cu * S.num_cu;
+    return (int)std::max<long long>(1, std::min<long long>(g, nwork_groups));
+}
+
 // ====================================================================== C-ABI
 
 extern "C" {
@@ -524,6 +551,10 @@ int lsjk_init(int device)
             int v = std::atoi(env);
             S.nside = std::min(kSide, std::max(1, v));
         }
+        HIPCHK(hipDeviceGetAttribute(&S.num_cu, hipDeviceAttributeMultiprocessorCount, device));
+        int lds = 0;
+        HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        S.max_lds = (size_t)lds;
         std::vector<double> coef(lsjk_rys_coef, lsjk_rys_coef + sizeof(lsjk_rys_coef) / sizeof(double));
         std::vector<double> r2(lsjk_rys_asym_r2, lsjk_rys_asym_r2 + sizeof(lsjk_rys_asym_r2) / sizeof(double));
         std::vector<double> w(lsjk_rys_asym_w, lsjk_rys_asym_w + sizeof(lsjk_rys_asym_w) / sizeof(double));
@@ -781,7 +812,8 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
                     qa.sc = S.d_sc;
                     hipStream_t ss = fo.pick();
                     lt.begin(jcount_index(L12, L34), ss);
-                    launch_j_quartets(L12, L34, qa, (const JTask*)seg.d_tasks + seg.off[L12][L34], seg.cnt[L12][L34], ss);
+                    launch_j_quartets(L12, L34, qa, (const JTask*)seg.d_tasks + seg.off[L12][L34], seg.cnt[L12][L34],
+                                      &S.d_sc->queue[jcount_index(L12, L34)], ss);
                     lt.end();
                 }
             }
@@ -915,7 +947,8 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
             for (auto& k : kls) {
                 hipStream_t ss = fo.pick();
                 lt.begin(kcount_index(k.L1, k.L2, k.L3, k.L4), ss);
-                launch_k_quartets(k.L1, k.L2, k.L3, k.L4, k.ka, k.nunits, ss);
+                launch_k_quartets(k.L1, k.L2, k.L3, k.L4, k.ka, k.nunits, &S.d_sc->queue[kqueue_index(k.L1, k.L2, k.L3, k.L4)],
+                                  ss);
                 lt.end();
             }
             fo.end();
@@ -1089,6 +1122,14 @@ int lsjk_synchronize(void)
     return guarded([&] {
         require_init();
         HIPCHK(hipStreamSynchronize(S.stream));
+    });
+}
+int lsjk_set_streams(int n)
+{
+    return guarded([&] {
+        require_init();
+        if (n < 1 || n > kSide) throw LsjkError(LSJK_E_INVALID, "lsjk_set_streams: n must be 1..4");
+        S.nside = n;
     });
 }
 int lsjk_set_counting(int enable)

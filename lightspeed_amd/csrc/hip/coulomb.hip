@@ -96,8 +96,8 @@ __global__ __launch_bounds__(256) void j_ket_prep(JPrepArgs a, int k0, int nk, i
 // R_{tuv} = pref * sum_i w_i Hx_t Hy_u Hz_v with H_0 = 1, H_1 = U X, H_{n+1} = U (X H_n + n H_{n-1}),
 // U = -2 rho x_i (coulomb.cpp:138-158).
 template <int L>
-__device__ __forceinline__ void hermite_R(const RysTable& tab, const Ewald& ew, double p12, double p34,
-                                          double X, double Y, double Z, double* R)
+__device__ __forceinline__ void hermite_R(const RysTable& tab, const double* s_rys, const Ewald& ew, double p12,
+                                          double p34, double X, double Y, double Z, double* R)
 {
     constexpr int NR = L / 2 + 1;
     constexpr int HL = nherm(L);
@@ -117,7 +117,7 @@ __device__ __forceinline__ void hermite_R(const RysTable& tab, const Ewald& ew, 
         }
         const double T = rho * d2 * r2;
         double x[NR], w[NR];
-        rys_roots<NR>(tab, T, x, w);
+        rys_roots_any<NR>(tab, s_rys, T, x, w);
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const double U = -2.0 * rho * d2 * x[i];
@@ -149,8 +149,9 @@ __device__ __forceinline__ void hermite_R(const RysTable& tab, const Ewald& ew, 
 //   A[uy][tz] = sum_uz Hz[tz+uz] d[ux,uy,uz];  B[ty][tz] = sum_uy Hy[ty+uy] A[uy][tz];
 //   jp[tx,ty,tz] += Hx[tx+ux] B[ty][tz]   (for each ux).
 template <int L12, int L34>
-__device__ __forceinline__ void hermite_contract_factorized(const RysTable& tab, const Ewald& ew, double p12, double p34,
-                                                            double X, double Y, double Z, const double* d, double* jp)
+__device__ __forceinline__ void hermite_contract_factorized(const RysTable& tab, const double* s_rys, const Ewald& ew,
+                                                            double p12, double p34, double X, double Y, double Z,
+                                                            const double* d, double* jp)
 {
     constexpr int L = L12 + L34;
     constexpr int NR = L / 2 + 1;
@@ -167,7 +168,7 @@ __device__ __forceinline__ void hermite_contract_factorized(const RysTable& tab,
             pref *= sqrt(d2);
         }
         double x[NR], w[NR];
-        rys_roots<NR>(tab, rho * d2 * r2, x, w);
+        rys_roots_any<NR>(tab, s_rys, rho * d2 * r2, x, w);
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const double U = -2.0 * rho * d2 * x[i];
@@ -219,95 +220,93 @@ __device__ __forceinline__ void hermite_contract_factorized(const RysTable& tab,
     }
 }
 
-constexpr int kJTile = 64;  // kets staged per LDS round
-
+// Persistent kernel: each wave takes (64 bra pairs x ket range) tasks from a queue until it is
+// empty.  The workgroup's waves share the LDS copy of the Rys table for nrys = (L12+L34)/2 + 1
+// (when rys_staged(nrys)).
+// Lanes own bra pairs; the kets are read at wave-uniform addresses (scalar loads), in bound order.
 template <int L12, int L34>
-__global__ __launch_bounds__(64) void j_quartets(JQuartetArgs a, const JTask* __restrict__ tasks)
+__global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* __restrict__ tasks, int ntask,
+                                                  unsigned int* qctr)
 {
-    constexpr int H12 = nherm(L12), H34 = nherm(L34), L = L12 + L34, HL = nherm(L);
-    __shared__ double s_geom[4][kJTile];
-    __shared__ float s_b[kJTile];
-    __shared__ double s_d[kJTile * H34];
+    constexpr int H12 = nherm(L12), H34 = nherm(L34), L = L12 + L34, HL = nherm(L), NR = L / 2 + 1;
+    extern __shared__ double s_rys[];
+    if constexpr (rys_staged(NR)) {
+        rys_stage_lds<NR>(a.rys, s_rys);
+        __syncthreads();
+    }
 
-    const JTask task = tasks[blockIdx.x];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const float thresp = a.thresp;
     const float max34 = __uint_as_float(a.sc->max34_bits);
     const float thre_bra = thresp / max34;
     const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
-
-    const int bra = task.bra0 + lane;
-    const bool active = lane < task.nbra;
-    float B12 = 0.0f;
-    double p12 = 1.0, Px = 0.0, Py = 0.0, Pz = 0.0;
-    if (active) {
-        B12 = a.bra_b[bra];
-        p12 = a.bra_geom[bra];
-        Px = a.bra_geom[a.bra_n + bra];
-        Py = a.bra_geom[2 * a.bra_n + bra];
-        Pz = a.bra_geom[3 * a.bra_n + bra];
-    }
-    const bool bra_ok = active && (B12 >= thre_bra);
-    // largest bra bound of the tile (sorted, descending): lane 0
-    const float B12max = a.bra_b[task.bra0];
-    if (!(a.bra_b[task.bra0] >= thre_bra)) return;  // no bra of the tile survives truncation
-
-    double jp[H12];
-#pragma unroll
-    for (int t = 0; t < H12; t++) jp[t] = 0.0;
     unsigned long long nq = 0;
 
-    for (int k0 = task.ket0; k0 < task.ket1; k0 += kJTile) {
-        // Exact early exit: B34' <= float(dmax * B34) and the static ket bounds are sorted.
-        const float bk = (float)(dmax * (double)a.ket_static_b[k0]);
-        if (B12max * bk < thresp) break;
-        const int nk = min(kJTile, task.ket1 - k0);
-        __syncthreads();
-        if (lane < nk) {
-            const int k = k0 + lane;
-            s_b[lane] = a.ket_b[k];
-            s_geom[0][lane] = a.ket_geom[k];
-            s_geom[1][lane] = a.ket_geom[a.ket_n + k];
-            s_geom[2][lane] = a.ket_geom[2 * a.ket_n + k];
-            s_geom[3][lane] = a.ket_geom[3 * a.ket_n + k];
+    for (;;) {
+        const int t = (int)wave_take(qctr);
+        if (t >= ntask) break;
+        JTask task;
+        task.bra0 = ld_const(&tasks[t].bra0);
+        task.nbra = ld_const(&tasks[t].nbra);
+        task.ket0 = ld_const(&tasks[t].ket0);
+        task.ket1 = ld_const(&tasks[t].ket1);
+        // largest bra bound of the tile (sorted, descending): its first pair
+        const float B12max = ld_const(a.bra_b + task.bra0);
+        if (!(B12max >= thre_bra)) continue;  // no bra of the tile survives truncation
+        const int bra = task.bra0 + lane;
+        const bool active = lane < task.nbra;
+        float B12 = 0.0f;
+        double p12 = 1.0, Px = 0.0, Py = 0.0, Pz = 0.0;
+        if (active) {
+            B12 = a.bra_b[bra];
+            p12 = a.bra_geom[bra];
+            Px = a.bra_geom[a.bra_n + bra];
+            Py = a.bra_geom[2 * a.bra_n + bra];
+            Pz = a.bra_geom[3 * a.bra_n + bra];
         }
-        const double* src = a.ket_d + (size_t)(k0 - a.ket_hstart) * H34;
-        for (int q = lane; q < nk * H34; q += 64) s_d[q] = src[q];
-        __syncthreads();
-        if (!bra_ok) continue;
-        for (int j = 0; j < nk; j++) {
-            const float B34 = s_b[j];
-            if (B12 * B34 < thresp) continue;
+        const bool bra_ok = active && (B12 >= thre_bra);
+        double jp[H12];
+#pragma unroll
+        for (int u = 0; u < H12; u++) jp[u] = 0.0;
+
+        for (int k = task.ket0; k < task.ket1; k++) {
+            // Exact early exit: B34' <= float(dmax * B34) and the static ket bounds are sorted.
+            const float bk = (float)(dmax * (double)ld_const(a.ket_static_b + k));
+            if (B12max * bk < thresp) break;
+            const float B34 = ld_const(a.ket_b + k);
+            if (!(bra_ok && B12 * B34 >= thresp)) continue;
             nq++;
+            const double q = ld_const(a.ket_geom + k);
+            const double X = Px - ld_const(a.ket_geom + a.ket_n + k);
+            const double Y = Py - ld_const(a.ket_geom + 2 * a.ket_n + k);
+            const double Z = Pz - ld_const(a.ket_geom + 3 * a.ket_n + k);
+            const double* d = a.ket_d + (size_t)(k - a.ket_hstart) * H34;
             if constexpr (L >= 5) {
                 double dreg[H34];
 #pragma unroll
-                for (int u = 0; u < H34; u++) dreg[u] = s_d[j * H34 + u];
-                hermite_contract_factorized<L12, L34>(a.rys, a.ew, p12, s_geom[0][j], Px - s_geom[1][j],
-                                                      Py - s_geom[2][j], Pz - s_geom[3][j], dreg, jp);
+                for (int u = 0; u < H34; u++) dreg[u] = ld_const(d + u);
+                hermite_contract_factorized<L12, L34>(a.rys, s_rys, a.ew, p12, q, X, Y, Z, dreg, jp);
             } else {
                 double R[HL];
-                hermite_R<L>(a.rys, a.ew, p12, s_geom[0][j], Px - s_geom[1][j], Py - s_geom[2][j], Pz - s_geom[3][j],
-                             R);
-                const double* d = s_d + j * H34;
+                hermite_R<L>(a.rys, s_rys, a.ew, p12, q, X, Y, Z, R);
                 static_for<H34>([&](auto U_) {
                     constexpr int u = decltype(U_)::value;
                     constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
-                    const double du = d[u];
+                    const double du = ld_const(d + u);
                     static_for<H12>([&](auto T_) {
-                        constexpr int t = decltype(T_)::value;
-                        constexpr int idx = herm_index(herm_tx(t) + ux, herm_ty(t) + uy, herm_tz(t) + uz);
-                        jp[t] = fma(R[idx], du, jp[t]);
+                        constexpr int tt = decltype(T_)::value;
+                        constexpr int idx = herm_index(herm_tx(tt) + ux, herm_ty(tt) + uy, herm_tz(tt) + uz);
+                        jp[tt] = fma(R[idx], du, jp[tt]);
                     });
                 });
             }
         }
-    }
-    if (bra_ok) {
-        double* dst = a.jp + (size_t)(bra - a.bra_hstart) * H12;
+        if (bra_ok) {
+            double* dst = a.jp + (size_t)(bra - a.bra_hstart) * H12;
 #pragma unroll
-        for (int t = 0; t < H12; t++)
-            if (jp[t] != 0.0) atomic_add(dst + t, jp[t]);
+            for (int u = 0; u < H12; u++)
+                if (jp[u] != 0.0) atomic_add(dst + u, jp[u]);
+        }
     }
     if (a.count) {
         // per-wave reduction of the evaluated-quartet count
@@ -393,12 +392,21 @@ void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk,
 #undef CASE
 }
 
-void launch_j_quartets(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask, hipStream_t s)
+template <int L12, int L34>
+static void launch_jq(const JQuartetArgs& a, const JTask* tasks, int ntask, unsigned int* qctr, hipStream_t s)
+{
+    constexpr int NR = (L12 + L34) / 2 + 1;
+    const size_t lds = rys_staged(NR) ? rys_lds_doubles(a.rys.nbin[NR], NR) * sizeof(double) : 0;
+    const int grid = persistent_grid((const void*)j_quartets<L12, L34>, 256, lds, ((long long)ntask + 3) / 4);
+    hipLaunchKernelGGL((j_quartets<L12, L34>), dim3(grid), dim3(256), lds, s, a, tasks, ntask, qctr);
+}
+
+void launch_j_quartets(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask, unsigned int* qctr,
+                       hipStream_t s)
 {
     if (ntask <= 0) return;
-    dim3 grid(ntask), block(64);
 #define CASE(X, Y) \
-    if (L12 == X && L34 == Y) { hipLaunchKernelGGL((j_quartets<X, Y>), grid, block, 0, s, a, tasks); return; }
+    if (L12 == X && L34 == Y) { launch_jq<X, Y>(a, tasks, ntask, qctr, s); return; }
 #define ROW(X) CASE(X, 0) CASE(X, 1) CASE(X, 2) CASE(X, 3) CASE(X, 4)
     ROW(0) ROW(1) ROW(2) ROW(3) ROW(4)
 #undef ROW

@@ -143,6 +143,95 @@ __device__ __forceinline__ void rys_roots(const RysTable& tab, double T, double*
     }
 }
 
+// Doubles of LDS that hold the table block of one nrys ([bin][root][x|w][12] doubles).
+__host__ __device__ constexpr size_t rys_lds_doubles(int nbin, int nrys) { return (size_t)nbin * nrys * 2 * 12; }
+
+// Kernels stage the table block in LDS for nrys <= 3 (at most 27.6 KB, so several workgroups
+// share a CU); for more roots they read it from global memory (rys_roots).
+__host__ __device__ constexpr bool rys_staged(int nrys) { return nrys <= 3; }
+
+// Upper limit of the dynamic LDS one workgroup of these kernels requests.
+constexpr size_t kMaxLdsPerGroup = 65536;
+
+// Roots from the LDS copy when the kernel staged one, else from the global table.
+template <int NRYS>
+__device__ __forceinline__ void rys_roots_any(const RysTable& tab, const double* s_coef, double T, double* x,
+                                              double* w);
+
+// Cooperative copy of the nrys block of the table into LDS (all threads of the workgroup; the
+// caller synchronises the workgroup afterwards).  The block is a multiple of 24 doubles and
+// starts at a multiple of 24 doubles, so 16-byte copies are aligned.
+template <int NRYS>
+__device__ __forceinline__ void rys_stage_lds(const RysTable& tab, double* s_coef)
+{
+    const int n2 = (int)(rys_lds_doubles(tab.nbin[NRYS], NRYS) / 2);
+    const double2* src = reinterpret_cast<const double2*>(tab.coef + tab.offset[NRYS]);
+    double2* dst = reinterpret_cast<double2*>(s_coef);
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) dst[i] = src[i];
+}
+
+// rys_roots with the coefficients of this nrys read from LDS (staged by rys_stage_lds).
+template <int NRYS>
+__device__ __forceinline__ void rys_roots_lds(const RysTable& tab, const double* s_coef, double T, double* x,
+                                              double* w)
+{
+    constexpr int NC = 12;
+    if (!(T < tab.Tc[NRYS])) {
+        const double iT = 1.0 / T;
+        const double s = sqrt(iT);
+#pragma unroll
+        for (int i = 0; i < NRYS; i++) {
+            x[i] = tab.asym_r2[NRYS * (NRYS - 1) / 2 + i] * iT;
+            w[i] = tab.asym_w[NRYS * (NRYS - 1) / 2 + i] * s;
+        }
+        return;
+    }
+    const double sT = fmax(T, 0.0) * tab.h_inv[NRYS];
+    const int b = min((int)sT, tab.nbin[NRYS] - 1);
+    const double u = 2.0 * (sT - b) - 1.0;
+    const double* c = s_coef + (size_t)b * NRYS * 2 * NC;
+#pragma unroll
+    for (int i = 0; i < NRYS; i++) {
+        const double* cx = c + i * 2 * NC;
+        const double* cw = cx + NC;
+        double px = cx[NC - 1];
+        double pw = cw[NC - 1];
+#pragma unroll
+        for (int k = NC - 2; k >= 0; k--) {
+            px = fma(px, u, cx[k]);
+            pw = fma(pw, u, cw[k]);
+        }
+        x[i] = px;
+        w[i] = pw;
+    }
+}
+
+template <int NRYS>
+__device__ __forceinline__ void rys_roots_any(const RysTable& tab, const double* s_coef, double T, double* x,
+                                              double* w)
+{
+    if constexpr (rys_staged(NRYS))
+        rys_roots_lds<NRYS>(tab, s_coef, T, x, w);
+    else
+        rys_roots<NRYS>(tab, T, x, w);
+}
+
+// Load through the constant address space: a wave-uniform address becomes a scalar (SMEM) load,
+// a divergent one an ordinary vector load.  Only for data no kernel of the launch writes.
+template <typename T>
+__device__ __forceinline__ T ld_const(const T* p)
+{
+    return *(const __attribute__((address_space(4))) T*)(p);
+}
+
+// Next work item of a dynamic queue for the calling wave (lane 0 takes it; all lanes get it).
+__device__ __forceinline__ unsigned int wave_take(unsigned int* ctr)
+{
+    unsigned int t = 0;
+    if ((threadIdx.x & 63) == 0) t = atomicAdd(ctr, 1u);
+    return __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+}
+
 // McMurchie-Davidson Hermite expansion coefficients for one axis:
 // E[t][i][j] for i <= LA, j <= LB, t <= i + j (Helgaker, Jorgensen & Olsen eqs. 9.5.6-9.5.7):
 //   E^{i+1,j}_t = E^{ij}_{t-1}/(2p) + X_PA E^{ij}_t + (t+1) E^{ij}_{t+1}

@@ -96,7 +96,8 @@ __device__ __forceinline__ KEnt load_ent(const double* eg, int n, int i)
 
 // One primitive quartet: acc[a][c] += sum_{b,d} (ab|cd) Db[b][d].
 template <int L1, int L2, int L3, int L4>
-__device__ __forceinline__ void k_quartet(const RysTable& tab, const Ewald& ew, double Ax, double Ay, double Az,
+__device__ __forceinline__ void k_quartet(const RysTable& tab, const double* s_rys, const Ewald& ew, double Ax,
+                                          double Ay, double Az,
                                           double Cx, double Cy, double Cz, const KEnt& e12, const KEnt& e34,
                                           const double (&Db)[ncart(L2)][ncart(L4)],
                                           double (&acc)[ncart(L1)][ncart(L3)])
@@ -125,7 +126,7 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const Ewald& ew, 
             pref *= sqrt(d2);
         }
         double x[NR], w[NR];
-        rys_roots<NR>(tab, rho * d2 * r2, x, w);
+        rys_roots_any<NR>(tab, s_rys, rho * d2 * r2, x, w);
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const double u = x[i] * d2;
@@ -169,143 +170,176 @@ __device__ __forceinline__ double wave_sum(double v)
 
 constexpr int kKQueue = 128;
 
-// One wave per tile = (row1, block of kKTile row3s) of a class (L1 L2 | L3 L4).  For each unit
-// (row1, row3) of the tile the lanes test the sieve over the candidate rectangle (partners that can
-// pass with the global D maximum), compact the passing (2,4) combinations into an LDS queue shared
-// by all units of the tile, and evaluate the quartets 64 at a time; each quartet's N1 x N3 block is
-// added to the tile's LDS accumulator, which is flushed to K once at the end (plus the transposed
-// block in the symmetric mode).
+// Per-wave LDS of k_tiles: the tile accumulator (kKTile x N1 x N3 doubles) and the work queue
+// (3 x kKQueue ints).
+__host__ __device__ constexpr size_t k_wave_lds_bytes(int N13) { return (size_t)kKTile * N13 * 8 + 3 * kKQueue * 4; }
+
+// Persistent kernel over the tiles of one class (L1 L2 | L3 L4).  A tile is (row1, block of
+// kKTile row3s); tiles are numbered row1-major and split into 8 contiguous partitions, one per
+// XCD, so that the waves of one XCD (one L2) work on neighbouring rows of D and K; a wave drains
+// its own partition's queue first and then helps the others.  The workgroup's waves share the
+// LDS copy of the Rys table (when rys_staged(nrys)).
+// For each unit (row1, row3) of a tile the lanes test the sieve over the candidate rectangle
+// (partners that can pass with the global D maximum), compact the passing (2,4) combinations
+// into the wave's LDS queue, and evaluate the quartets 64 at a time; each quartet's N1 x N3 block
+// is added to the tile's LDS accumulator, which is flushed to K once per tile (plus the
+// transposed block in the symmetric mode).
 template <int L1, int L2, int L3, int L4>
-__global__ __launch_bounds__(64) void k_tiles(KArgs a, int nb3)
+__global__ __launch_bounds__(512) void k_tiles(KArgs a, int nb3, long long nunits, unsigned int* qctr)
 {
     constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4), N13 = N1 * N3;
-    __shared__ double s_acc[kKTile * N13];
-    __shared__ int s_e2[kKQueue], s_e4[kKQueue], s_t[kKQueue];
-    const int lane = threadIdx.x;
-    const int i1 = (int)(blockIdx.x / nb3);
-    const int ib = (int)(blockIdx.x - (unsigned)i1 * nb3);
-    const int r1 = a.s12.rows[i1];
-    const int j3a = ib * kKTile;
-    const int j3b = min(j3a + kKTile, a.s34.nrows);
-    if (a.tri && a.s34.rows[j3b - 1] < r1) return;  // rows are in primitive order
-    for (int q = lane; q < kKTile * N13; q += 64) s_acc[q] = 0.0;
+    constexpr int NR = (L1 + L2 + L3 + L4) / 2 + 1;
+    extern __shared__ double s_mem[];
+    const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t ntab = rys_staged(NR) ? rys_lds_doubles(a.rys.nbin[NR], NR) : 0;
+    double* s_rys = s_mem;
+    double* s_acc = s_mem + ntab + (size_t)wave * (kKTile * N13);
+    int* s_e2 = reinterpret_cast<int*>(s_mem + ntab + (size_t)nw * (kKTile * N13)) + wave * 3 * kKQueue;
+    int* s_e4 = s_e2 + kKQueue;
+    int* s_t = s_e4 + kKQueue;
+    if constexpr (rys_staged(NR)) {
+        rys_stage_lds<NR>(a.rys, s_rys);
+        __syncthreads();
+    }
 
     const float thresp = a.thresp;
     const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
     const float thre12 = (float)((double)thresp / ((double)a.maxb34 * dmax));
     const float thre34 = (float)((double)thresp / ((double)a.maxb12 * dmax));
-    const int a0 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2];
-    const int a1 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2 + 1];
-    const float b12top = a.s12.eb[a0];
-    const double Ax = a.s12.x[r1], Ay = a.s12.y[r1], Az = a.s12.z[r1];
-    // partners of row1 that can pass with the largest possible ket bound are a prefix
-    int n2max = 0;
-    for (int k = lane; k < a1 - a0; k += 64) n2max += (a.s12.eb[a0 + k] >= thre12) ? 1 : 0;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) n2max += __shfl_xor(n2max, o);
-
     unsigned long long nq = 0;
-    int qn = 0;  // queue length (wave-uniform)
-    auto evaluate = [&](int slot) {
-        const int e2 = s_e2[slot], e4 = s_e4[slot], tt = s_t[slot];
-        const int r3 = a.s34.rows[j3a + tt];
-        const int o2 = a.s12.ecart[e2], o4 = a.s34.ecart[e4];
-        double Db[N2][N4];
-#pragma unroll
-        for (int b = 0; b < N2; b++)
-#pragma unroll
-            for (int d = 0; d < N4; d++) Db[b][d] = a.D[(size_t)(o2 + b) * a.ldD + (o4 + d)];
-        const KEnt E12 = load_ent(a.s12.eg, a.s12.nent, e2);
-        const KEnt E34 = load_ent(a.s34.eg, a.s34.nent, e4);
-        double acc[N1][N3];
-#pragma unroll
-        for (int i = 0; i < N1; i++)
-#pragma unroll
-            for (int j = 0; j < N3; j++) acc[i][j] = 0.0;
-        k_quartet<L1, L2, L3, L4>(a.rys, a.ew, Ax, Ay, Az, a.s34.x[r3], a.s34.y[r3], a.s34.z[r3], E12, E34, Db, acc);
-#pragma unroll
-        for (int i = 0; i < N1; i++)
-#pragma unroll
-            for (int j = 0; j < N3; j++) atomicAdd(&s_acc[tt * N13 + i * N3 + j], acc[i][j]);
-        nq++;
-    };
+    const int xcd = blockIdx.x & 7;
 
-    for (int t = j3a; t < j3b && n2max > 0; t++) {
-        const int r3 = a.s34.rows[t];
-        if (a.tri && r3 < r1) continue;
-        const int c0 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4];
-        const int c1 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4 + 1];
-        const float b34top = a.s34.eb[c0];
-        if (!(b34top >= thre34)) continue;
-        int n2 = 0, n4 = 0;
-        for (int k = lane; k < n2max; k += 64)
-            n2 += ((double)(a.s12.eb[a0 + k] * b34top) * dmax >= (double)thresp) ? 1 : 0;
-        for (int k = lane; k < c1 - c0; k += 64) {
-            const float b = a.s34.eb[c0 + k];
-            n4 += (b >= thre34 && (double)(b12top * b) * dmax >= (double)thresp) ? 1 : 0;
-        }
+    for (int sweep = 0; sweep < 8; sweep++) {
+        const int part = (xcd + sweep) & 7;
+        const long long lo = nunits * part / 8, hi = nunits * (part + 1) / 8;
+        for (;;) {
+            const long long unit = lo + (long long)wave_take(qctr + part);
+            if (unit >= hi) break;
+            const int i1 = (int)(unit / nb3);
+            const int ib = (int)(unit - (long long)i1 * nb3);
+            const int r1 = a.s12.rows[i1];
+            const int j3a = ib * kKTile;
+            const int j3b = min(j3a + kKTile, a.s34.nrows);
+            if (a.tri && a.s34.rows[j3b - 1] < r1) continue;  // rows are in primitive order
+            for (int q = lane; q < kKTile * N13; q += 64) s_acc[q] = 0.0;
+            __syncwarp();
+
+            const int a0 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2];
+            const int a1 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2 + 1];
+            const float b12top = a.s12.eb[a0];
+            const double Ax = a.s12.x[r1], Ay = a.s12.y[r1], Az = a.s12.z[r1];
+            // partners of row1 that can pass with the largest possible ket bound are a prefix
+            int n2max = 0;
+            for (int k = lane; k < a1 - a0; k += 64) n2max += (a.s12.eb[a0 + k] >= thre12) ? 1 : 0;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            n2 += __shfl_xor(n2, o);
-            n4 += __shfl_xor(n4, o);
+            for (int o = 32; o > 0; o >>= 1) n2max += __shfl_xor(n2max, o);
+
+            int qn = 0;  // queue length (wave-uniform)
+            auto evaluate = [&](int slot) {
+                const int e2 = s_e2[slot], e4 = s_e4[slot], tt = s_t[slot];
+                const int r3 = a.s34.rows[j3a + tt];
+                const int o2 = a.s12.ecart[e2], o4 = a.s34.ecart[e4];
+                double Db[N2][N4];
+#pragma unroll
+                for (int b = 0; b < N2; b++)
+#pragma unroll
+                    for (int d = 0; d < N4; d++) Db[b][d] = a.D[(size_t)(o2 + b) * a.ldD + (o4 + d)];
+                const KEnt E12 = load_ent(a.s12.eg, a.s12.nent, e2);
+                const KEnt E34 = load_ent(a.s34.eg, a.s34.nent, e4);
+                double acc[N1][N3];
+#pragma unroll
+                for (int i = 0; i < N1; i++)
+#pragma unroll
+                    for (int j = 0; j < N3; j++) acc[i][j] = 0.0;
+                k_quartet<L1, L2, L3, L4>(a.rys, s_rys, a.ew, Ax, Ay, Az, a.s34.x[r3], a.s34.y[r3], a.s34.z[r3], E12,
+                                          E34, Db, acc);
+#pragma unroll
+                for (int i = 0; i < N1; i++)
+#pragma unroll
+                    for (int j = 0; j < N3; j++) atomicAdd(&s_acc[tt * N13 + i * N3 + j], acc[i][j]);
+                nq++;
+            };
+
+            for (int t = j3a; t < j3b && n2max > 0; t++) {
+                const int r3 = a.s34.rows[t];
+                if (a.tri && r3 < r1) continue;
+                const int c0 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4];
+                const int c1 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4 + 1];
+                const float b34top = a.s34.eb[c0];
+                if (!(b34top >= thre34)) continue;
+                int n2 = 0, n4 = 0;
+                for (int k = lane; k < n2max; k += 64)
+                    n2 += ((double)(a.s12.eb[a0 + k] * b34top) * dmax >= (double)thresp) ? 1 : 0;
+                for (int k = lane; k < c1 - c0; k += 64) {
+                    const float b = a.s34.eb[c0 + k];
+                    n4 += (b >= thre34 && (double)(b12top * b) * dmax >= (double)thresp) ? 1 : 0;
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    n2 += __shfl_xor(n2, o);
+                    n4 += __shfl_xor(n4, o);
+                }
+                const int ntot = n2 * n4;
+                for (int base = 0; base < ntot; base += 64) {
+                    const int f = base + lane;
+                    bool pass = false;
+                    int e2 = 0, e4 = 0;
+                    if (f < ntot) {
+                        const int j2 = f / n4;
+                        e2 = a0 + j2;
+                        e4 = c0 + (f - j2 * n4);
+                        const float b12 = a.s12.eb[e2], b34 = a.s34.eb[e4];
+                        if (b12 >= thre12 && b34 >= thre34) {
+                            const double dblk = a.Dsh[(size_t)a.s12.eblk[e2] * a.ldDsh + a.s34.eblk[e4]];
+                            pass = (double)(b12 * b34) * dblk >= (double)thresp;
+                        }
+                    }
+                    const unsigned long long m = __ballot(pass);
+                    if (pass) {
+                        const int pos = qn + __popcll(m & ((1ull << lane) - 1ull));
+                        s_e2[pos] = e2;
+                        s_e4[pos] = e4;
+                        s_t[pos] = t - j3a;
+                    }
+                    qn += __popcll(m);
+                    if (qn >= 64) {
+                        __syncwarp();
+                        evaluate(lane);
+                        __syncwarp();
+                        const int rest = qn - 64;
+                        int m2 = 0, m4 = 0, mt = 0;
+                        if (lane < rest) {
+                            m2 = s_e2[64 + lane];
+                            m4 = s_e4[64 + lane];
+                            mt = s_t[64 + lane];
+                        }
+                        __syncwarp();
+                        if (lane < rest) {
+                            s_e2[lane] = m2;
+                            s_e4[lane] = m4;
+                            s_t[lane] = mt;
+                        }
+                        __syncwarp();
+                        qn = rest;
+                    }
+                }
+            }
+            __syncwarp();
+            if (lane < qn) evaluate(lane);
+            __syncwarp();
+            // flush the tile accumulator
+            const int o1 = a.s12.cart[r1];
+            for (int q = lane; q < (j3b - j3a) * N13; q += 64) {
+                const double v = s_acc[q];
+                if (v == 0.0) continue;
+                const int tt = q / N13, ij = q - tt * N13, i = ij / N3, j = ij - i * N3;
+                const int r3 = a.s34.rows[j3a + tt];
+                const int o3 = a.s34.cart[r3];
+                atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
+                if (a.symmetric && r1 != r3) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
+            }
+            __syncwarp();
         }
-        const int ntot = n2 * n4;
-        for (int base = 0; base < ntot; base += 64) {
-            const int f = base + lane;
-            bool pass = false;
-            int e2 = 0, e4 = 0;
-            if (f < ntot) {
-                const int j2 = f / n4;
-                e2 = a0 + j2;
-                e4 = c0 + (f - j2 * n4);
-                const float b12 = a.s12.eb[e2], b34 = a.s34.eb[e4];
-                if (b12 >= thre12 && b34 >= thre34) {
-                    const double dblk = a.Dsh[(size_t)a.s12.eblk[e2] * a.ldDsh + a.s34.eblk[e4]];
-                    pass = (double)(b12 * b34) * dblk >= (double)thresp;
-                }
-            }
-            const unsigned long long m = __ballot(pass);
-            if (pass) {
-                const int pos = qn + __popcll(m & ((1ull << lane) - 1ull));
-                s_e2[pos] = e2;
-                s_e4[pos] = e4;
-                s_t[pos] = t - j3a;
-            }
-            qn += __popcll(m);
-            if (qn >= 64) {
-                __syncthreads();
-                evaluate(lane);
-                __syncthreads();
-                const int rest = qn - 64;
-                int m2 = 0, m4 = 0, mt = 0;
-                if (lane < rest) {
-                    m2 = s_e2[64 + lane];
-                    m4 = s_e4[64 + lane];
-                    mt = s_t[64 + lane];
-                }
-                __syncthreads();
-                if (lane < rest) {
-                    s_e2[lane] = m2;
-                    s_e4[lane] = m4;
-                    s_t[lane] = mt;
-                }
-                qn = rest;
-            }
-        }
-    }
-    __syncthreads();
-    if (lane < qn) evaluate(lane);
-    __syncthreads();
-    // flush the tile accumulator
-    const int o1 = a.s12.cart[r1];
-    for (int q = lane; q < (j3b - j3a) * N13; q += 64) {
-        const double v = s_acc[q];
-        if (v == 0.0) continue;
-        const int tt = q / N13, ij = q - tt * N13, i = ij / N3, j = ij - i * N3;
-        const int r3 = a.s34.rows[j3a + tt];
-        const int o3 = a.s34.cart[r3];
-        atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
-        if (a.symmetric && r1 != r3) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
     }
     if (a.count) {
 #pragma unroll
@@ -372,15 +406,44 @@ void launch_k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2,
                        bL4, nb4, Dsh, sc);
 }
 
-void launch_k_quartets(int L1, int L2, int L3, int L4, const KArgs& a, long long ntiles, hipStream_t s)
+template <int L1, int L2, int L3, int L4>
+static void launch_kt(const KArgs& a, long long nunits, unsigned int* qctr, hipStream_t s)
 {
-    if (ntiles <= 0) return;
+    constexpr int NR = (L1 + L2 + L3 + L4) / 2 + 1, N13 = ncart(L1) * ncart(L3);
     const int nb3 = (a.s34.nrows + kKTile - 1) / kKTile;
-    dim3 grid((unsigned)ntiles), block(64);
-#define CASE(A, B, C, D)                                                                            \
-    if (L1 == A && L2 == B && L3 == C && L4 == D) {                                                 \
-        hipLaunchKernelGGL((k_tiles<A, B, C, D>), grid, block, 0, s, a, nb3);                       \
-        return;                                                                                     \
+    const size_t tab = rys_staged(NR) ? rys_lds_doubles(a.rys.nbin[NR], NR) * sizeof(double) : 0;
+    // waves per workgroup: the count that keeps the most waves resident (the table is shared by
+    // the workgroup's waves; the accumulators and queues are per wave)
+    int best_nw = 4, best_res = -1, best_grid = 1;
+    for (int nw : {8, 4, 2, 1}) {
+        const size_t lds = tab + (size_t)nw * k_wave_lds_bytes(N13);
+        int grid = 0;
+        try {
+            grid = persistent_grid((const void*)k_tiles<L1, L2, L3, L4>, 64 * nw, lds, 1LL << 40);
+        } catch (...) {
+            continue;  // LDS above the per-workgroup limit at this size
+        }
+        if (grid * nw > best_res) {
+            best_res = grid * nw;
+            best_nw = nw;
+            best_grid = grid;
+        }
+    }
+    if (best_res < 0) throw std::runtime_error("lsjk: exchange kernel does not fit the device's LDS");
+    const size_t lds = tab + (size_t)best_nw * k_wave_lds_bytes(N13);
+    const long long need = (nunits + best_nw - 1) / best_nw;
+    const int grid = (int)std::max<long long>(1, std::min<long long>(best_grid, need));
+    hipLaunchKernelGGL((k_tiles<L1, L2, L3, L4>), dim3(grid), dim3(64 * best_nw), lds, s, a, nb3, nunits, qctr);
+}
+
+void launch_k_quartets(int L1, int L2, int L3, int L4, const KArgs& a, long long nunits, unsigned int* qctr,
+                       hipStream_t s)
+{
+    if (nunits <= 0) return;
+#define CASE(A, B, C, D)                                   \
+    if (L1 == A && L2 == B && L3 == C && L4 == D) {        \
+        launch_kt<A, B, C, D>(a, nunits, qctr, s);         \
+        return;                                            \
     }
 #define C4(A, B, C) CASE(A, B, C, 0) CASE(A, B, C, 1) CASE(A, B, C, 2)
 #define C3(A, B) C4(A, B, 0) C4(A, B, 1) C4(A, B, 2)

@@ -96,6 +96,7 @@ struct CallScalars {
     unsigned int pad;
     unsigned long long dmax_bits;  // max |D| (double bits)
     unsigned long long counts[1024]; // evaluated primitive quartets (see jcount_index / kcount_index)
+    unsigned int queue[1024];        // work-queue counters of the persistent kernels
 };
 
 struct JTask {

@@ -143,12 +143,26 @@ void launch_count_k(const CountKArgs& a, hipStream_t s);
 void launch_j_ket_prep(int LA, int LB, const JPrepArgs& a, int k0, int nk, int hstart, size_t hoff, hipStream_t s);
 void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk, int hstart, size_t hoff,
                           hipStream_t s);
-void launch_j_quartets(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask, hipStream_t s);
+void launch_j_quartets(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask, unsigned int* qctr,
+                       hipStream_t s);
+
+// Work-queue counters of one call (CallScalars::queue, zeroed with the call's scalars): J classes
+// at jcount_index, K classes at kqueue_index (8 per class, one per XCD partition).
+__host__ __device__ constexpr int kqueue_index(int L1, int L2, int L3, int L4)
+{
+    return 64 + (((L1 * 3 + L2) * 3 + L3) * 3 + L4) * 8;
+}
+
+// Grid of a persistent kernel: as many workgroups as fit on the device at once (occupancy at
+// this block size and LDS), but no more than the work needs.  Raises if the LDS request exceeds
+// the device's per-workgroup limit.
+int persistent_grid(const void* kernel, int block, size_t lds, long long nwork_groups);
 
 void launch_k_blockmax(const double* D, int ldD, const int* blk_cart2, const int* blk_L2, int nblk2,
                        const int* blk_cart4, const int* blk_L4, int nblk4, double* Dsh, CallScalars* sc,
                        hipStream_t s);
-void launch_k_quartets(int L1, int L2, int L3, int L4, const KArgs& a, long long nunits, hipStream_t s);
+void launch_k_quartets(int L1, int L2, int L3, int L4, const KArgs& a, long long nunits, unsigned int* qctr,
+                       hipStream_t s);
 void launch_fill(void* p, size_t bytes, hipStream_t s);
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 

@@ -245,6 +245,10 @@ PYBIND11_MODULE(_lightspeed, m)
     m.def("hip_finalize", &hip_session_finalize);
     m.def("device_count", &hip_device_count);
     m.def("set_counting", [](bool on) { check(lsjk_set_counting(on ? 1 : 0)); });
+    m.def("set_streams", [](int n) {
+        hip_session_init(0);
+        check(lsjk_set_streams(n));
+    }, py::arg("n"));
     m.def("last_counts", []() {
         py::array_t<uint64_t> a(1024);
         check(lsjk_last_counts((uint64_t*)a.mutable_data()));
