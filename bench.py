@@ -5,14 +5,26 @@ One step = one direct Coulomb build plus one direct exchange build (IntBox::coul
 IntBox::exchange, symmetric Schwarz pair list, D24symm) over an HBM-resident density, on the
 workload the target is quoted on: (H2O)64 random cluster (20 A cube, seed 64), 6-31G** (1600
 functions), Schwarz cutoff 1e-12, thresp = 1e-12, D = C C^T from a seeded random orthonormal C
-(nocc = 320).  Route: plain FP64.
+(nocc = 320).
+
+Two routes are timed, each over `--steps` steps after `--warmup`:
+  * fp64:  thredp = thresp (every quartet FP64);
+  * mixed: thredp = `--thredp` (1e-6): quartets with est >= thredp in FP64, the rest in FP32,
+    accumulation FP64 (the reference's documented mixed precision, intbox.hpp:736-742).
+The mixed route is the headline `value` when its results meet the accuracy bar against the FP64
+route on the same input (max|dJ|, max|dK| <= 1e-7, |dE2| <= 1e-6 with E2 = 2 tr(DJ) - tr(DK));
+otherwise the FP64 route is reported.  Both routes' times and the errors are in `config`.
 
 value = ERI quartets/s = (N_J + N_K) / t_JK with N counted under the reference's screening
-conventions (the plain FP64 route's quartets), t_JK the wall time of one J+K step.
-roofline: the dominant quartet kernel (largest mean launch time over the timed steps, HIP events
-on the library stream): algorithmic FLOPs per launch = sum over its classes of
-N(class) x F(class), F = 40 + nrys (30 + 9 G + 3 Pi) + 2 Pi (Pi = N1 N2 N3 N4,
-G = (L1+L2+1)(L3+L4+1), nrys = (L1+L2+L3+L4)/2 + 1), against the FP64 peak 78.6 TFLOP/s.
+conventions (the plain FP64 route's quartets, for both routes), t_JK the wall time of one step.
+roofline: the dominant quartet kernel of the headline route (largest mean launch time, HIP
+events around each launch on the stream it runs on, measured in `--attribution-steps` extra
+steps after the timed region with the class launches serialised, so each duration is the
+kernel's own): algorithmic FLOPs per launch = sum over its classes of N(class) x F(class),
+F = 40 + nrys (30 + 9 G + 3 Pi) + 2 Pi (Pi = N1 N2 N3 N4, G = (L1+L2+1)(L3+L4+1),
+nrys = (L1+L2+L3+L4)/2 + 1), against the FP64 peak 78.6 TFLOP/s (FP32-route launches: the FP32
+peak 157.3 TFLOP/s).  For a J launch the FP32 share of its classes' quartets is the launch's
+evaluated FP32 fraction.  jk_frac = [sum_FP64 N F / 78.6e12 + sum_FP32 N F / 157.3e12] / t_JK.
 cpu_baseline: the reference's own CPU IntBox::coulombCPU + IntBox::exchangeCPU, compiled from
 its sources into oracle/_ref/ref_jk, timed on a bounded sample of the same workload (the bra list
 keeps every `--cpu-stride`-th pair of each class, the ket list is whole), all threads of the host.
@@ -35,6 +47,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "J+K build time (s) & ERI quartets/s, mixed vs FP64 GPU at |ΔE|≤1e-6 Eh"
 FP64_PEAK = 78.6e12
+FP32_PEAK = 157.3e12
+LO_ID = 4096  # launch ids of FP32-route launches are offset by this
 
 
 def log(msg):
@@ -101,7 +115,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="h2o64_631gss")
     ap.add_argument("--cpu-stride", type=int, default=32)
+    ap.add_argument("--thredp", type=float, default=1.0e-6, help="FP64 threshold of the mixed route")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--attribution-steps", type=int, default=2,
+                    help="extra steps with serialised class launches for the per-kernel roofline")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="short run for hardware-counter passes: one 256 MiB device copy (the byte-counter "
                          "calibration kernel, copy_bytes) then one J+K step; prints no JSON line")
@@ -146,94 +163,148 @@ def main():
         log("pmc pass: one J+K step done")
         return 0
 
-    # quartet counts: reference convention (metric denominator) and per-launch evaluated counts
+    # quartet counts: reference convention (metric denominator, the plain FP64 route's quartets)
     NJ, NK = ls.count_quartets(pl, pl, dD.ptr, dD.ptr, True, thresp)
     NJ = NJ.astype(np.int64)
     NK = NK.astype(np.int64)
-    ls.set_counting(True)
-    ls.IntBox.coulomb_device(ew, pl, pl, dD.ptr, thresp, thresp, dJ.ptr)
-    evJ = ls.last_counts().astype(np.int64)
-    ls.IntBox.exchange_device(ew, pl, pl, dD.ptr, True, thresp, thresp, dK.ptr)
-    evK = ls.last_counts().astype(np.int64)
-    ls.set_counting(False)
+    quartets = int(NJ.sum() + NK.sum())
     log("quartets (reference convention): N_J=%d N_K=%d" % (NJ.sum(), NK.sum()))
 
-    for w in range(args.warmup):
+    def step(thredp):
+        ls.IntBox.coulomb_device(ew, pl, pl, dD.ptr, thresp, thredp, dJ.ptr)
+        ls.IntBox.exchange_device(ew, pl, pl, dD.ptr, True, thresp, thredp, dK.ptr)
+
+    # evaluated counts per route (the mixed route's FP32 share) and results for the accuracy check
+    routes = {"fp64": thresp, "mixed": args.thredp}
+    res_JK, cnt = {}, {}
+    ls.set_counting(True)
+    for name, thredp in routes.items():
         dJ.zero()
         dK.zero()
-        ls.IntBox.coulomb_device(ew, pl, pl, dD.ptr, thresp, thresp, dJ.ptr)
-        ls.IntBox.exchange_device(ew, pl, pl, dD.ptr, True, thresp, thresp, dK.ptr)
+        ls.IntBox.coulomb_device(ew, pl, pl, dD.ptr, thresp, thredp, dJ.ptr)
+        cj, cj32 = ls.last_counts().astype(np.int64), ls.last_counts_fp32().astype(np.int64)
+        ls.IntBox.exchange_device(ew, pl, pl, dD.ptr, True, thresp, thredp, dK.ptr)
+        ck, ck32 = ls.last_counts().astype(np.int64), ls.last_counts_fp32().astype(np.int64)
+        res_JK[name] = (dJ.download(), dK.download())
+        cnt[name] = (cj, cj32, ck, ck32)
+    ls.set_counting(False)
+    J64, K64 = res_JK["fp64"]
+    Jmx, Kmx = res_JK["mixed"]
+    dJmax = float(np.abs(Jmx - J64).max())
+    dKmax = float(np.abs(Kmx - K64).max())
+    # closed-shell two-electron energy with D = C C^T: E2 = 2 tr(D J) - tr(D K)
+    dE = float(abs(2.0 * np.sum(Dc * (Jmx - J64)) - np.sum(Dc * (Kmx - K64))))
+    log("mixed vs fp64: max|dJ|=%.3e max|dK|=%.3e |dE2|=%.3e" % (dJmax, dKmax, dE))
+
+    for w in range(args.warmup):
+        for name, thredp in routes.items():
+            step(thredp)
         log("warmup %d done" % w)
 
+    # timed regions: the library's default launch concurrency (class launches over 4 streams)
+    timing = {}
+    for name, thredp in routes.items():
+        ls.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            dJ.zero()
+            dK.zero()
+            step(thredp)
+        ls.synchronize()
+        timing[name] = (time.perf_counter() - t0) / args.steps
+        log("%s: %d steps, %.3f ms/step" % (name, args.steps, 1000 * timing[name]))
+    head = "mixed" if (dJmax <= 1e-7 and dKmax <= 1e-7 and dE <= 1e-6) else "fp64"
+    t_step = timing[head]
+
+    # kernel attribution: the headline route with the class launches serialised on one stream, so
+    # that each launch's HIP-event duration is its own (under concurrency launches share the device)
+    ls.set_streams(1)
     ls.set_profiling(True)
     launch_ms = {}
-    tJ = tK = 0.0
-    ls.synchronize()
-    t_start = time.perf_counter()
-    for s in range(args.steps):
+    for _ in range(args.attribution_steps):
         dJ.zero()
         dK.zero()
-        a = time.perf_counter()
-        ls.IntBox.coulomb_device(ew, pl, pl, dD.ptr, thresp, thresp, dJ.ptr)
-        b = time.perf_counter()
+        ls.IntBox.coulomb_device(ew, pl, pl, dD.ptr, thresp, routes[head], dJ.ptr)
         for (i, ms) in ls.last_launches():
             launch_ms.setdefault(("J", i), []).append(ms)
-        ls.IntBox.exchange_device(ew, pl, pl, dD.ptr, True, thresp, thresp, dK.ptr)
-        c = time.perf_counter()
+        ls.IntBox.exchange_device(ew, pl, pl, dD.ptr, True, thresp, routes[head], dK.ptr)
         for (i, ms) in ls.last_launches():
             launch_ms.setdefault(("K", i), []).append(ms)
-        tJ += b - a
-        tK += c - b
-    ls.synchronize()
-    t_total = time.perf_counter() - t_start
     ls.set_profiling(False)
-    ms_per_step = 1000.0 * t_total / args.steps
-    log("timed %d steps: %.3f ms/step (J %.3f ms, K %.3f ms)" % (args.steps, ms_per_step, 1000 * tJ / args.steps,
-                                                               1000 * tK / args.steps))
+    ls.set_streams(4)
 
-    # algorithmic FLOPs per launch
-    def launch_flops(kind, idx):
+    # algorithmic FLOPs (frozen convention) per class, split by route
+    def cls_flops(c):
+        return float(flops_per_quartet(*cls_of(c)))
+
+    cj, cj32, ck, ck32 = cnt[head]
+
+    def launch_flops(kind, lid):
+        lo = lid >= LO_ID
+        idx = lid - LO_ID if lo else lid
         if kind == "J":
             L12, L34 = idx // 5, idx % 5
-            f = 0.0
-            for c in range(625):
-                L1, L2, L3, L4 = cls_of(c)
-                if L1 + L2 == L12 and L3 + L4 == L34 and NJ[c]:
-                    f += NJ[c] * flops_per_quartet(L1, L2, L3, L4)
-            return f
-        L1, L2, L3, L4 = cls_of(idx - 256)
-        return evK[idx] * flops_per_quartet(L1, L2, L3, L4)
+            full = sum(NJ[c] * cls_flops(c) for c in range(625)
+                       if sum(cls_of(c)[:2]) == L12 and sum(cls_of(c)[2:]) == L34)
+            f32 = cj32[idx] / cj[idx] if cj[idx] else 0.0
+            return full * (f32 if lo else 1.0 - f32), lo
+        c = idx - 256
+        n = ck32[idx] if lo else ck[idx] - ck32[idx]
+        return n * cls_flops(c), lo
+
+    def launch_name(kind, lid):
+        lo = lid >= LO_ID
+        idx = lid - LO_ID if lo else lid
+        route = ", %s" % ("kRouteLo" if lo else ("kRouteHi" if head == "mixed" else "kRoutePlain"))
+        if kind == "J":
+            return "j_quartets<%d, %d%s>" % (idx // 5, idx % 5, route)
+        return "k_tiles<%d, %d, %d, %d%s>" % (cls_of(idx - 256) + (route,))
 
     means = {k: float(np.mean(v)) for k, v in launch_ms.items()}
     for k in sorted(means, key=lambda k: -means[k])[:24]:
-        fl = launch_flops(*k)
-        name = ("J<%d,%d>" % (k[1] // 5, k[1] % 5)) if k[0] == "J" else ("K<%d,%d,%d,%d>" % cls_of(k[1] - 256))
-        log("  launch %-12s %9.3f ms  %8.3f TFLOP/s  (%.3g quartets)" % (
-            name, means[k], fl / (means[k] * 1e-3) / 1e12 if means[k] > 0 else 0.0,
-            (sum(NJ[c] for c in range(625) if sum(cls_of(c)[:2]) == k[1] // 5 and sum(cls_of(c)[2:]) == k[1] % 5)
-             if k[0] == "J" else evK[k[1]])))
+        fl, lo = launch_flops(*k)
+        log("  launch %-36s %9.3f ms  %8.3f TFLOP/s" % (launch_name(*k), means[k], fl / (means[k] * 1e-3) / 1e12))
     dom = max(means, key=lambda k: means[k])
-    dom_fl = launch_flops(*dom)
+    dom_fl, dom_lo = launch_flops(*dom)
+    peak = FP32_PEAK if dom_lo else FP64_PEAK
     achieved = dom_fl / (means[dom] * 1e-3)
-    total_fl = sum(NJ[c] * flops_per_quartet(*cls_of(c)) + NK[c] * flops_per_quartet(*cls_of(c)) for c in range(625))
-    quartets = int(NJ.sum() + NK.sum())
-    total_fl = float(total_fl)
-    dom_fl = float(dom_fl)
-    dom_name = ("j_quartets<%d,%d>" % (dom[1] // 5, dom[1] % 5)) if dom[0] == "J" else \
-        ("k_quartets<%d,%d,%d,%d>" % cls_of(dom[1] - 256))
-    roofline = {"bound": "mfma", "pipe": "fp64 (vector ALU; the MI355X FP64 peak)", "kernel": dom_name,
-                "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK, "traffic": None,
-                "launch_ms": means[dom], "flops_per_launch": dom_fl,
-                "jk_frac": (total_fl / FP64_PEAK) / (t_total / args.steps)}
+    # whole-step fraction: sum_FP64 N F / 78.6e12 + sum_FP32 N F / 157.3e12, over the step time
+    n32 = {c: 0.0 for c in range(625)}
+    for c in range(625):
+        n32[c] = float(ck32[256 + c])
+    for idx in range(25):
+        if cj[idx]:
+            f32 = cj32[idx] / cj[idx]
+            for c in range(625):
+                L1, L2, L3, L4 = cls_of(c)
+                if (L1 + L2) * 5 + (L3 + L4) == idx:
+                    n32[c] += f32 * NJ[c]
+    t_ideal = 0.0
+    for c in range(625):
+        F = cls_flops(c)
+        tot = float(NJ[c] + NK[c])
+        t_ideal += (tot - n32[c]) * F / FP64_PEAK + n32[c] * F / FP32_PEAK
+    roofline = {"bound": "mfma", "pipe": "%s vector ALU (MI355X peak)" % ("fp32" if dom_lo else "fp64"),
+                "kernel": launch_name(*dom), "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None, "launch_ms": means[dom], "flops_per_launch": float(dom_fl),
+                "launch_timing": "HIP events around each launch on its stream, class launches serialised "
+                                 "(%d steps after the timed region)" % args.attribution_steps,
+                "jk_frac": t_ideal / t_step}
+    fp32_share = float(sum(n32.values()) / quartets) if head == "mixed" else 0.0
 
-    out = {"metric": METRIC, "value": quartets / (t_total / args.steps), "unit": "ERI quartets/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+    out = {"metric": METRIC, "value": quartets / t_step, "unit": "ERI quartets/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000 * t_step, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32" if head == "mixed" else "f64",
+           "data": "synthetic",
            "config": {"workload": args.workload, "basis": bname, "nao": bas.nao, "nprim": bas.nprim,
-                      "npair": pl.npair, "threpq": 1.0e-12, "thresp": thresp, "route": "fp64",
-                      "J_ms": 1000 * tJ / args.steps, "K_ms": 1000 * tK / args.steps,
-                      "N_J": int(NJ.sum()), "N_K": int(NK.sum()), "pairlist_s": t_pl},
+                      "npair": pl.npair, "threpq": 1.0e-12, "thresp": thresp, "route": head,
+                      "thredp": routes[head], "N_J": int(NJ.sum()), "N_K": int(NK.sum()),
+                      "fp32_quartet_share": fp32_share,
+                      "fp64_ms_per_step": 1000 * timing["fp64"], "fp64_quartets_per_s": quartets / timing["fp64"],
+                      "mixed_ms_per_step": 1000 * timing["mixed"],
+                      "mixed_vs_fp64_speedup": timing["fp64"] / timing["mixed"],
+                      "mixed_max_abs_dJ": dJmax, "mixed_max_abs_dK": dKmax, "mixed_abs_dE2": dE,
+                      "pairlist_s": t_pl},
            "roofline": roofline}
     if not args.no_cpu_baseline:
         log("cpu baseline (sample stride %d)" % args.cpu_stride)

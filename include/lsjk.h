@@ -79,7 +79,9 @@ int lsjk_pairlist_destroy(lsjk_pairlist* pl);
  * IntBox::coulomb (src/lightspeed/intbox.hpp:721-771):
  *   J12_dev (ncart1 x ncart2) += sum_34 (12|O|34) D34_dev (ncart3 x ncart4),
  *   O(r) = sum_i scales[i] erf(omegas[i] r)/r  (omegas[i] == -1: 1/r), nterm <= 8.
- * thredp is accepted for the API; plain FP64 evaluation uses thresp (intbox.hpp:736-742). */
+ * Precision (intbox.hpp:736-742): with thredp <= thresp every quartet with est >= thresp is FP64;
+ * with thredp > thresp quartets with est >= thredp are FP64 and those with thresp <= est < thredp
+ * are evaluated in FP32; accumulation is FP64 throughout. */
 int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
                  const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev);
 
@@ -87,7 +89,8 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
  * tc/exchange.cpp:14-75), with the semantics of IntBox::exchange (intbox.hpp:773-828):
  *   K13_dev (ncart1 x ncart3) += sum_24 (12|O|34) D24_dev (ncart2 x ncart4).
  * D24symm: the caller asserts D24 symmetric and bases 1==3, 2==4 (the effective flag of
- * exchange.cpp:73-81); with pl12 == pl34 the library then halves the work. */
+ * exchange.cpp:73-81); with pl12 == pl34 the library then halves the work.  Precision as for
+ * lsjk_coulomb (intbox.hpp:787-793). */
 int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
                   int nterm, const double* scales, const double* omegas, float thresp, float thredp,
                   double* K13_dev);
@@ -113,6 +116,8 @@ int lsjk_set_streams(int n);
  * ms[0] = whole call, ms[1] = quartet kernels only. */
 int lsjk_set_counting(int enable);
 int lsjk_last_counts(uint64_t* counts1024);
+/* The part of lsjk_last_counts evaluated on the FP32 route (mixed precision calls), same layout. */
+int lsjk_last_counts_fp32(uint64_t* counts1024);
 int lsjk_last_timing(double* ms2);
 
 /* Quartet counts under the reference's screening conventions, no integrals evaluated (the

@@ -44,6 +44,8 @@ hip_finalize = _native.hip_finalize
 device_count = _native.device_count
 set_counting = _native.set_counting
 last_counts = _native.last_counts
+last_counts_fp32 = _native.last_counts_fp32
+set_streams = _native.set_streams
 last_timing = _native.last_timing
 synchronize = _native.synchronize
 set_profiling = _native.set_profiling

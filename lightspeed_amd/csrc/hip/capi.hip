@@ -40,6 +40,9 @@ struct LsjkError : public std::runtime_error {
 thread_local std::string g_err;
 std::recursive_mutex g_mu;
 
+// Per-launch ids of FP32-route launches are the class's counter index plus this offset.
+constexpr int kLoRouteId = 4096;
+
 struct Session {
     bool init = false;
     int device = 0;
@@ -47,6 +50,7 @@ struct Session {
     double* d_rys_coef = nullptr;
     double* d_rys_r2 = nullptr;
     double* d_rys_w = nullptr;
+    float* d_rys_coef32 = nullptr;
     lsjk::RysTable rys{};
     lsjk::CallScalars* d_sc = nullptr;
     void* ws = nullptr;
@@ -54,6 +58,7 @@ struct Session {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool counting = false;
     uint64_t counts[1024] = {0};
+    uint64_t counts32[1024] = {0};
     double ms[2] = {0.0, 0.0};
     hipStream_t side[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t fork = nullptr;
@@ -86,6 +91,49 @@ void dev_free(T*& p)
     p = nullptr;
 }
 
+
+// FP32 Rys rule derived from the FP64 table: on each bin, the degree-(kRysNcoef32-1) Chebyshev
+// interpolant (at the Chebyshev points of the first kind) of each FP64 polynomial, converted to
+// monomial coefficients in the bin variable u and rounded to FP32.  Blocks are padded to a
+// multiple of 4 floats so every block starts 16-byte aligned.
+void build_rys_f32(std::vector<float>& out, int* offset32)
+{
+    constexpr int NC = LSJK_RYS_NCOEF, M = lsjk::kRysNcoef32;
+    // monomial coefficients of the Chebyshev polynomials T_0..T_{M-1}
+    double Tm[M][M] = {};
+    Tm[0][0] = 1.0;
+    if (M > 1) Tm[1][1] = 1.0;
+    for (int k = 2; k < M; k++)
+        for (int j = 0; j < M; j++) Tm[k][j] = (j > 0 ? 2.0 * Tm[k - 1][j - 1] : 0.0) - Tm[k - 2][j];
+    double node[M];
+    for (int j = 0; j < M; j++) node[j] = std::cos(M_PI * (j + 0.5) / M);
+    out.clear();
+    offset32[0] = 0;
+    for (int n = 1; n <= 9; n++) {
+        while (out.size() % 4) out.push_back(0.0f);
+        offset32[n] = (int)out.size();
+        for (int b = 0; b < lsjk_rys_nbin[n]; b++)
+            for (int i = 0; i < n; i++)
+                for (int xw = 0; xw < 2; xw++) {
+                    const double* c = lsjk_rys_coef + lsjk_rys_offset[n] + ((size_t)b * n + i) * 2 * NC + xw * NC;
+                    double f[M];
+                    for (int j = 0; j < M; j++) {
+                        double v = c[NC - 1];
+                        for (int k = NC - 2; k >= 0; k--) v = v * node[j] + c[k];
+                        f[j] = v;
+                    }
+                    double mono[M] = {};
+                    for (int k = 0; k < M; k++) {
+                        double ck = 0.0;
+                        for (int j = 0; j < M; j++) ck += f[j] * std::cos(M_PI * k * (j + 0.5) / M);
+                        ck *= (k == 0 ? 1.0 : 2.0) / M;
+                        for (int q = 0; q < M; q++) mono[q] += ck * Tm[k][q];
+                    }
+                    for (int q = 0; q < M; q++) out.push_back((float)mono[q]);
+                }
+    }
+    while (out.size() % 4) out.push_back(0.0f);
+}
 
 void require_init()
 {
@@ -255,7 +303,9 @@ void build_pairlist(lsjk_pairlist* pl)
     pl->nent = ne;
     std::vector<float> eb(ne);
     std::vector<int> eprim(ne), ecart(ne), eblk(ne);
-    std::vector<double> eg(8 * (size_t)std::max(ne, 1));
+    // packed entries (kKEntDoubles per entry, see KEnt): p, K, P, P - A, A - B, {cart, blk}
+    std::vector<double> eg((size_t)kKEntDoubles * (size_t)std::max(ne, 1), 0.0);
+    std::vector<double> eg32((size_t)kKEnt32Doubles * (size_t)std::max(ne, 1), 0.0);
     for (int q = 0; q < ne; q++) {
         const Ent& e = ents[q];
         int r = e.row, s = e.part;
@@ -269,10 +319,27 @@ void build_pairlist(lsjk_pairlist* pl)
         eprim[q] = s;
         ecart[q] = b2->h_cart[s];
         eblk[q] = b2->h_blk[s];
-        eg[q] = p;
-        for (int d = 0; d < 3; d++) eg[(size_t)(1 + d) * ne + q] = (er * b1->h_xyz[3 * r + d] + es * b2->h_xyz[3 * s + d]) * ip;
-        eg[(size_t)4 * ne + q] = b1->h_c[r] * b2->h_c[s] * std::exp(-er * es * ip * R2);
-        for (int d = 0; d < 3; d++) eg[(size_t)(5 + d) * ne + q] = b2->h_xyz[3 * s + d];
+        double* g = eg.data() + (size_t)kKEntDoubles * q;
+        g[0] = p;
+        g[1] = b1->h_c[r] * b2->h_c[s] * std::exp(-er * es * ip * R2);
+        for (int d = 0; d < 3; d++) {
+            const double A = b1->h_xyz[3 * r + d], B = b2->h_xyz[3 * s + d];
+            const double P = (er * A + es * B) * ip;
+            g[2 + d] = P;
+            g[5 + d] = P - A;
+            g[8 + d] = A - B;
+        }
+        const int ids[2] = {ecart[q], eblk[q]};
+        std::memcpy(&g[11], ids, sizeof(ids));
+        // FP32-route copy: P in FP64, the rest rounded to FP32
+        double* h = eg32.data() + (size_t)kKEnt32Doubles * q;
+        h[0] = g[2];
+        h[1] = g[3];
+        h[2] = g[4];
+        const float f[8] = {(float)g[0], (float)g[1], (float)g[5], (float)g[6],
+                            (float)g[7], (float)g[8], (float)g[9], (float)g[10]};
+        std::memcpy(&h[3], f, sizeof(f));
+        std::memcpy(&h[7], ids, sizeof(ids));
     }
     for (int L = 0; L <= kMaxL; L++)
         for (int Lp = 0; Lp <= kMaxL; Lp++) pl->h_rows[L][Lp].clear();
@@ -315,6 +382,7 @@ void build_pairlist(lsjk_pairlist* pl)
     pl->d_ecart = dev_upload(ecart);
     pl->d_eblk = dev_upload(eblk);
     pl->d_egeom = dev_upload(eg);
+    pl->d_egeom32 = dev_upload(eg32);
     for (int L = 0; L <= kMaxL; L++)
         for (int Lp = 0; Lp <= kMaxL; Lp++) {
             pl->d_rows[L][Lp] = dev_upload(pl->h_rows[L][Lp]);
@@ -364,6 +432,7 @@ void free_pairlist(lsjk_pairlist* pl)
     dev_free(pl->d_ecart);
     dev_free(pl->d_eblk);
     dev_free(pl->d_egeom);
+    dev_free(pl->d_egeom32);
     for (int L = 0; L <= lsjk::kMaxL; L++)
         for (int Lp = 0; Lp <= lsjk::kMaxL; Lp++) {
             dev_free(pl->d_rows[L][Lp]);
@@ -485,8 +554,10 @@ void finish_call(bool count)
         lsjk::CallScalars h;
         HIPCHK(hipMemcpy(&h, S.d_sc, sizeof(h), hipMemcpyDeviceToHost));
         std::memcpy(S.counts, h.counts, sizeof(S.counts));
+        std::memcpy(S.counts32, h.counts32, sizeof(S.counts32));
     } else {
         std::memset(S.counts, 0, sizeof(S.counts));
+        std::memset(S.counts32, 0, sizeof(S.counts32));
     }
 }
 
@@ -570,6 +641,12 @@ int lsjk_init(int device)
             S.rys.h_inv[n] = (n > 0) ? 1.0 / lsjk_rys_h[n] : 0.0;
             S.rys.Tc[n] = lsjk_rys_Tc[n];
         }
+        {
+            std::vector<float> c32;
+            build_rys_f32(c32, S.rys.offset32);
+            S.d_rys_coef32 = dev_upload(c32);
+            S.rys.coef32 = S.d_rys_coef32;
+        }
         HIPCHK(hipMalloc((void**)&S.d_sc, sizeof(lsjk::CallScalars)));
         for (auto& e : S.ev) HIPCHK(hipEventCreate(&e));
         S.device = device;
@@ -585,6 +662,7 @@ int lsjk_finalize(void)
         dev_free(S.d_rys_coef);
         dev_free(S.d_rys_r2);
         dev_free(S.d_rys_w);
+        dev_free(S.d_rys_coef32);
         dev_free(S.d_sc);
         if (S.ws) (void)hipFree(S.ws);
         S.ws = nullptr;
@@ -728,11 +806,13 @@ int lsjk_pairlist_destroy(lsjk_pairlist* pl)
 int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
                  const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev)
 {
-    (void)thredp;
     return guarded([&] {
         using namespace lsjk;
         require_init();
         if (!pl12 || !pl34 || !D34_dev || !J12_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_coulomb: null argument");
+        // mixed precision (intbox.hpp:736-742) when thredp > thresp: FP64 at or above thredp,
+        // FP32 between thresp and thredp
+        const bool mixed = thredp > thresp;
         Ewald ew;
         check_ewald(nterm, scales, omegas, ew);
         hipStream_t st = S.stream;
@@ -745,6 +825,8 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
         wsp.add(&kb, (size_t)pl34->npair);
         wsp.add(&kd, pl34->hoff[2 * kMaxL + 1]);
         wsp.add(&jp, pl12->hoff[2 * kMaxL + 1]);
+        float* kd32 = nullptr;
+        if (mixed) wsp.add(&kd32, pl34->hoff[2 * kMaxL + 1]);
         wsp.commit();
         JSegments seg = get_jtasks(pl12, pl34);
 
@@ -768,6 +850,7 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
         pa.thre_ket = thre_ket;
         pa.kb = kb;
         pa.kd = kd;
+        pa.kd32 = kd32;
         pa.sc = S.d_sc;
         for (size_t c = 0; c + 1 < pl34->cls_start.size(); c++) {
             int L = pl34->cls_L1[c] + pl34->cls_L2[c];
@@ -809,12 +892,23 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
                     qa.ket_n = pl34->npair;
                     qa.ket_hstart = pl34->hstart[L34];
                     qa.ket_d = kd + pl34->hoff[L34];
+                    qa.ket_d32 = kd32 ? kd32 + pl34->hoff[L34] : nullptr;
+                    qa.thredp = thredp;
                     qa.sc = S.d_sc;
+                    const JTask* tk = (const JTask*)seg.d_tasks + seg.off[L12][L34];
+                    const int idx = jcount_index(L12, L34);
                     hipStream_t ss = fo.pick();
-                    lt.begin(jcount_index(L12, L34), ss);
-                    launch_j_quartets(L12, L34, qa, (const JTask*)seg.d_tasks + seg.off[L12][L34], seg.cnt[L12][L34],
-                                      &S.d_sc->queue[jcount_index(L12, L34)], ss);
+                    lt.begin(idx, ss);
+                    launch_j_quartets(L12, L34, mixed ? kRouteHi : kRoutePlain, qa, tk, seg.cnt[L12][L34],
+                                      &S.d_sc->queue[idx], ss);
                     lt.end();
+                    if (mixed) {
+                        // the FP32 launch of the class: launch id and queue counter offset by kLoRouteId
+                        hipStream_t sl = fo.pick();
+                        lt.begin(idx + kLoRouteId, sl);
+                        launch_j_quartets(L12, L34, kRouteLo, qa, tk, seg.cnt[L12][L34], &S.d_sc->queue[idx + 32], sl);
+                        lt.end();
+                    }
                 }
             }
             fo.end();
@@ -852,11 +946,12 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
 int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
                   int nterm, const double* scales, const double* omegas, float thresp, float thredp, double* K13_dev)
 {
-    (void)thredp;
     return guarded([&] {
         using namespace lsjk;
         require_init();
         if (!pl12 || !pl34 || !D24_dev || !K13_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_exchange: null argument");
+        // mixed precision when thredp > thresp (intbox.hpp:787-793)
+        const bool mixed = thredp > thresp;
         Ewald ew;
         check_ewald(nterm, scales, omegas, ew);
         hipStream_t st = S.stream;
@@ -902,6 +997,7 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                             ka.rys = S.rys;
                             ka.ew = ew;
                             ka.thresp = thresp;
+                            ka.thredp = thredp;
                             ka.count = S.counting ? 1 : 0;
                             ka.symmetric = symmode ? 1 : 0;
                             ka.tri = tri ? 1 : 0;
@@ -919,6 +1015,7 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                                 s.ecart = pl->d_ecart;
                                 s.eblk = pl->d_eblk;
                                 s.eg = pl->d_egeom;
+                                s.eg32 = pl->d_egeom32;
                                 s.nent = pl->nent;
                                 s.rows = pl->d_rows[Ls[q]][Lps[q]];
                                 s.nrows = (int)pl->h_rows[Ls[q]][Lps[q]].size();
@@ -945,11 +1042,17 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                 return a.nunits > b.nunits;
             });
             for (auto& k : kls) {
+                const int idx = kcount_index(k.L1, k.L2, k.L3, k.L4);
                 hipStream_t ss = fo.pick();
-                lt.begin(kcount_index(k.L1, k.L2, k.L3, k.L4), ss);
-                launch_k_quartets(k.L1, k.L2, k.L3, k.L4, k.ka, k.nunits, &S.d_sc->queue[kqueue_index(k.L1, k.L2, k.L3, k.L4)],
-                                  ss);
+                lt.begin(idx, ss);
+                launch_k_quartets(k.L1, k.L2, k.L3, k.L4, mixed ? kRouteHi : kRoutePlain, k.ka, k.nunits, ss);
                 lt.end();
+                if (mixed) {
+                    hipStream_t sl = fo.pick();
+                    lt.begin(idx + kLoRouteId, sl);
+                    launch_k_quartets(k.L1, k.L2, k.L3, k.L4, kRouteLo, k.ka, k.nunits, sl);
+                    lt.end();
+                }
             }
             fo.end();
             HIPCHK(hipGetLastError());
@@ -1141,6 +1244,13 @@ int lsjk_last_counts(uint64_t* counts)
     return guarded([&] {
         if (!counts) throw LsjkError(LSJK_E_INVALID, "lsjk_last_counts: null pointer");
         std::memcpy(counts, S.counts, sizeof(S.counts));
+    });
+}
+int lsjk_last_counts_fp32(uint64_t* counts)
+{
+    return guarded([&] {
+        if (!counts) throw LsjkError(LSJK_E_INVALID, "lsjk_last_counts_fp32: null pointer");
+        std::memcpy(counts, S.counts32, sizeof(S.counts32));
     });
 }
 int lsjk_last_timing(double* ms)

@@ -57,9 +57,13 @@ __global__ __launch_bounds__(256) void j_ket_prep(JPrepArgs a, int k0, int nk, i
     const bool keep = B2 > a.thre_ket;
     a.kb[pos] = keep ? B2 : -1.0f;
     double* dst = a.kd + hoff + (size_t)(pos - hstart) * H;
+    float* dst32 = a.kd32 ? a.kd32 + hoff + (size_t)(pos - hstart) * H : nullptr;
     if (!keep) {
 #pragma unroll
         for (int t = 0; t < H; t++) dst[t] = 0.0;
+        if (dst32)
+#pragma unroll
+            for (int t = 0; t < H; t++) dst32[t] = 0.0f;
         return;
     }
     atomicMax(&a.sc->max34_bits, __float_as_uint(B2));
@@ -87,156 +91,184 @@ __global__ __launch_bounds__(256) void j_ket_prep(JPrepArgs a, int k0, int nk, i
                     s += Ex.v[tx][ax][bx] * Ey.v[ty][ay][by] * Ez.v[tz][az][bz] * Dt[p][q];
             });
         });
-        dst[t] = (((tx + ty + tz) & 1) ? -Kab : Kab) * s;
+        const double v = (((tx + ty + tz) & 1) ? -Kab : Kab) * s;
+        dst[t] = v;
+        if (dst32) dst32[t] = (float)v;
     });
 }
 
 // ------------------------------------------------------------------ quartets
-// Hermite Coulomb integrals R_{tuv} (t+u+v <= L) for one bra-ket pair by Rys quadrature:
-// R_{tuv} = pref * sum_i w_i Hx_t Hy_u Hz_v with H_0 = 1, H_1 = U X, H_{n+1} = U (X H_n + n H_{n-1}),
-// U = -2 rho x_i (coulomb.cpp:138-158).
-template <int L>
-__device__ __forceinline__ void hermite_R(const RysTable& tab, const double* s_rys, const Ewald& ew, double p12,
-                                          double p34, double X, double Y, double Z, double* R)
+// Arithmetic of the two routes: Real = double (FP64 route) or float (FP32 route).  The FP32 route
+// forms T in FP64 (from FP64 centre differences) and uses the FP32 Rys rule; everything else of
+// the quartet is FP32; the caller accumulates in FP64.
+template <typename Real>
+__device__ __forceinline__ Real rdiv(Real a, Real b)
+{
+    if constexpr (std::is_same<Real, float>::value)
+        return __fdividef(a, b);
+    else
+        return a / b;
+}
+
+template <typename Real, int NR>
+__device__ __forceinline__ void roots_real(const RysTable& tab, const void* s_tab, double T, Real* x, Real* w)
+{
+    if constexpr (std::is_same<Real, float>::value)
+        rys_roots_f32<NR>(tab, static_cast<const float*>(s_tab), T, x, w);
+    else
+        rys_roots_any<NR>(tab, static_cast<const double*>(s_tab), T, x, w);
+}
+
+// Per-root one-dimensional Hermite factors of one bra-ket pair and operator term:
+// H_0 = 1 (x: pref w_i), H_1 = U X, H_{n+1} = U (X H_n + n H_{n-1}), U = -2 rho d^2 x_i
+// (coulomb.cpp:138-158).
+template <typename Real, int L>
+struct HermRoot {
+    Real x[L + 1], y[L + 1], z[L + 1];
+};
+
+template <typename Real, int L>
+__device__ __forceinline__ void herm_factors(Real pref_w, Real U, Real X, Real Y, Real Z, HermRoot<Real, L>& H)
+{
+    H.x[0] = pref_w;
+    H.y[0] = Real(1);
+    H.z[0] = Real(1);
+    if constexpr (L > 0) {
+        H.x[1] = U * X * H.x[0];
+        H.y[1] = U * Y;
+        H.z[1] = U * Z;
+    }
+#pragma unroll
+    for (int u = 1; u < L; u++) {
+        H.x[u + 1] = U * (X * H.x[u] + Real(u) * H.x[u - 1]);
+        H.y[u + 1] = U * (Y * H.y[u] + Real(u) * H.y[u - 1]);
+        H.z[u + 1] = U * (Z * H.z[u] + Real(u) * H.z[u - 1]);
+    }
+}
+
+// Calls f(HermRoot) for every Rys root of every operator term of one bra-ket pair.
+template <typename Real, int L, typename F>
+__device__ __forceinline__ void for_each_root(const RysTable& tab, const void* s_tab, const Ewald& ew, double p12,
+                                              double p34, double X, double Y, double Z, F&& f)
 {
     constexpr int NR = L / 2 + 1;
-    constexpr int HL = nherm(L);
-#pragma unroll
-    for (int t = 0; t < HL; t++) R[t] = 0.0;
-    const double psum = p12 + p34;
-    const double rho = p12 * p34 / psum;
-    const double base = 2.0 * 17.493418327624862 / (p12 * p34 * sqrt(psum));  // 2 pi^{5/2} / (p q sqrt(p+q))
+    const Real p = (Real)p12, q = (Real)p34;
+    const Real psum = p + q;
+    const Real rho = rdiv(p * q, psum);
+    const Real base = rdiv(Real(2.0 * 17.493418327624862), p * q * sqrt(psum));  // 2 pi^{5/2} / (p q sqrt(p+q))
     const double r2 = X * X + Y * Y + Z * Z;
+    const Real Xr = (Real)X, Yr = (Real)Y, Zr = (Real)Z;
     for (int it = 0; it < ew.n; it++) {
         const double om = ew.omega[it];
-        double d2 = 1.0;
-        double pref = ew.alpha[it] * base;
+        Real d2 = Real(1);
+        Real pref = (Real)ew.alpha[it] * base;
         if (om != -1.0) {
-            d2 = om * om / (rho + om * om);
+            const Real o = (Real)om;
+            d2 = rdiv(o * o, rho + o * o);
             pref *= sqrt(d2);
         }
-        const double T = rho * d2 * r2;
-        double x[NR], w[NR];
-        rys_roots_any<NR>(tab, s_rys, T, x, w);
+        const double T = (double)(rho * d2) * r2;
+        Real x[NR], w[NR];
+        roots_real<Real, NR>(tab, s_tab, T, x, w);
 #pragma unroll
         for (int i = 0; i < NR; i++) {
-            const double U = -2.0 * rho * d2 * x[i];
-            double Hx[L + 1], Hy[L + 1], Hz[L + 1];
-            Hx[0] = pref * w[i];
-            Hy[0] = 1.0;
-            Hz[0] = 1.0;
-            if (L > 0) {
-                Hx[1] = U * X * Hx[0];
-                Hy[1] = U * Y;
-                Hz[1] = U * Z;
-            }
-#pragma unroll
-            for (int u = 1; u < L; u++) {
-                Hx[u + 1] = U * (X * Hx[u] + u * Hx[u - 1]);
-                Hy[u + 1] = U * (Y * Hy[u] + u * Hy[u - 1]);
-                Hz[u + 1] = U * (Z * Hz[u] + u * Hz[u - 1]);
-            }
-            static_for<HL>([&](auto T_) {
-                constexpr int t = decltype(T_)::value;
-                R[t] += Hx[herm_tx(t)] * Hy[herm_ty(t)] * Hz[herm_tz(t)];
-            });
+            HermRoot<Real, L> H;
+            herm_factors<Real, L>(pref * w[i], Real(-2) * rho * d2 * x[i], Xr, Yr, Zr, H);
+            f(H);
         }
     }
+}
+
+// Hermite Coulomb integrals R_{tuv} (t+u+v <= L) of one bra-ket pair by Rys quadrature:
+// R_{tuv} = pref * sum_i w_i Hx_t Hy_u Hz_v.
+template <typename Real, int L>
+__device__ __forceinline__ void hermite_R(const RysTable& tab, const void* s_tab, const Ewald& ew, double p12,
+                                          double p34, double X, double Y, double Z, Real (&R)[nherm(L)])
+{
+    constexpr int HL = nherm(L);
+#pragma unroll
+    for (int t = 0; t < HL; t++) R[t] = Real(0);
+    for_each_root<Real, L>(tab, s_tab, ew, p12, p34, X, Y, Z, [&](const HermRoot<Real, L>& H) __attribute__((always_inline)) {
+        static_for<HL>([&](auto T_) {
+            constexpr int t = decltype(T_)::value;
+            constexpr int tx = herm_tx(t), ty = herm_ty(t), tz = herm_tz(t);
+            R[t] += H.x[tx] * H.y[ty] * H.z[tz];
+        });
+    });
 }
 
 // High angular momentum: the same contraction factorised per Rys root, so the (L+1)(L+2)(L+3)/6
 // Hermite integrals are never held at once:
 //   A[uy][tz] = sum_uz Hz[tz+uz] d[ux,uy,uz];  B[ty][tz] = sum_uy Hy[ty+uy] A[uy][tz];
 //   jp[tx,ty,tz] += Hx[tx+ux] B[ty][tz]   (for each ux).
-template <int L12, int L34>
-__device__ __forceinline__ void hermite_contract_factorized(const RysTable& tab, const double* s_rys, const Ewald& ew,
+template <typename Real, int L12, int L34, typename Acc>
+__device__ __forceinline__ void hermite_contract_factorized(const RysTable& tab, const void* s_tab, const Ewald& ew,
                                                             double p12, double p34, double X, double Y, double Z,
-                                                            const double* d, double* jp)
+                                                            const Real (&d)[nherm(L34)], Acc& jp)
 {
     constexpr int L = L12 + L34;
-    constexpr int NR = L / 2 + 1;
-    const double psum = p12 + p34;
-    const double rho = p12 * p34 / psum;
-    const double base = 2.0 * 17.493418327624862 / (p12 * p34 * sqrt(psum));
-    const double r2 = X * X + Y * Y + Z * Z;
-    for (int it = 0; it < ew.n; it++) {
-        const double om = ew.omega[it];
-        double d2 = 1.0;
-        double pref = ew.alpha[it] * base;
-        if (om != -1.0) {
-            d2 = om * om / (rho + om * om);
-            pref *= sqrt(d2);
-        }
-        double x[NR], w[NR];
-        rys_roots_any<NR>(tab, s_rys, rho * d2 * r2, x, w);
-#pragma unroll
-        for (int i = 0; i < NR; i++) {
-            const double U = -2.0 * rho * d2 * x[i];
-            double Hx[L + 1], Hy[L + 1], Hz[L + 1];
-            Hx[0] = pref * w[i];
-            Hy[0] = 1.0;
-            Hz[0] = 1.0;
-            Hx[1] = U * X * Hx[0];
-            Hy[1] = U * Y;
-            Hz[1] = U * Z;
-#pragma unroll
-            for (int u = 1; u < L; u++) {
-                Hx[u + 1] = U * (X * Hx[u] + u * Hx[u - 1]);
-                Hy[u + 1] = U * (Y * Hy[u] + u * Hy[u - 1]);
-                Hz[u + 1] = U * (Z * Hz[u] + u * Hz[u - 1]);
-            }
-            static_for<L34 + 1>([&](auto UX_) {
-                constexpr int ux = decltype(UX_)::value;
-                double A[L34 + 1 - ux][L12 + 1];
-                static_for<L34 + 1 - ux>([&](auto UY_) {
-                    constexpr int uy = decltype(UY_)::value;
-                    static_for<L12 + 1>([&](auto TZ_) {
-                        constexpr int tz = decltype(TZ_)::value;
-                        double s = 0.0;
-                        static_for<L34 + 1 - ux - uy>([&](auto UZ_) {
-                            constexpr int uz = decltype(UZ_)::value;
-                            s = fma(Hz[tz + uz], d[herm_index(ux, uy, uz)], s);
-                        });
-                        A[uy][tz] = s;
+    for_each_root<Real, L>(tab, s_tab, ew, p12, p34, X, Y, Z, [&](const HermRoot<Real, L>& H) __attribute__((always_inline)) {
+        static_for<L34 + 1>([&](auto UX_) {
+            constexpr int ux = decltype(UX_)::value;
+            Real A[L34 + 1 - ux][L12 + 1];
+            static_for<L34 + 1 - ux>([&](auto UY_) {
+                constexpr int uy = decltype(UY_)::value;
+                static_for<L12 + 1>([&](auto TZ_) {
+                    constexpr int tz = decltype(TZ_)::value;
+                    Real s = Real(0);
+                    static_for<L34 + 1 - ux - uy>([&](auto UZ_) {
+                        constexpr int uz = decltype(UZ_)::value;
+                        constexpr int iu = herm_index(ux, uy, uz);
+                        s = fma(H.z[tz + uz], d[iu], s);
                     });
+                    A[uy][tz] = s;
                 });
-                static_for<L12 + 1>([&](auto TY_) {
-                    constexpr int ty = decltype(TY_)::value;
-                    static_for<L12 + 1 - ty>([&](auto TZ_) {
-                        constexpr int tz = decltype(TZ_)::value;
-                        double s = 0.0;
-                        static_for<L34 + 1 - ux>([&](auto UY_) {
-                            constexpr int uy = decltype(UY_)::value;
-                            s = fma(Hy[ty + uy], A[uy][tz], s);
-                        });
-                        static_for<L12 + 1 - ty - tz>([&](auto TX_) {
-                            constexpr int tx = decltype(TX_)::value;
-                            jp[herm_index(tx, ty, tz)] = fma(Hx[tx + ux], s, jp[herm_index(tx, ty, tz)]);
-                        });
+            });
+            static_for<L12 + 1>([&](auto TY_) {
+                constexpr int ty = decltype(TY_)::value;
+                static_for<L12 + 1 - ty>([&](auto TZ_) {
+                    constexpr int tz = decltype(TZ_)::value;
+                    Real s = Real(0);
+                    static_for<L34 + 1 - ux>([&](auto UY_) {
+                        constexpr int uy = decltype(UY_)::value;
+                        s = fma(H.y[ty + uy], A[uy][tz], s);
+                    });
+                    static_for<L12 + 1 - ty - tz>([&](auto TX_) {
+                        constexpr int tx = decltype(TX_)::value;
+                        constexpr int it = herm_index(tx, ty, tz);
+                        jp[it] = fma(H.x[tx + ux], s, jp[it]);
                     });
                 });
             });
-        }
-    }
+        });
+    });
 }
 
 // Persistent kernel: each wave takes (64 bra pairs x ket range) tasks from a queue until it is
-// empty.  The workgroup's waves share the LDS copy of the Rys table for nrys = (L12+L34)/2 + 1
-// (when rys_staged(nrys)).
-// Lanes own bra pairs; the kets are read at wave-uniform addresses (scalar loads), in bound order.
-template <int L12, int L34>
+// empty.  The workgroup's waves share an LDS copy of the Rys table (FP64 rule for nrys <= 3 on the
+// FP64 routes, the FP32 rule on the FP32 route).  Lanes own bra pairs; the kets are read at
+// wave-uniform addresses (scalar loads), in bound order.  ROUTE selects the quartets by
+// est = B12 * B34' (kRoutePlain / kRouteHi / kRouteLo).
+template <int L12, int L34, int ROUTE>
 __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* __restrict__ tasks, int ntask,
                                                   unsigned int* qctr)
 {
     constexpr int H12 = nherm(L12), H34 = nherm(L34), L = L12 + L34, HL = nherm(L), NR = L / 2 + 1;
-    extern __shared__ double s_rys[];
-    if constexpr (rys_staged(NR)) {
-        rys_stage_lds<NR>(a.rys, s_rys);
+    constexpr bool LO = ROUTE == kRouteLo;
+    using Real = typename std::conditional<LO, float, double>::type;
+    extern __shared__ double s_tab[];
+    if constexpr (LO) {
+        rys32_stage_lds<NR>(a.rys, reinterpret_cast<float*>(s_tab));
+        __syncthreads();
+    } else if constexpr (rys_staged(NR)) {
+        rys_stage_lds<NR>(a.rys, s_tab);
         __syncthreads();
     }
 
     const int lane = threadIdx.x & 63;
     const float thresp = a.thresp;
+    // quartets at or above `thre_lo` (and, on the FP32 route, below thredp) are evaluated
+    const float thre_lo = (ROUTE == kRouteHi) ? a.thredp : thresp;
     const float max34 = __uint_as_float(a.sc->max34_bits);
     const float thre_bra = thresp / max34;
     const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
@@ -272,33 +304,52 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
         for (int k = task.ket0; k < task.ket1; k++) {
             // Exact early exit: B34' <= float(dmax * B34) and the static ket bounds are sorted.
             const float bk = (float)(dmax * (double)ld_const(a.ket_static_b + k));
-            if (B12max * bk < thresp) break;
-            const float B34 = ld_const(a.ket_b + k);
-            if (!(bra_ok && B12 * B34 >= thresp)) continue;
+            if (B12max * bk < thre_lo) break;
+            const float est = B12 * ld_const(a.ket_b + k);
+            bool take = bra_ok && est >= thre_lo;
+            if constexpr (LO) take = take && !(est >= a.thredp);
+            if (!take) continue;
             nq++;
             const double q = ld_const(a.ket_geom + k);
             const double X = Px - ld_const(a.ket_geom + a.ket_n + k);
             const double Y = Py - ld_const(a.ket_geom + 2 * a.ket_n + k);
             const double Z = Pz - ld_const(a.ket_geom + 3 * a.ket_n + k);
-            const double* d = a.ket_d + (size_t)(k - a.ket_hstart) * H34;
-            if constexpr (L >= 5) {
-                double dreg[H34];
+            const size_t dofs = (size_t)(k - a.ket_hstart) * H34;
+            Real dreg[H34];
 #pragma unroll
-                for (int u = 0; u < H34; u++) dreg[u] = ld_const(d + u);
-                hermite_contract_factorized<L12, L34>(a.rys, s_rys, a.ew, p12, q, X, Y, Z, dreg, jp);
-            } else {
-                double R[HL];
-                hermite_R<L>(a.rys, s_rys, a.ew, p12, q, X, Y, Z, R);
-                static_for<H34>([&](auto U_) {
-                    constexpr int u = decltype(U_)::value;
-                    constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
-                    const double du = ld_const(d + u);
-                    static_for<H12>([&](auto T_) {
-                        constexpr int tt = decltype(T_)::value;
-                        constexpr int idx = herm_index(herm_tx(tt) + ux, herm_ty(tt) + uy, herm_tz(tt) + uz);
-                        jp[tt] = fma(R[idx], du, jp[tt]);
+            for (int u = 0; u < H34; u++) {
+                if constexpr (LO)
+                    dreg[u] = ld_const(a.ket_d32 + dofs + u);
+                else
+                    dreg[u] = ld_const(a.ket_d + dofs + u);
+            }
+            auto contract = [&](auto& acc) __attribute__((always_inline)) {
+                if constexpr (L >= 5) {
+                    hermite_contract_factorized<Real, L12, L34>(a.rys, s_tab, a.ew, p12, q, X, Y, Z, dreg, acc);
+                } else {
+                    Real R[HL];
+                    hermite_R<Real, L>(a.rys, s_tab, a.ew, p12, q, X, Y, Z, R);
+                    static_for<H34>([&](auto U_) {
+                        constexpr int u = decltype(U_)::value;
+                        constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
+                        static_for<H12>([&](auto T_) {
+                            constexpr int tt = decltype(T_)::value;
+                            constexpr int idx = herm_index(herm_tx(tt) + ux, herm_ty(tt) + uy, herm_tz(tt) + uz);
+                            acc[tt] = fma(R[idx], dreg[u], acc[tt]);
+                        });
                     });
-                });
+                }
+            };
+            if constexpr (LO) {
+                // FP32 quartet, FP64 accumulation
+                float jq[H12];
+#pragma unroll
+                for (int u = 0; u < H12; u++) jq[u] = 0.0f;
+                contract(jq);
+#pragma unroll
+                for (int u = 0; u < H12; u++) jp[u] += (double)jq[u];
+            } else {
+                contract(jp);
             }
         }
         if (bra_ok) {
@@ -311,7 +362,10 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
     if (a.count) {
         // per-wave reduction of the evaluated-quartet count
         for (int o = 32; o > 0; o >>= 1) nq += __shfl_xor(nq, o);
-        if (lane == 0 && nq) atomicAdd(&a.sc->counts[a.count_cls], nq);
+        if (lane == 0 && nq) {
+            atomicAdd(&a.sc->counts[a.count_cls], nq);
+            if constexpr (LO) atomicAdd(&a.sc->counts32[a.count_cls], nq);
+        }
     }
 }
 
@@ -392,21 +446,33 @@ void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk,
 #undef CASE
 }
 
-template <int L12, int L34>
+template <int L12, int L34, int ROUTE>
 static void launch_jq(const JQuartetArgs& a, const JTask* tasks, int ntask, unsigned int* qctr, hipStream_t s)
 {
     constexpr int NR = (L12 + L34) / 2 + 1;
-    const size_t lds = rys_staged(NR) ? rys_lds_doubles(a.rys.nbin[NR], NR) * sizeof(double) : 0;
-    const int grid = persistent_grid((const void*)j_quartets<L12, L34>, 256, lds, ((long long)ntask + 3) / 4);
-    hipLaunchKernelGGL((j_quartets<L12, L34>), dim3(grid), dim3(256), lds, s, a, tasks, ntask, qctr);
+    size_t lds = 0;
+    if (ROUTE == kRouteLo)
+        lds = rys32_lds_floats(a.rys.nbin[NR], NR) * sizeof(float);
+    else if (rys_staged(NR))
+        lds = rys_lds_doubles(a.rys.nbin[NR], NR) * sizeof(double);
+    const int grid = persistent_grid((const void*)j_quartets<L12, L34, ROUTE>, 256, lds, ((long long)ntask + 3) / 4);
+    hipLaunchKernelGGL((j_quartets<L12, L34, ROUTE>), dim3(grid), dim3(256), lds, s, a, tasks, ntask, qctr);
 }
 
-void launch_j_quartets(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask, unsigned int* qctr,
-                       hipStream_t s)
+void launch_j_quartets(int L12, int L34, int route, const JQuartetArgs& a, const JTask* tasks, int ntask,
+                       unsigned int* qctr, hipStream_t s)
 {
     if (ntask <= 0) return;
-#define CASE(X, Y) \
-    if (L12 == X && L34 == Y) { launch_jq<X, Y>(a, tasks, ntask, qctr, s); return; }
+#define CASE(X, Y)                                                                     \
+    if (L12 == X && L34 == Y) {                                                        \
+        if (route == kRouteLo)                                                         \
+            launch_jq<X, Y, kRouteLo>(a, tasks, ntask, qctr, s);                      \
+        else if (route == kRouteHi)                                                    \
+            launch_jq<X, Y, kRouteHi>(a, tasks, ntask, qctr, s);                      \
+        else                                                                           \
+            launch_jq<X, Y, kRoutePlain>(a, tasks, ntask, qctr, s);                   \
+        return;                                                                        \
+    }
 #define ROW(X) CASE(X, 0) CASE(X, 1) CASE(X, 2) CASE(X, 3) CASE(X, 4)
     ROW(0) ROW(1) ROW(2) ROW(3) ROW(4)
 #undef ROW

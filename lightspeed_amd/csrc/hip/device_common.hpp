@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
 #include <utility>
 
 namespace lsjk {
@@ -104,7 +105,15 @@ struct RysTable {
     int nbin[10];
     double h_inv[10];
     double Tc[10];
+    // FP32 rule for the single-precision route: same bins, kRysNcoef32 coefficients per
+    // polynomial, layout [offset32[n] + bin*n*2*kRysNcoef32 + root*2*kRysNcoef32 + {x|w}*kRysNcoef32 + k]
+    const float* coef32;
+    int offset32[10];
 };
+
+// Coefficients per polynomial of the FP32 rule: a degree-5 Chebyshev interpolant of the FP64
+// polynomial on each bin reaches the FP32 rounding floor (1.2e-7 relative to F0).
+constexpr int kRysNcoef32 = 6;
 
 // Evaluate the nrys-root rule: x_i = t_i^2, w_i.  Same arithmetic as the oracle's rys_eval.
 // A non-finite T takes the asymptotic branch (no table access), and the bin is clamped, so the
@@ -214,6 +223,57 @@ __device__ __forceinline__ void rys_roots_any(const RysTable& tab, const double*
         rys_roots_lds<NRYS>(tab, s_coef, T, x, w);
     else
         rys_roots<NRYS>(tab, T, x, w);
+}
+
+// FP32 Rys rule for the single-precision route.  T is formed in FP64 by the caller; for T >= Tc
+// the asymptotic roots are formed in FP64 and rounded; otherwise the FP32 bin polynomial is read
+// from `c32` (the global table or an LDS copy of this nrys block).
+template <int NRYS>
+__device__ __forceinline__ void rys_roots_f32(const RysTable& tab, const float* c32, double T, float* x, float* w)
+{
+    constexpr int NC = kRysNcoef32;
+    if (!(T < tab.Tc[NRYS])) {
+        const double iT = 1.0 / T;
+        const double s = sqrt(iT);
+#pragma unroll
+        for (int i = 0; i < NRYS; i++) {
+            x[i] = (float)(tab.asym_r2[NRYS * (NRYS - 1) / 2 + i] * iT);
+            w[i] = (float)(tab.asym_w[NRYS * (NRYS - 1) / 2 + i] * s);
+        }
+        return;
+    }
+    const double sT = fmax(T, 0.0) * tab.h_inv[NRYS];
+    const int b = min((int)sT, tab.nbin[NRYS] - 1);
+    const float u = (float)(2.0 * (sT - b) - 1.0);
+    const float* c = c32 + (size_t)b * NRYS * 2 * NC;
+#pragma unroll
+    for (int i = 0; i < NRYS; i++) {
+        const float* cx = c + i * 2 * NC;
+        const float* cw = cx + NC;
+        float px = cx[NC - 1];
+        float pw = cw[NC - 1];
+#pragma unroll
+        for (int k = NC - 2; k >= 0; k--) {
+            px = fmaf(px, u, cx[k]);
+            pw = fmaf(pw, u, cw[k]);
+        }
+        x[i] = px;
+        w[i] = pw;
+    }
+}
+
+// Floats of the FP32 table block of one nrys.
+__host__ __device__ constexpr size_t rys32_lds_floats(int nbin, int nrys) { return (size_t)nbin * nrys * 2 * kRysNcoef32; }
+
+// Cooperative copy of the FP32 nrys block into LDS (the block is a multiple of 12 floats and
+// starts at a multiple of 12 floats: 16-byte copies stay aligned when the base is).
+template <int NRYS>
+__device__ __forceinline__ void rys32_stage_lds(const RysTable& tab, float* s_coef)
+{
+    const int n4 = (int)(rys32_lds_floats(tab.nbin[NRYS], NRYS) / 4);
+    const float4* src = reinterpret_cast<const float4*>(tab.coef32 + tab.offset32[NRYS]);
+    float4* dst = reinterpret_cast<float4*>(s_coef);
+    for (int i = threadIdx.x; i < n4; i += blockDim.x) dst[i] = src[i];
 }
 
 // Load through the constant address space: a wave-uniform address becomes a scalar (SMEM) load,

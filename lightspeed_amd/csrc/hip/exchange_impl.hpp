@@ -1,0 +1,431 @@
+// Exchange (K) quartet kernel templates, shared by the translation units that instantiate them
+// (exchange_l0.hip, exchange_l1.hip, exchange_l2.hip: one per bra row angular momentum L1, so the
+// 243 instantiations compile in parallel).
+// Direct exchange (K) build on gfx950.
+//
+// K13 += sum_{2,4} (12|34) D24 over the quartets the reference evaluates
+// (IntBox::exchangeCPU, reference src/cpp/intbox/cpu/exchange.cpp:17-355):
+//   * pairs (1,2) and (3,4) taken from the K-invert partner lists with b >= thresp/(maxb*D24max)
+//     (KBlock::build21 / build_symmetric, src/cpp/core/kvec.cpp:128-232; exchange.cpp:54-71),
+//   * quartet sieve float(b12*b34) * max|D24 block| >= thresp (exchange.cpp:145-152, 186).
+// The reference loops pivot pairs (2,4) and scatters each quartet's (1,3) block with atomics.  Here
+// one wave owns a tile of outputs (one row-1 primitive x 16 row-3 primitives): its lanes test the
+// (2,4) partner combinations of each output pair, compact the survivors into an LDS queue, evaluate
+// them 64 at a time, and add each quartet's N1 x N3 block to the tile's LDS accumulator, which is
+// flushed to K with one atomic add per element.  With a symmetric D24 and one list on both sides
+// only row1 <= row3 is computed and the transposed block is added as well (K = K^T), the same 2x
+// the reference's D24symm half-sieve (exchange.cpp:138-139, 326-336) obtains.
+//
+// Integrals: Rys quadrature with the 2-D integrals built by the vertical recurrence on centres
+// A and C and the horizontal transfer to B and D (Rys, Dupuis & King, J. Comput. Chem. 4, 154
+// (1983)), in registers, for each root x_i = t_i^2 (attenuated as t^2 d^2 for erf operators).
+#pragma once
+#include "internal.hpp"
+#include "kernels.hpp"
+
+namespace lsjk {
+
+template <typename Real, int L1, int L2, int L3, int L4>
+struct Axis4 {
+    Real v[L1 + 1][L2 + 1][L3 + 1][L4 + 1];
+};
+
+// 2-D Rys integrals of one axis and one root: vertical recurrence on centres A and C
+// (G[n][m] = (n 0 | m 0)), then the horizontal transfers to D and B.  All loops are compile-time
+// expansions, so every array index is a constant and the arrays live in registers.
+template <typename Real, int L1, int L2, int L3, int L4>
+__device__ __forceinline__ void rys_axis(Real g00, Real C00, Real D00, Real B10, Real B01, Real B00, Real AB, Real CD,
+                                         Axis4<Real, L1, L2, L3, L4>& I)
+{
+    constexpr int NX = L1 + L2 + 1, MX = L3 + L4 + 1;
+    Real G[NX][MX];
+    G[0][0] = g00;
+    if constexpr (NX > 1) G[1][0] = C00 * g00;
+    static_for<(NX > 2 ? NX - 2 : 0)>([&](auto N_) {
+        constexpr int n = decltype(N_)::value + 1;
+        G[n + 1][0] = C00 * G[n][0] + Real(n) * B10 * G[n - 1][0];
+    });
+    static_for<MX - 1>([&](auto M_) {
+        constexpr int m = decltype(M_)::value;
+        static_for<NX>([&](auto N_) {
+            constexpr int n = decltype(N_)::value;
+            Real v = D00 * G[n][m];
+            if constexpr (m > 0) v += Real(m) * B01 * G[n][m - 1];
+            if constexpr (n > 0) v += Real(n) * B00 * G[n - 1][m];
+            G[n][m + 1] = v;
+        });
+    });
+    // ket transfer: T[n][c][d] = (n 0 | c d)
+    Real T[NX][MX][L4 + 1];
+    static_for<NX>([&](auto N_) {
+        constexpr int n = decltype(N_)::value;
+        static_for<MX>([&](auto C_) {
+            constexpr int c = decltype(C_)::value;
+            T[n][c][0] = G[n][c];
+        });
+    });
+    static_for<L4>([&](auto D_) {
+        constexpr int d = decltype(D_)::value;
+        static_for<NX>([&](auto N_) {
+            constexpr int n = decltype(N_)::value;
+            static_for<MX - 1 - d>([&](auto C_) {
+                constexpr int c = decltype(C_)::value;
+                T[n][c][d + 1] = T[n][c + 1][d] + CD * T[n][c][d];
+            });
+        });
+    });
+    // bra transfer
+    static_for<L3 + 1>([&](auto C_) {
+        constexpr int c = decltype(C_)::value;
+        static_for<L4 + 1>([&](auto D_) {
+            constexpr int d = decltype(D_)::value;
+            Real U[NX][L2 + 1];
+            static_for<NX>([&](auto A_) {
+                constexpr int a = decltype(A_)::value;
+                U[a][0] = T[a][c][d];
+            });
+            static_for<L2>([&](auto B_) {
+                constexpr int b = decltype(B_)::value;
+                static_for<NX - 1 - b>([&](auto A_) {
+                    constexpr int a = decltype(A_)::value;
+                    U[a][b + 1] = U[a + 1][b] + AB * U[a][b];
+                });
+            });
+            static_for<L1 + 1>([&](auto A_) {
+                constexpr int a = decltype(A_)::value;
+                static_for<L2 + 1>([&](auto B_) {
+                    constexpr int b = decltype(B_)::value;
+                    I.v[a][b][c][d] = U[a][b];
+                });
+            });
+        });
+    });
+}
+
+// A partner entry in the arithmetic of a route: the centre P stays FP64 on both routes (P - Q is
+// formed in FP64); the other fields are Real.
+template <typename Real>
+struct KEnt {
+    double Px, Py, Pz;
+    Real p, K, PAx, PAy, PAz, ABx, ABy, ABz;
+    int cart, blk;
+};
+
+// One packed FP64 entry: a 96-byte, 16-byte aligned record (p, K, P, P - A, A - B, {cart, blk}).
+__device__ __forceinline__ KEnt<double> load_ent(const double* __restrict__ eg, int i)
+{
+    const double* g = static_cast<const double*>(__builtin_assume_aligned(eg + (size_t)i * kKEntDoubles, 16));
+    KEnt<double> e;
+    e.p = g[0];
+    e.K = g[1];
+    e.Px = g[2];
+    e.Py = g[3];
+    e.Pz = g[4];
+    e.PAx = g[5];
+    e.PAy = g[6];
+    e.PAz = g[7];
+    e.ABx = g[8];
+    e.ABy = g[9];
+    e.ABz = g[10];
+    const int* ids = reinterpret_cast<const int*>(g + 11);
+    e.cart = ids[0];
+    e.blk = ids[1];
+    return e;
+}
+
+// One packed FP32-route entry: a 64-byte record (P as 3 doubles; p, K, P - A, A - B as 8 floats;
+// {cart, blk}).
+__device__ __forceinline__ KEnt<float> load_ent32(const double* __restrict__ eg32, int i)
+{
+    const double* g = static_cast<const double*>(__builtin_assume_aligned(eg32 + (size_t)i * kKEnt32Doubles, 16));
+    KEnt<float> e;
+    e.Px = g[0];
+    e.Py = g[1];
+    e.Pz = g[2];
+    const float* f = reinterpret_cast<const float*>(g + 3);
+    e.p = f[0];
+    e.K = f[1];
+    e.PAx = f[2];
+    e.PAy = f[3];
+    e.PAz = f[4];
+    e.ABx = f[5];
+    e.ABy = f[6];
+    e.ABz = f[7];
+    const int* ids = reinterpret_cast<const int*>(g + 7);
+    e.cart = ids[0];
+    e.blk = ids[1];
+    return e;
+}
+
+template <typename Real>
+__device__ __forceinline__ Real kdiv(Real a, Real b)
+{
+    if constexpr (std::is_same<Real, float>::value)
+        return __fdividef(a, b);
+    else
+        return a / b;
+}
+
+// One primitive quartet: acc[a][c] += sum_{b,d} (ab|cd) Db[b][d], in the route's arithmetic.
+template <typename Real, int L1, int L2, int L3, int L4>
+__device__ __forceinline__ void k_quartet(const RysTable& tab, const Ewald& ew, const KEnt<Real>& e12,
+                                          const KEnt<Real>& e34, const Real (&Db)[ncart(L2)][ncart(L4)],
+                                          Real (&acc)[ncart(L1)][ncart(L3)])
+{
+    constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4);
+    constexpr int NR = (L1 + L2 + L3 + L4) / 2 + 1;
+    const Real p = e12.p, q = e34.p;
+    const Real pq = p + q;
+    const Real ipq = kdiv(Real(1), pq);
+    const Real rho = p * q * ipq;
+    const double PQxd = e12.Px - e34.Px, PQyd = e12.Py - e34.Py, PQzd = e12.Pz - e34.Pz;
+    const double r2 = PQxd * PQxd + PQyd * PQyd + PQzd * PQzd;
+    const Real PQx = (Real)PQxd, PQy = (Real)PQyd, PQz = (Real)PQzd;
+    const Real base = kdiv(Real(2.0 * 17.493418327624862), p * q * sqrt(pq)) * e12.K * e34.K;
+    const Real PAx = e12.PAx, PAy = e12.PAy, PAz = e12.PAz;
+    const Real QCx = e34.PAx, QCy = e34.PAy, QCz = e34.PAz;
+    const Real ABx = e12.ABx, ABy = e12.ABy, ABz = e12.ABz;
+    const Real CDx = e34.ABx, CDy = e34.ABy, CDz = e34.ABz;
+    const Real qf = q * ipq, pf = p * ipq;  // rho/p, rho/q
+    const Real i2p = kdiv(Real(0.5), p), i2q = kdiv(Real(0.5), q);
+    for (int it = 0; it < ew.n; it++) {
+        const double om = ew.omega[it];
+        Real d2 = Real(1);
+        Real pref = (Real)ew.alpha[it] * base;
+        if (om != -1.0) {
+            const Real o = (Real)om;
+            d2 = kdiv(o * o, rho + o * o);
+            pref *= sqrt(d2);
+        }
+        Real x[NR], w[NR];
+        if constexpr (std::is_same<Real, float>::value)
+            rys_roots_f32<NR>(tab, tab.coef32 + tab.offset32[NR], (double)(rho * d2) * r2, x, w);
+        else
+            rys_roots<NR>(tab, rho * d2 * r2, x, w);
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const Real u = x[i] * d2;
+            const Real B00 = Real(0.5) * u * ipq;
+            const Real B10 = i2p * (Real(1) - u * qf);
+            const Real B01 = i2q * (Real(1) - u * pf);
+            const Real cq = u * qf, cp = u * pf;
+            Axis4<Real, L1, L2, L3, L4> Ix, Iy, Iz;
+            rys_axis<Real, L1, L2, L3, L4>(pref * w[i], PAx - cq * PQx, QCx + cp * PQx, B10, B01, B00, ABx, CDx, Ix);
+            rys_axis<Real, L1, L2, L3, L4>(Real(1), PAy - cq * PQy, QCy + cp * PQy, B10, B01, B00, ABy, CDy, Iy);
+            rys_axis<Real, L1, L2, L3, L4>(Real(1), PAz - cq * PQz, QCz + cp * PQz, B10, B01, B00, ABz, CDz, Iz);
+            static_for<N1>([&](auto A_) {
+                constexpr int a = decltype(A_)::value;
+                constexpr int ax = cart_lx(L1, a), ay = cart_ly(L1, a), az = cart_lz(L1, a);
+                static_for<N3>([&](auto C_) {
+                    constexpr int c = decltype(C_)::value;
+                    constexpr int cx = cart_lx(L3, c), cy = cart_ly(L3, c), cz = cart_lz(L3, c);
+                    Real s = Real(0);
+                    static_for<N2>([&](auto B_) {
+                        constexpr int b = decltype(B_)::value;
+                        constexpr int bx = cart_lx(L2, b), by = cart_ly(L2, b), bz = cart_lz(L2, b);
+                        static_for<N4>([&](auto D_) {
+                            constexpr int d = decltype(D_)::value;
+                            constexpr int dx = cart_lx(L4, d), dy = cart_ly(L4, d), dz = cart_lz(L4, d);
+                            s = fma(Ix.v[ax][bx][cx][dx] * Iy.v[ay][by][cy][dy] * Iz.v[az][bz][cz][dz], Db[b][d], s);
+                        });
+                    });
+                    acc[a][c] += s;
+                });
+            });
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+constexpr int kKQueue = 128;
+
+// One wave per tile = (row1, block of kKTile row3s) of a class (L1 L2 | L3 L4).  For each unit
+// (row1, row3) of the tile the lanes test the sieve over the candidate rectangle (partners that can
+// pass with the global D maximum), compact the passing (2,4) combinations into an LDS queue shared
+// by all units of the tile, and evaluate the quartets 64 at a time; each quartet's N1 x N3 block is
+// added to the tile's LDS accumulator, which is flushed to K once at the end (plus the transposed
+// block in the symmetric mode).
+template <int L1, int L2, int L3, int L4, int ROUTE>
+__global__ __launch_bounds__(64) void k_tiles(KArgs a, int nb3)
+{
+    constexpr bool LO = ROUTE == kRouteLo;
+    using Real = typename std::conditional<LO, float, double>::type;
+    constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4), N13 = N1 * N3;
+    __shared__ double s_acc[kKTile * N13];
+    __shared__ int s_e2[kKQueue], s_e4[kKQueue], s_t[kKQueue];
+    const int lane = threadIdx.x;
+    const int i1 = (int)(blockIdx.x / nb3);
+    const int ib = (int)(blockIdx.x - (unsigned)i1 * nb3);
+    const int r1 = a.s12.rows[i1];
+    const int j3a = ib * kKTile;
+    const int j3b = min(j3a + kKTile, a.s34.nrows);
+    if (a.tri && a.s34.rows[j3b - 1] < r1) return;  // rows are in primitive order
+    for (int q = lane; q < kKTile * N13; q += 64) s_acc[q] = 0.0;
+
+    // quartets with est at or above `thresp` here (and, on the FP32 route, below thredp) are evaluated
+    const float thresp = (ROUTE == kRouteHi) ? a.thredp : a.thresp;
+    const double thredp = (double)a.thredp;
+    const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
+    const float thre12 = (float)((double)thresp / ((double)a.maxb34 * dmax));
+    const float thre34 = (float)((double)thresp / ((double)a.maxb12 * dmax));
+    const int a0 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2];
+    const int a1 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2 + 1];
+    const float b12top = a.s12.eb[a0];
+    // partners of row1 that can pass with the largest possible ket bound are a prefix
+    int n2max = 0;
+    for (int k = lane; k < a1 - a0; k += 64) n2max += (a.s12.eb[a0 + k] >= thre12) ? 1 : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) n2max += __shfl_xor(n2max, o);
+
+    unsigned long long nq = 0;
+    int qn = 0;  // queue length (wave-uniform)
+    // inlined at both call sites: an outlined copy would take the address of the kernel arguments
+    auto evaluate = [&](int slot) __attribute__((always_inline)) {
+        const int e2 = s_e2[slot], e4 = s_e4[slot], tt = s_t[slot];
+        KEnt<Real> E12, E34;
+        if constexpr (LO) {
+            E12 = load_ent32(a.s12.eg32, e2);
+            E34 = load_ent32(a.s34.eg32, e4);
+        } else {
+            E12 = load_ent(a.s12.eg, e2);
+            E34 = load_ent(a.s34.eg, e4);
+        }
+        Real Db[N2][N4];
+#pragma unroll
+        for (int b = 0; b < N2; b++)
+#pragma unroll
+            for (int d = 0; d < N4; d++) Db[b][d] = (Real)a.D[(size_t)(E12.cart + b) * a.ldD + (E34.cart + d)];
+        Real acc[N1][N3];
+#pragma unroll
+        for (int i = 0; i < N1; i++)
+#pragma unroll
+            for (int j = 0; j < N3; j++) acc[i][j] = Real(0);
+        k_quartet<Real, L1, L2, L3, L4>(a.rys, a.ew, E12, E34, Db, acc);
+        // FP64 accumulation on both routes
+#pragma unroll
+        for (int i = 0; i < N1; i++)
+#pragma unroll
+            for (int j = 0; j < N3; j++) atomicAdd(&s_acc[tt * N13 + i * N3 + j], (double)acc[i][j]);
+        nq++;
+    };
+
+    for (int t = j3a; t < j3b && n2max > 0; t++) {
+        const int r3 = a.s34.rows[t];
+        if (a.tri && r3 < r1) continue;
+        const int c0 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4];
+        const int c1 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4 + 1];
+        const float b34top = a.s34.eb[c0];
+        if (!(b34top >= thre34)) continue;
+        int n2 = 0, n4 = 0;
+        for (int k = lane; k < n2max; k += 64)
+            n2 += ((double)(a.s12.eb[a0 + k] * b34top) * dmax >= (double)thresp) ? 1 : 0;
+        for (int k = lane; k < c1 - c0; k += 64) {
+            const float b = a.s34.eb[c0 + k];
+            n4 += (b >= thre34 && (double)(b12top * b) * dmax >= (double)thresp) ? 1 : 0;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            n2 += __shfl_xor(n2, o);
+            n4 += __shfl_xor(n4, o);
+        }
+        const int ntot = n2 * n4;
+        for (int base = 0; base < ntot; base += 64) {
+            const int f = base + lane;
+            bool pass = false;
+            int e2 = 0, e4 = 0;
+            if (f < ntot) {
+                const int j2 = f / n4;
+                e2 = a0 + j2;
+                e4 = c0 + (f - j2 * n4);
+                const float b12 = a.s12.eb[e2], b34 = a.s34.eb[e4];
+                if (b12 >= thre12 && b34 >= thre34) {
+                    const double dblk = a.Dsh[(size_t)a.s12.eblk[e2] * a.ldDsh + a.s34.eblk[e4]];
+                    const double est = (double)(b12 * b34) * dblk;
+                    pass = est >= (double)thresp;
+                    if constexpr (LO) pass = pass && !(est >= thredp);
+                }
+            }
+            const unsigned long long m = __ballot(pass);
+            if (pass) {
+                const int pos = qn + __popcll(m & ((1ull << lane) - 1ull));
+                s_e2[pos] = e2;
+                s_e4[pos] = e4;
+                s_t[pos] = t - j3a;
+            }
+            qn += __popcll(m);
+            if (qn >= 64) {
+                __syncthreads();
+                evaluate(lane);
+                __syncthreads();
+                const int rest = qn - 64;
+                int m2 = 0, m4 = 0, mt = 0;
+                if (lane < rest) {
+                    m2 = s_e2[64 + lane];
+                    m4 = s_e4[64 + lane];
+                    mt = s_t[64 + lane];
+                }
+                __syncthreads();
+                if (lane < rest) {
+                    s_e2[lane] = m2;
+                    s_e4[lane] = m4;
+                    s_t[lane] = mt;
+                }
+                qn = rest;
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < qn) evaluate(lane);
+    __syncthreads();
+    // flush the tile accumulator
+    const int o1 = a.s12.cart[r1];
+    for (int q = lane; q < (j3b - j3a) * N13; q += 64) {
+        const double v = s_acc[q];
+        if (v == 0.0) continue;
+        const int tt = q / N13, ij = q - tt * N13, i = ij / N3, j = ij - i * N3;
+        const int r3 = a.s34.rows[j3a + tt];
+        const int o3 = a.s34.cart[r3];
+        atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
+        if (a.symmetric && r1 != r3) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
+    }
+    if (a.count) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) nq += __shfl_xor(nq, o);
+        if (lane == 0 && nq) {
+            atomicAdd(&a.sc->counts[kcount_index(L1, L2, L3, L4)], nq);
+            if constexpr (LO) atomicAdd(&a.sc->counts32[kcount_index(L1, L2, L3, L4)], nq);
+        }
+    }
+}
+
+// Launch of one class at row angular momentum L1 (instantiated in exchange_l<L1>.hip).
+template <int L1>
+void launch_k_quartets_l1(int L2, int L3, int L4, int route, const KArgs& a, long long ntiles, hipStream_t s)
+{
+    const int nb3 = (a.s34.nrows + kKTile - 1) / kKTile;
+    dim3 grid((unsigned)ntiles), block(64);
+#define CASE(B, C, D)                                                                               \
+    if (L2 == B && L3 == C && L4 == D) {                                                            \
+        if (route == kRouteLo)                                                                      \
+            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRouteLo>), grid, block, 0, s, a, nb3);       \
+        else if (route == kRouteHi)                                                                 \
+            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRouteHi>), grid, block, 0, s, a, nb3);       \
+        else                                                                                        \
+            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRoutePlain>), grid, block, 0, s, a, nb3);    \
+        return;                                                                                     \
+    }
+#define C4(B, C) CASE(B, C, 0) CASE(B, C, 1) CASE(B, C, 2)
+#define C3(B) C4(B, 0) C4(B, 1) C4(B, 2)
+    C3(0) C3(1) C3(2)
+#undef C3
+#undef C4
+#undef CASE
+}
+
+}  // namespace lsjk

@@ -1,0 +1,6 @@
+// Instantiations of the exchange kernels with bra row angular momentum L1 = 0.
+#include "exchange_impl.hpp"
+
+namespace lsjk {
+template void launch_k_quartets_l1<0>(int, int, int, int, const KArgs&, long long, hipStream_t);
+}  // namespace lsjk

@@ -59,7 +59,8 @@ struct PairList {
     int* d_eprim = nullptr;   // partner primitive
     int* d_ecart = nullptr;   // partner Cartesian offset
     int* d_eblk = nullptr;    // partner block id
-    double* d_egeom = nullptr; // SoA: p, Px, Py, Pz, K, Bx, By, Bz  (8 * nent)
+    double* d_egeom = nullptr; // packed entries (12 doubles each: p, K, P, P-A, A-B, {cart, blk})
+    double* d_egeom32 = nullptr; // FP32-route entries (64 bytes each: P in FP64, 8 floats, {cart, blk})
     // pivot view for counting (rows = basis-2 primitives, partners = basis-1 primitives, bounds
     // only); a symmetric list uses the row view above
     int* d_piv_row_start = nullptr;
@@ -96,6 +97,7 @@ struct CallScalars {
     unsigned int pad;
     unsigned long long dmax_bits;  // max |D| (double bits)
     unsigned long long counts[1024]; // evaluated primitive quartets (see jcount_index / kcount_index)
+    unsigned long long counts32[1024]; // the part of them evaluated on the FP32 route
     unsigned int queue[1024];        // work-queue counters of the persistent kernels
 };
 

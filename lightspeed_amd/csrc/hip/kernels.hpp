@@ -11,6 +11,12 @@ __host__ __device__ constexpr int kcount_index(int L1, int L2, int L3, int L4)
     return 256 + ((L1 * 5 + L2) * 5 + L3) * 5 + L4;
 }
 
+// Precision routes of a quartet kernel launch (the reference's documented mixed precision,
+// intbox.hpp:736-742: FP64 when est >= thredp, FP32 when thresp <= est < thredp, else skipped).
+constexpr int kRoutePlain = 0;  // est >= thresp, FP64 (thredp <= thresp)
+constexpr int kRouteHi = 1;     // est >= thredp, FP64
+constexpr int kRouteLo = 2;     // thresp <= est < thredp, FP32 arithmetic, FP64 accumulation
+
 struct JPrepArgs {
     const int* p1;
     const int* p2;
@@ -27,6 +33,7 @@ struct JPrepArgs {
     float thre_ket;
     float* kb;
     double* kd;
+    float* kd32;  // FP32 copy of the densities for the FP32 route (nullptr: not needed)
     CallScalars* sc;
 };
 
@@ -49,6 +56,8 @@ struct JQuartetArgs {
     int ket_n;
     int ket_hstart;
     const double* ket_d;  // Hermite densities of the ket group
+    const float* ket_d32; // their FP32 copy (FP32 route)
+    float thredp;
     CallScalars* sc;
 };
 
@@ -72,13 +81,19 @@ struct JScatterArgs {
 
 // K: one wave per tile = (row1, kKTile consecutive row3s) of fixed angular momenta.
 constexpr int kKTile = 16;
+// A packed partner entry (row primitive A, partner primitive B): p, K_AB, P (3), P - A (3),
+// A - B (3), then the partner's Cartesian offset and block id as two ints in the last double.
+constexpr int kKEntDoubles = 12;
+// The FP32-route copy: P (3 doubles), p, K, P - A, A - B (8 floats), {cart, blk}: 64 bytes.
+constexpr int kKEnt32Doubles = 8;
 struct KSide {
     const int* row_start;  // [nrow * (maxLp+1) + 1]
     int maxLp;
     const float* eb;       // entry bound
     const int* ecart;      // partner Cartesian offset
     const int* eblk;       // partner block id
-    const double* eg;      // SoA p, Px, Py, Pz, K, Bx, By, Bz (stride nent)
+    const double* eg;      // packed entries, kKEntDoubles doubles each (see KEnt)
+    const double* eg32;    // FP32-route entries, kKEnt32Doubles doubles each
     int nent;
     const int* rows;       // rows of this class with partners of the class's partner L (primitive indices)
     int nrows;
@@ -90,6 +105,7 @@ struct KArgs {
     RysTable rys;
     Ewald ew;
     float thresp;
+    float thredp;
     int count;
     int symmetric;  // compute only row1 <= row3 and add the transposed block
     int tri;        // units are the (row1 <= row3) pairs given by tri_pre / tri_start
@@ -143,15 +159,11 @@ void launch_count_k(const CountKArgs& a, hipStream_t s);
 void launch_j_ket_prep(int LA, int LB, const JPrepArgs& a, int k0, int nk, int hstart, size_t hoff, hipStream_t s);
 void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk, int hstart, size_t hoff,
                           hipStream_t s);
-void launch_j_quartets(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask, unsigned int* qctr,
-                       hipStream_t s);
+void launch_j_quartets(int L12, int L34, int route, const JQuartetArgs& a, const JTask* tasks, int ntask,
+                       unsigned int* qctr, hipStream_t s);
 
 // Work-queue counters of one call (CallScalars::queue, zeroed with the call's scalars): J classes
-// at jcount_index, K classes at kqueue_index (8 per class, one per XCD partition).
-__host__ __device__ constexpr int kqueue_index(int L1, int L2, int L3, int L4)
-{
-    return 64 + (((L1 * 3 + L2) * 3 + L3) * 3 + L4) * 8;
-}
+// at jcount_index.
 
 // Grid of a persistent kernel: as many workgroups as fit on the device at once (occupancy at
 // this block size and LDS), but no more than the work needs.  Raises if the LDS request exceeds
@@ -161,8 +173,7 @@ int persistent_grid(const void* kernel, int block, size_t lds, long long nwork_g
 void launch_k_blockmax(const double* D, int ldD, const int* blk_cart2, const int* blk_L2, int nblk2,
                        const int* blk_cart4, const int* blk_L4, int nblk4, double* Dsh, CallScalars* sc,
                        hipStream_t s);
-void launch_k_quartets(int L1, int L2, int L3, int L4, const KArgs& a, long long nunits, unsigned int* qctr,
-                       hipStream_t s);
+void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a, long long ntiles, hipStream_t s);
 void launch_fill(void* p, size_t bytes, hipStream_t s);
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 

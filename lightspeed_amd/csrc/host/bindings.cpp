@@ -254,6 +254,11 @@ PYBIND11_MODULE(_lightspeed, m)
         check(lsjk_last_counts((uint64_t*)a.mutable_data()));
         return a;
     });
+    m.def("last_counts_fp32", []() {
+        py::array_t<uint64_t> a(1024);
+        check(lsjk_last_counts_fp32((uint64_t*)a.mutable_data()));
+        return a;
+    });
     m.def("last_timing", []() {
         double ms[2];
         check(lsjk_last_timing(ms));

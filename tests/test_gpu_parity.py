@@ -152,3 +152,74 @@ def test_reference_convention_counts(product_backend, oracle_backend):
     assert np.array_equal(nj.astype(np.int64), cj)
     assert np.array_equal(nk.astype(np.int64), ck)
     assert np.array_equal(nk_ns.astype(np.int64), ck_ns)
+
+
+# ---------------------------------------------------------------- mixed precision route
+
+def _mixed_problem(ls, name, seed=1):
+    from lightspeed_amd import workloads as W
+    mol, bname, nocc = W.config(name)
+    bas = ls.Basis.from_gbs_file(mol, bname)
+    D = W.random_density(bas.nao, nocc, seed)
+    return bas, D
+
+
+def test_mixed_route_h2o10_against_oracle(product_backend, oracle_backend):
+    """thredp > thresp: FP64 for est >= thredp, FP32 for thresp <= est < thredp, FP64 accumulation
+    (intbox.hpp:736-742).  Against the FP64 oracle: max|dJ|, max|dK| <= 1e-7 at thredp = 1e-6 (the
+    accuracy bar of the mixed route); the FP64 and FP32 launches together evaluate exactly the
+    quartets of the plain route (per-class counts)."""
+    ls = product_backend.ls
+    _say("mixed route h2o10 vs oracle")
+    bas, D = _mixed_problem(ls, "h2o10_631g")
+    pl = ls.PairList.build_schwarz(bas, bas, True, 1.0e-12)
+    ob = oracle_backend.O.OBasis(bas)
+    opl = oracle_backend.O.OPairList(ob, ob, True, 1.0e-12)
+    Jo = oracle_backend.O.coulomb(FC.COULOMB, opl, opl, D, 1.0e-12)
+    Ko = oracle_backend.O.exchange(FC.COULOMB, opl, opl, D, True, 1.0e-12)
+    res = ls.ResourceList.build()
+    ew = ls.Ewald([1.0], [-1.0])
+    ls.set_counting(True)
+    try:
+        J0 = ls.IntBox.coulomb(res, ew, pl, pl, ls.Tensor.array(D), 1.0e-12, 1.0e-12).np
+        cj0 = ls.last_counts().astype(np.int64)
+        J = ls.IntBox.coulomb(res, ew, pl, pl, ls.Tensor.array(D), 1.0e-12, 1.0e-6).np
+        cj = ls.last_counts().astype(np.int64)
+        K0 = ls.IntBox.exchange(res, ew, pl, pl, ls.Tensor.array(D), True, 1.0e-12, 1.0e-12).np
+        ck0 = ls.last_counts().astype(np.int64)
+        K = ls.IntBox.exchange(res, ew, pl, pl, ls.Tensor.array(D), True, 1.0e-12, 1.0e-6).np
+        ck = ls.last_counts().astype(np.int64)
+    finally:
+        ls.set_counting(False)
+    assert np.array_equal(cj, cj0) and np.array_equal(ck, ck0)
+    dJ, dK = np.abs(J - Jo).max(), np.abs(K - Ko).max()
+    _say("  max|dJ| = %.3e  max|dK| = %.3e" % (dJ, dK))
+    assert dJ <= 1.0e-7 and dK <= 1.0e-7
+    # the FP32 route changes results (it is not silently the FP64 one) ...
+    assert np.abs(J - J0).max() > 0.0 and np.abs(K - K0).max() > 0.0
+    # ... and with thredp <= thresp it is the FP64 route (equal up to the order of the FP64
+    # atomic accumulation)
+    J1 = ls.IntBox.coulomb(res, ew, pl, pl, ls.Tensor.array(D), 1.0e-12, 1.0e-13).np
+    assert np.abs(J1 - J0).max() <= 1.0e-12
+
+
+def test_mixed_route_ewald_and_nonsymmetric(product_backend, oracle_backend):
+    """The FP32 route with erf-attenuated operators and a non-symmetric K call (D24symm False)."""
+    ls = product_backend.ls
+    _say("mixed route ewald / nonsymmetric")
+    mol = FC.molecule("h2o")
+    bas = ls.Basis.from_gbs_file(mol, "cc-pvdz")
+    rng = np.random.default_rng(5)
+    D = rng.standard_normal((bas.nao, bas.nao))
+    pl = ls.PairList.build_schwarz(bas, bas, False, 1.0e-14)
+    ob = oracle_backend.O.OBasis(bas)
+    opl = oracle_backend.O.OPairList(ob, ob, False, 1.0e-14)
+    res = ls.ResourceList.build()
+    for ew in (FC.COULOMB, FC.EWALD_OPS["MIX"], FC.EWALD_OPS["SR"]):
+        J = ls.IntBox.coulomb(res, ls.Ewald(*ew), pl, pl, ls.Tensor.array(D), 1.0e-14, 1.0e-4).np
+        K = ls.IntBox.exchange(res, ls.Ewald(*ew), pl, pl, ls.Tensor.array(D), False, 1.0e-14, 1.0e-4).np
+        Jo = oracle_backend.O.coulomb(ew, opl, opl, D, 1.0e-14)
+        Ko = oracle_backend.O.exchange(ew, opl, opl, D, False, 1.0e-14)
+        scale = max(1.0, np.abs(Jo).max())
+        assert np.abs(J - Jo).max() <= 1.0e-6 * scale, ew
+        assert np.abs(K - Ko).max() <= 1.0e-6 * scale, ew
