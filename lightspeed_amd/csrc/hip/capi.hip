@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -448,6 +449,9 @@ void free_pairlist(lsjk_pairlist* pl)
     dev_free(pl->d_cls_L2);
     for (auto& kv : pl->jtasks) dev_free(kv.second.first);
     pl->jtasks.clear();
+    for (auto& kv : pl->kunits) dev_free(kv.second.first);
+    pl->kunits.clear();
+    pl->kcost.clear();
 }
 
 // J task list for (bra list, ket list): per (L12, L34) group pair, tiles of 64 bra pairs x chunks of
@@ -461,6 +465,61 @@ struct JSegments {
     int off[5][5] = {{0}};
     int cnt[5][5] = {{0}};
 };
+
+// K work units of one class, heaviest first.  A unit is (row1, block of kKTile row3s, chunk of at
+// most kKChunk2 of row1's partners); its cost estimate is (chunk partners) x (partners of the
+// block's row3s) from the bound-screened lists alone (D only removes quartets, it never adds).
+// In the triangle mode row3s below row1 are not counted and units without work are dropped.
+// Dispatching heaviest first keeps long units from finishing after the rest of the launch.
+std::pair<const lsjk::KUnit*, int> get_kunits(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34, int L1, int L2,
+                                              int L3, int L4, bool tri, double* cost_out)
+{
+    using lsjk::KUnit;
+    lsjk_pairlist* pl12 = const_cast<lsjk_pairlist*>(pl12c);
+    const int cls = ((((L1 * 3 + L2) * 3 + L3) * 3 + L4) << 1) | (tri ? 1 : 0);
+    const auto key = std::make_pair(pl34->uid, cls);
+    auto it = pl12->kunits.find(key);
+    if (it == pl12->kunits.end()) {
+        const auto& rows1 = pl12->h_rows[L1][L2];
+        const auto& rows3 = pl34->h_rows[L3][L4];
+        const int m12 = pl12->kmaxL2 + 1, m34 = pl34->kmaxL2 + 1;
+        std::vector<long long> n4(rows3.size());
+        for (size_t j = 0; j < rows3.size(); j++) {
+            const size_t r3 = (size_t)rows3[j];
+            n4[j] = pl34->h_row_start[r3 * m34 + L4 + 1] - pl34->h_row_start[r3 * m34 + L4];
+        }
+        const int nb3 = (int)((rows3.size() + lsjk::kKTile - 1) / lsjk::kKTile);
+        std::vector<std::pair<long long, KUnit>> v;
+        double total = 0.0;
+        for (size_t i1 = 0; i1 < rows1.size(); i1++) {
+            const size_t r1 = (size_t)rows1[i1];
+            const int n2 = pl12->h_row_start[r1 * m12 + L2 + 1] - pl12->h_row_start[r1 * m12 + L2];
+            for (int ib = 0; ib < nb3; ib++) {
+                long long sum4 = 0;
+                const size_t j0 = (size_t)ib * lsjk::kKTile, j1 = std::min(rows3.size(), j0 + lsjk::kKTile);
+                for (size_t j = j0; j < j1; j++)
+                    if (!tri || rows3[j] >= (int)r1) sum4 += n4[j];
+                if (sum4 == 0) continue;
+                for (int e = 0; e < n2; e += lsjk::kKChunk2) {
+                    const int cnt = std::min(lsjk::kKChunk2, n2 - e);
+                    v.push_back({(long long)cnt * sum4, KUnit{(int)i1, ib, e, cnt}});
+                    total += (double)cnt * (double)sum4;
+                }
+            }
+        }
+        std::stable_sort(v.begin(), v.end(), [](const std::pair<long long, KUnit>& a,
+                                                const std::pair<long long, KUnit>& b) { return a.first > b.first; });
+        std::vector<KUnit> u(v.size());
+        for (size_t i = 0; i < v.size(); i++) u[i] = v[i].second;
+        if (u.size() > (size_t)std::numeric_limits<int>::max())
+            throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk: too many exchange work units");
+        KUnit* d = u.empty() ? nullptr : dev_upload(u);
+        it = pl12->kunits.insert({key, {(void*)d, (int)u.size()}}).first;
+        pl12->kcost[key] = total;
+    }
+    if (cost_out) *cost_out = pl12->kcost[key];
+    return {static_cast<const KUnit*>(it->second.first), it->second.second};
+}
 
 JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
 {
@@ -975,7 +1034,9 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
             fo.begin();
             struct KL {
                 int L1, L2, L3, L4;
-                long long nunits;
+                const KUnit* units;
+                int nunits;
+                double cost;
                 KArgs ka;
             };
             std::vector<KL> kls;
@@ -988,11 +1049,11 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                             const auto& rows3 = pl34->h_rows[L3][L4];
                             if (rows1.empty() || rows3.empty()) continue;
                             // in the symmetric mode with equal row momenta only row1 <= row3 units
-                            // are evaluated (tiles below the diagonal exit at once)
+                            // are evaluated
                             const bool tri = symmode && (L1 == L3);
-                            const long long nunits =
-                                (long long)rows1.size() * (long long)((rows3.size() + kKTile - 1) / kKTile);
-                            if (nunits == 0) continue;
+                            double cost = 0.0;
+                            const auto units = get_kunits(pl12, pl34, L1, L2, L3, L4, tri, &cost);
+                            if (units.second == 0) continue;
                             KArgs ka{};
                             ka.rys = S.rys;
                             ka.ew = ew;
@@ -1033,24 +1094,25 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                             ka.maxb12 = pl12->maxbound;
                             ka.maxb34 = pl34->maxbound;
                             ka.sc = S.d_sc;
-                            kls.push_back({L1, L2, L3, L4, nunits, ka});
+                            // class cost: candidate pairs x FLOPs of one quartet of the class
+                            const int Lt = L1 + L2 + L3 + L4;
+                            const double Pi = (double)ncart(L1) * ncart(L2) * ncart(L3) * ncart(L4);
+                            const double G = (double)(L1 + L2 + 1) * (L3 + L4 + 1);
+                            const double F = 40.0 + (Lt / 2 + 1) * (30.0 + 9.0 * G + 3.0 * Pi) + 2.0 * Pi;
+                            kls.push_back({L1, L2, L3, L4, units.first, units.second, cost * F, ka});
                         }
-            // heaviest classes first (angular momentum, then number of output pairs)
-            std::stable_sort(kls.begin(), kls.end(), [](const KL& a, const KL& b) {
-                int la = a.L1 + a.L2 + a.L3 + a.L4, lb = b.L1 + b.L2 + b.L3 + b.L4;
-                if (la != lb) return la > lb;
-                return a.nunits > b.nunits;
-            });
+            // heaviest classes first (estimated cost)
+            std::stable_sort(kls.begin(), kls.end(), [](const KL& a, const KL& b) { return a.cost > b.cost; });
             for (auto& k : kls) {
                 const int idx = kcount_index(k.L1, k.L2, k.L3, k.L4);
                 hipStream_t ss = fo.pick();
                 lt.begin(idx, ss);
-                launch_k_quartets(k.L1, k.L2, k.L3, k.L4, mixed ? kRouteHi : kRoutePlain, k.ka, k.nunits, ss);
+                launch_k_quartets(k.L1, k.L2, k.L3, k.L4, mixed ? kRouteHi : kRoutePlain, k.ka, k.units, k.nunits, ss);
                 lt.end();
                 if (mixed) {
                     hipStream_t sl = fo.pick();
                     lt.begin(idx + kLoRouteId, sl);
-                    launch_k_quartets(k.L1, k.L2, k.L3, k.L4, kRouteLo, k.ka, k.nunits, sl);
+                    launch_k_quartets(k.L1, k.L2, k.L3, k.L4, kRouteLo, k.ka, k.units, k.nunits, sl);
                     lt.end();
                 }
             }

@@ -6,10 +6,11 @@
 namespace lsjk {
 
 template <int L1>
-void launch_k_quartets_l1(int L2, int L3, int L4, int route, const KArgs& a, long long ntiles, hipStream_t s);
-extern template void launch_k_quartets_l1<0>(int, int, int, int, const KArgs&, long long, hipStream_t);
-extern template void launch_k_quartets_l1<1>(int, int, int, int, const KArgs&, long long, hipStream_t);
-extern template void launch_k_quartets_l1<2>(int, int, int, int, const KArgs&, long long, hipStream_t);
+void launch_k_quartets_l1(int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,
+                          hipStream_t s);
+extern template void launch_k_quartets_l1<0>(int, int, int, int, const KArgs&, const KUnit*, int, hipStream_t);
+extern template void launch_k_quartets_l1<1>(int, int, int, int, const KArgs&, const KUnit*, int, hipStream_t);
+extern template void launch_k_quartets_l1<2>(int, int, int, int, const KArgs&, const KUnit*, int, hipStream_t);
 
 // Block maxima of |D24| per (block2, block4) and the global maximum.
 __global__ void k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2, int nb2, const int* bc4,
@@ -69,15 +70,16 @@ void launch_k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2,
                        bL4, nb4, Dsh, sc);
 }
 
-void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a, long long ntiles, hipStream_t s)
+void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,
+                       hipStream_t s)
 {
-    if (ntiles <= 0) return;
+    if (nunits <= 0) return;
     if (L1 == 0)
-        launch_k_quartets_l1<0>(L2, L3, L4, route, a, ntiles, s);
+        launch_k_quartets_l1<0>(L2, L3, L4, route, a, units, nunits, s);
     else if (L1 == 1)
-        launch_k_quartets_l1<1>(L2, L3, L4, route, a, ntiles, s);
+        launch_k_quartets_l1<1>(L2, L3, L4, route, a, units, nunits, s);
     else
-        launch_k_quartets_l1<2>(L2, L3, L4, route, a, ntiles, s);
+        launch_k_quartets_l1<2>(L2, L3, L4, route, a, units, nunits, s);
 }
 
 }  // namespace lsjk

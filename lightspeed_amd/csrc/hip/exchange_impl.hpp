@@ -252,7 +252,7 @@ constexpr int kKQueue = 128;
 // added to the tile's LDS accumulator, which is flushed to K once at the end (plus the transposed
 // block in the symmetric mode).
 template <int L1, int L2, int L3, int L4, int ROUTE>
-__global__ __launch_bounds__(64) void k_tiles(KArgs a, int nb3)
+__global__ __launch_bounds__(64) void k_tiles(KArgs a, const KUnit* __restrict__ units)
 {
     constexpr bool LO = ROUTE == kRouteLo;
     using Real = typename std::conditional<LO, float, double>::type;
@@ -260,12 +260,10 @@ __global__ __launch_bounds__(64) void k_tiles(KArgs a, int nb3)
     __shared__ double s_acc[kKTile * N13];
     __shared__ int s_e2[kKQueue], s_e4[kKQueue], s_t[kKQueue];
     const int lane = threadIdx.x;
-    const int i1 = (int)(blockIdx.x / nb3);
-    const int ib = (int)(blockIdx.x - (unsigned)i1 * nb3);
-    const int r1 = a.s12.rows[i1];
-    const int j3a = ib * kKTile;
+    const KUnit unit = units[blockIdx.x];
+    const int r1 = a.s12.rows[unit.i1];
+    const int j3a = unit.ib * kKTile;
     const int j3b = min(j3a + kKTile, a.s34.nrows);
-    if (a.tri && a.s34.rows[j3b - 1] < r1) return;  // rows are in primitive order
     for (int q = lane; q < kKTile * N13; q += 64) s_acc[q] = 0.0;
 
     // quartets with est at or above `thresp` here (and, on the FP32 route, below thredp) are evaluated
@@ -274,8 +272,9 @@ __global__ __launch_bounds__(64) void k_tiles(KArgs a, int nb3)
     const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
     const float thre12 = (float)((double)thresp / ((double)a.maxb34 * dmax));
     const float thre34 = (float)((double)thresp / ((double)a.maxb12 * dmax));
-    const int a0 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2];
-    const int a1 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2 + 1];
+    // this unit's chunk of row1's partners (bound order, so the surviving ones are a prefix)
+    const int a0 = a.s12.row_start[r1 * (a.s12.maxLp + 1) + L2] + unit.e2off;
+    const int a1 = a0 + unit.e2cnt;
     const float b12top = a.s12.eb[a0];
     // partners of row1 that can pass with the largest possible ket bound are a prefix
     int n2max = 0;
@@ -406,18 +405,18 @@ __global__ __launch_bounds__(64) void k_tiles(KArgs a, int nb3)
 
 // Launch of one class at row angular momentum L1 (instantiated in exchange_l<L1>.hip).
 template <int L1>
-void launch_k_quartets_l1(int L2, int L3, int L4, int route, const KArgs& a, long long ntiles, hipStream_t s)
+void launch_k_quartets_l1(int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,
+                          hipStream_t s)
 {
-    const int nb3 = (a.s34.nrows + kKTile - 1) / kKTile;
-    dim3 grid((unsigned)ntiles), block(64);
+    dim3 grid((unsigned)nunits), block(64);
 #define CASE(B, C, D)                                                                               \
     if (L2 == B && L3 == C && L4 == D) {                                                            \
         if (route == kRouteLo)                                                                      \
-            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRouteLo>), grid, block, 0, s, a, nb3);       \
+            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRouteLo>), grid, block, 0, s, a, units);       \
         else if (route == kRouteHi)                                                                 \
-            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRouteHi>), grid, block, 0, s, a, nb3);       \
+            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRouteHi>), grid, block, 0, s, a, units);       \
         else                                                                                        \
-            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRoutePlain>), grid, block, 0, s, a, nb3);    \
+            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRoutePlain>), grid, block, 0, s, a, units);    \
         return;                                                                                     \
     }
 #define C4(B, C) CASE(B, C, 0) CASE(B, C, 1) CASE(B, C, 2)

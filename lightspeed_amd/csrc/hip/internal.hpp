@@ -81,6 +81,10 @@ struct PairList {
     unsigned long long uid = 0;
     // cached J task lists keyed by the ket list's uid
     std::map<unsigned long long, std::pair<int*, int>> jtasks;
+    // cached K unit lists (device pointer, count) keyed by (ket list uid, class, triangle flag)
+    std::map<std::pair<unsigned long long, int>, std::pair<void*, int>> kunits;
+    // cost of the class's units (sum over units of partner-count products), same key
+    std::map<std::pair<unsigned long long, int>, double> kcost;
 };
 
 // ---------------------------------------------------------------- per-call argument blocks

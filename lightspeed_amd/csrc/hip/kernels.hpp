@@ -81,6 +81,14 @@ struct JScatterArgs {
 
 // K: one wave per tile = (row1, kKTile consecutive row3s) of fixed angular momenta.
 constexpr int kKTile = 16;
+// Row-1 partners per work unit: a tile's partner range is split into chunks of this size so that
+// no unit is much heavier than the rest.
+constexpr int kKChunk2 = 64;
+// A K work unit: row1 = rows1[i1], row3 block ib (rows3[ib*kKTile ...]), row1's partners
+// [e2off, e2off + e2cnt) of the class's partner angular momentum.
+struct KUnit {
+    int i1, ib, e2off, e2cnt;
+};
 // A packed partner entry (row primitive A, partner primitive B): p, K_AB, P (3), P - A (3),
 // A - B (3), then the partner's Cartesian offset and block id as two ints in the last double.
 constexpr int kKEntDoubles = 12;
@@ -173,7 +181,8 @@ int persistent_grid(const void* kernel, int block, size_t lds, long long nwork_g
 void launch_k_blockmax(const double* D, int ldD, const int* blk_cart2, const int* blk_L2, int nblk2,
                        const int* blk_cart4, const int* blk_L4, int nblk4, double* Dsh, CallScalars* sc,
                        hipStream_t s);
-void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a, long long ntiles, hipStream_t s);
+void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,
+                       hipStream_t s);
 void launch_fill(void* p, size_t bytes, hipStream_t s);
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
