@@ -95,6 +95,20 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                   int nterm, const double* scales, const double* omegas, float thresp, float thredp,
                   double* K13_dev);
 
+/* One-electron integrals of a pair list, accumulated into ncart1 x ncart2 device matrices
+ * (Cartesian, as lsjk_coulomb).  A symmetric list fills both triangles.
+ * lsjk_overlap   replaces IntBox::overlap   (reference src/cpp/intbox/intbox.cpp:30-40,
+ *                                            cpu/overlap.cpp:14-147): S += <a|b>;
+ * lsjk_kinetic   replaces IntBox::kinetic   (intbox.cpp:183-193, cpu/kinetic.cpp:14-175):
+ *                                            T += <a| -1/2 nabla^2 |b>;
+ * lsjk_potential replaces IntBox::potential (intbox.cpp:196-210, cpu/potential.cpp:14-147):
+ *                V += sum_A <a| O(r - R_A) |b> (-c_A) for ncharge point charges xyzc (host,
+ *                ncharge x 4: x, y, z, c), O the Ewald operator as in lsjk_coulomb. */
+int lsjk_overlap(const lsjk_pairlist* pl, double* S_dev);
+int lsjk_kinetic(const lsjk_pairlist* pl, double* T_dev);
+int lsjk_potential(const lsjk_pairlist* pl, int nterm, const double* scales, const double* omegas, int ncharge,
+                   const double* xyzc, double* V_dev);
+
 /* Device memory helpers for callers without their own HIP allocator. */
 int lsjk_malloc(void** ptr, size_t bytes);
 int lsjk_free(void* ptr);

@@ -987,6 +987,7 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
         sa.J = J12_dev;
         sa.ldJ = b2->ncart;
         sa.symmetric = pl12->symmetric;
+        sa.screen = 1;
         sa.thresp = thresp;
         sa.sc = S.d_sc;
         if (ew.n > 0) {
@@ -1122,6 +1123,109 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
         HIPCHK(hipEventRecord(S.ev[2], st));
         HIPCHK(hipEventRecord(S.ev[3], st));
         finish_call(S.counting);
+    });
+}
+
+// One-electron integrals of one pair list (accumulated, Cartesian): S and/or T.
+static void one_e_st(const lsjk_pairlist* pl, double* Sout, double* Tout)
+{
+    using namespace lsjk;
+    require_init();
+    if (!pl) throw LsjkError(LSJK_E_INVALID, "lsjk: null pair list");
+    OneEArgs a{};
+    const Basis *b1 = pl->b1, *b2 = pl->b2;
+    a.p1 = pl->d_p1;
+    a.p2 = pl->d_p2;
+    a.hpos = pl->d_hpos;
+    a.K = pl->d_K;
+    a.eA = b1->d_e; a.xA = b1->d_x; a.yA = b1->d_y; a.zA = b1->d_z; a.cartA = b1->d_cart;
+    a.eB = b2->d_e; a.xB = b2->d_x; a.yB = b2->d_y; a.zB = b2->d_z; a.cartB = b2->d_cart;
+    a.symmetric = pl->symmetric;
+    a.S = Sout;
+    a.T = Tout;
+    a.ld = b2->ncart;
+    for (size_t c = 0; c + 1 < pl->cls_start.size(); c++)
+        launch_one_e_st(pl->cls_L1[c], pl->cls_L2[c], a, pl->cls_start[c], pl->cls_start[c + 1] - pl->cls_start[c],
+                        S.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(S.stream));
+}
+
+int lsjk_overlap(const lsjk_pairlist* pl, double* S_dev)
+{
+    return guarded([&] {
+        if (!S_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_overlap: null output");
+        one_e_st(pl, S_dev, nullptr);
+    });
+}
+
+int lsjk_kinetic(const lsjk_pairlist* pl, double* T_dev)
+{
+    return guarded([&] {
+        if (!T_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_kinetic: null output");
+        one_e_st(pl, nullptr, T_dev);
+    });
+}
+
+int lsjk_potential(const lsjk_pairlist* pl, int nterm, const double* scales, const double* omegas, int ncharge,
+                   const double* xyzc, double* V_dev)
+{
+    return guarded([&] {
+        using namespace lsjk;
+        require_init();
+        if (!pl || !V_dev || (ncharge > 0 && !xyzc) || ncharge < 0)
+            throw LsjkError(LSJK_E_INVALID, "lsjk_potential: null or invalid argument");
+        Ewald ew;
+        check_ewald(nterm, scales, omegas, ew);
+        if (ew.n == 0 || ncharge == 0) return;
+        hipStream_t st = S.stream;
+        const Basis *b1 = pl->b1, *b2 = pl->b2;
+        double* vp = nullptr;
+        double* dxyzc = nullptr;
+        Workspace wsp;
+        wsp.add(&vp, pl->hoff[2 * kMaxL + 1]);
+        wsp.add(&dxyzc, (size_t)4 * ncharge);
+        wsp.commit();
+        HIPCHK(hipMemcpyAsync(dxyzc, xyzc, sizeof(double) * 4 * ncharge, hipMemcpyHostToDevice, st));
+        OneEArgs a{};
+        a.p1 = pl->d_p1;
+        a.p2 = pl->d_p2;
+        a.hpos = pl->d_hpos;
+        a.K = pl->d_K;
+        a.eA = b1->d_e; a.xA = b1->d_x; a.yA = b1->d_y; a.zA = b1->d_z; a.cartA = b1->d_cart;
+        a.eB = b2->d_e; a.xB = b2->d_x; a.yB = b2->d_y; a.zB = b2->d_z; a.cartB = b2->d_cart;
+        a.symmetric = pl->symmetric;
+        a.rys = S.rys;
+        a.ew = ew;
+        a.xyzc = dxyzc;
+        a.nA = ncharge;
+        a.vp = vp;
+        for (size_t c = 0; c + 1 < pl->cls_start.size(); c++) {
+            const int L = pl->cls_L1[c] + pl->cls_L2[c];
+            launch_one_e_v(pl->cls_L1[c], pl->cls_L2[c], a, pl->cls_start[c], pl->cls_start[c + 1] - pl->cls_start[c],
+                           pl->hstart[L], pl->hoff[L], st);
+        }
+        JScatterArgs sa{};
+        sa.p1 = pl->d_p1;
+        sa.p2 = pl->d_p2;
+        sa.bound = pl->d_bound;
+        sa.hpos = pl->d_hpos;
+        sa.K = pl->d_K;
+        sa.eA = b1->d_e; sa.xA = b1->d_x; sa.yA = b1->d_y; sa.zA = b1->d_z; sa.cartA = b1->d_cart;
+        sa.eB = b2->d_e; sa.xB = b2->d_x; sa.yB = b2->d_y; sa.zB = b2->d_z; sa.cartB = b2->d_cart;
+        sa.jp = vp;
+        sa.J = V_dev;
+        sa.ldJ = b2->ncart;
+        sa.symmetric = pl->symmetric;
+        sa.screen = 0;
+        sa.sc = S.d_sc;
+        for (size_t c = 0; c + 1 < pl->cls_start.size(); c++) {
+            const int L = pl->cls_L1[c] + pl->cls_L2[c];
+            launch_j_bra_scatter(pl->cls_L1[c], pl->cls_L2[c], sa, pl->cls_start[c],
+                                 pl->cls_start[c + 1] - pl->cls_start[c], pl->hstart[L], pl->hoff[L], st);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
     });
 }
 

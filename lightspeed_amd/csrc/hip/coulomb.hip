@@ -377,9 +377,11 @@ __global__ __launch_bounds__(256) void j_bra_scatter(JScatterArgs a, int k0, int
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nk) return;
     const int k = k0 + i;
-    const float max34 = __uint_as_float(a.sc->max34_bits);
-    const float thre_bra = a.thresp / max34;
-    if (!(a.bound[k] >= thre_bra)) return;
+    if (a.screen) {
+        const float max34 = __uint_as_float(a.sc->max34_bits);
+        const float thre_bra = a.thresp / max34;
+        if (!(a.bound[k] >= thre_bra)) return;
+    }
     const int ia = a.p1[k], ib = a.p2[k];
     const int pos = a.hpos[k];
     const double* jp = a.jp + hoff + (size_t)(pos - hstart) * H;

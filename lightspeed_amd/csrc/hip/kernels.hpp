@@ -75,9 +75,34 @@ struct JScatterArgs {
     double* J;
     int ldJ;
     int symmetric;
+    int screen;  // 1: only bra pairs that survive truncate_bra (J); 0: every pair (potential)
     float thresp;
     CallScalars* sc;
 };
+
+// One-electron integrals (overlap, kinetic, potential) of one pair list.
+struct OneEArgs {
+    const int* p1;
+    const int* p2;
+    const int* hpos;
+    const double* K;
+    const double *eA, *xA, *yA, *zA;
+    const int* cartA;
+    const double *eB, *xB, *yB, *zB;
+    const int* cartB;
+    int symmetric;
+    double* S;  // ncart1 x ncart2 outputs (nullptr: not wanted)
+    double* T;
+    int ld;
+    // potential: point charges and operator
+    RysTable rys;
+    Ewald ew;
+    const double* xyzc;  // nA x 4
+    int nA;
+    double* vp;  // Hermite accumulators (Hermite order, same layout as the J bra buffer)
+};
+void launch_one_e_st(int LA, int LB, const OneEArgs& a, int k0, int nk, hipStream_t s);
+void launch_one_e_v(int LA, int LB, const OneEArgs& a, int k0, int nk, int hstart, size_t hoff, hipStream_t s);
 
 // K: one wave per tile = (row1, kKTile consecutive row3s) of fixed angular momenta.
 constexpr int kKTile = 16;

@@ -238,6 +238,12 @@ PYBIND11_MODULE(_lightspeed, m)
                 IntBox::exchange_device(ew, a, b, (const double*)D, D24symm, thresp, thredp, (double*)K);
             },
             py::call_guard<py::gil_scoped_release>())
+        .def_static("overlap", &IntBox::overlap, py::arg("resources"), py::arg("pairlist"),
+                    py::arg("S") = std::shared_ptr<Tensor>(), py::call_guard<py::gil_scoped_release>())
+        .def_static("kinetic", &IntBox::kinetic, py::arg("resources"), py::arg("pairlist"),
+                    py::arg("T") = std::shared_ptr<Tensor>(), py::call_guard<py::gil_scoped_release>())
+        .def_static("potential", &IntBox::potential, py::arg("resources"), py::arg("ewald"), py::arg("pairlist"),
+                    py::arg("xyzc"), py::arg("V") = std::shared_ptr<Tensor>(), py::call_guard<py::gil_scoped_release>())
         .def_static("is_hip_ready", [](std::shared_ptr<ResourceList> r, std::shared_ptr<PairList> a,
                                        std::shared_ptr<PairList> b) { return IntBox::is_hip_ready(r, a, b); });
 

@@ -117,6 +117,59 @@ std::shared_ptr<Tensor> IntBox::exchange(const std::shared_ptr<ResourceList>& re
     return finish(b1, b3, Kc, K13);
 }
 
+namespace {
+enum class OneE { S, T, V };
+
+std::shared_ptr<Tensor> one_electron(OneE kind, const char* who, const std::shared_ptr<ResourceList>& resources,
+                                     const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl,
+                                     const std::shared_ptr<Tensor>& xyzc, const std::shared_ptr<Tensor>& Out)
+{
+    auto b1 = pl->basis1(), b2 = pl->basis2();
+    if (Out) Out->shape_error({b1->nao(), b2->nao()});
+    if (xyzc && (xyzc->shape().size() != 2 || xyzc->shape()[1] != 4))
+        throw std::runtime_error(std::string(who) + ": xyzc must be an (nA, 4) Tensor");
+    require_ready(who, resources, pl, pl);
+    const size_t n = b1->ncart() * b2->ncart();
+    DevBuf dM(std::max<size_t>(n, 1) * sizeof(double));
+    check(lsjk_memset_zero(dM.p, std::max<size_t>(n, 1) * sizeof(double)));
+    lsjk_pairlist* d = pl->device_handle();
+    if (kind == OneE::S) {
+        check(lsjk_overlap(d, dM.d()));
+    } else if (kind == OneE::T) {
+        check(lsjk_kinetic(d, dM.d()));
+    } else {
+        std::vector<double> s, o;
+        ewald_arrays(ewald, s, o);
+        check(lsjk_potential(d, (int)s.size(), s.data(), o.data(), (int)xyzc->shape()[0], xyzc->data().data(),
+                             dM.d()));
+    }
+    auto Mc = std::make_shared<Tensor>(std::vector<size_t>{b1->ncart(), b2->ncart()}, "M");
+    check(lsjk_memcpy_dtoh(Mc->data().data(), dM.p, n * sizeof(double)));
+    return finish(b1, b2, Mc, Out);
+}
+}  // namespace
+
+std::shared_ptr<Tensor> IntBox::overlap(const std::shared_ptr<ResourceList>& resources,
+                                        const std::shared_ptr<PairList>& pairlist, const std::shared_ptr<Tensor>& S)
+{
+    return one_electron(OneE::S, "IntBox::overlap", resources, nullptr, pairlist, nullptr, S);
+}
+
+std::shared_ptr<Tensor> IntBox::kinetic(const std::shared_ptr<ResourceList>& resources,
+                                        const std::shared_ptr<PairList>& pairlist, const std::shared_ptr<Tensor>& T)
+{
+    return one_electron(OneE::T, "IntBox::kinetic", resources, nullptr, pairlist, nullptr, T);
+}
+
+std::shared_ptr<Tensor> IntBox::potential(const std::shared_ptr<ResourceList>& resources,
+                                          const std::shared_ptr<Ewald>& ewald,
+                                          const std::shared_ptr<PairList>& pairlist,
+                                          const std::shared_ptr<Tensor>& xyzc, const std::shared_ptr<Tensor>& V)
+{
+    if (!xyzc) throw std::runtime_error("IntBox::potential: xyzc is required");
+    return one_electron(OneE::V, "IntBox::potential", resources, ewald, pairlist, xyzc, V);
+}
+
 void IntBox::coulomb_device(const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl12,
                             const std::shared_ptr<PairList>& pl34, const double* D34_dev, float thresp, float thredp,
                             double* J12_dev)

@@ -233,6 +233,20 @@ struct IntBox {
     static void exchange_device(const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pairlist12,
                                 const std::shared_ptr<PairList>& pairlist34, const double* D24_dev, bool D24symm,
                                 float thresp, float thredp, double* K13_dev);
+    // One-electron integrals (reference intbox.hpp:225-406, 620-652; dispatch intbox.cpp:30-40,
+    // 183-210): S += <a|b>, T += <a|-1/2 nabla^2|b>, V += sum_A <a|O(r_1A)|b> (-c_A) with xyzc an
+    // (nA, 4) Tensor of (x, y, z, c) rows.
+    static std::shared_ptr<Tensor> overlap(const std::shared_ptr<ResourceList>& resources,
+                                           const std::shared_ptr<PairList>& pairlist,
+                                           const std::shared_ptr<Tensor>& S = std::shared_ptr<Tensor>());
+    static std::shared_ptr<Tensor> kinetic(const std::shared_ptr<ResourceList>& resources,
+                                           const std::shared_ptr<PairList>& pairlist,
+                                           const std::shared_ptr<Tensor>& T = std::shared_ptr<Tensor>());
+    static std::shared_ptr<Tensor> potential(const std::shared_ptr<ResourceList>& resources,
+                                             const std::shared_ptr<Ewald>& ewald,
+                                             const std::shared_ptr<PairList>& pairlist,
+                                             const std::shared_ptr<Tensor>& xyzc,
+                                             const std::shared_ptr<Tensor>& V = std::shared_ptr<Tensor>());
     // Backend predicate, the counterpart of is_terachem_ready (reference intbox.cpp:12-26).
     static bool is_hip_ready(const std::shared_ptr<ResourceList>& resources,
                              const std::shared_ptr<PairList>& pairlist12, const std::shared_ptr<PairList>& pairlist34,

@@ -987,3 +987,160 @@ int oracle_exchange(const ob_basis* b1, const ob_basis* b2, const ob_basis* b3, 
     if (!same_list) free_kblocks(kb34, nkb34);
     return 0;
 }
+
+/* ---------------------------------------------------------------- one-electron (S, T, V) */
+
+/* IntBox::overlapCPU and IntBox::kineticCPU (src/cpp/intbox/cpu/overlap.cpp:14-147,
+ * src/cpp/intbox/cpu/kinetic.cpp:14-175), Cartesian representation, accumulated.  1-D integrals by
+ * the Gauss-Hermite rule with (L1+L2)/2+1 (S) and (L1+L2+2)/2+1 (T) nodes in x = t/sqrt(p) about
+ * the pair centre; T = 1/2 sum_axis <d a|d b> with d x^l = l x^(l-1) - 2 e x^(l+1).  A symmetric
+ * list adds the transposed block for pairs of distinct primitives.  S or T may be NULL. */
+int oracle_overlap_kinetic(const ob_basis* b1, const ob_basis* b2, const ob_pairlist* pl, double* S, double* T)
+{
+    size_t nc2 = b2->ncart;
+    for (int cls = 0; cls < pl->ncls; cls++) {
+        int L1 = pl->cls_L1[cls], L2 = pl->cls_L2[cls];
+        int V1 = L1 + 1, V2 = L2 + 1;
+        int ngs = (L1 + L2) / 2 + 1, ngt = (L1 + L2 + 2) / 2 + 1;
+        int s0 = pl->cls_start[cls], s1 = pl->cls_start[cls + 1];
+#pragma omp parallel for schedule(dynamic)
+        for (int k = s0; k < s1; k++) {
+            int i1 = pl->p1[k], i2 = pl->p2[k];
+            const double* r1 = b1->xyz + 3 * i1;
+            const double* r2 = b2->xyz + 3 * i2;
+            double e1 = b1->e[i1], e2 = b2->e[i2], p = e1 + e2, pm12 = pow(p, -0.5);
+            double R2 = 0.0, px1[3], px2[3];
+            for (int d = 0; d < 3; d++) {
+                double P = (e1 * r1[d] + e2 * r2[d]) / p;
+                px1[d] = P - r1[d];
+                px2[d] = P - r2[d];
+                R2 += (r1[d] - r2[d]) * (r1[d] - r2[d]);
+            }
+            double K12 = b1->c[i1] * b2->c[i2] * exp(-e1 * e2 / p * R2);
+            double S1[3][(OB_MAXL + 1) * (OB_MAXL + 1)], T1[3][(OB_MAXL + 1) * (OB_MAXL + 1)];
+            memset(S1, 0, sizeof(S1));
+            memset(T1, 0, sizeof(T1));
+            for (int pass = 0; pass < 2; pass++) {
+                int ng = pass ? ngt : ngs;
+                const double* tg = lsjk_gh_x + ng * (ng - 1) / 2;
+                const double* wg = lsjk_gh_w + ng * (ng - 1) / 2;
+                for (int v = 0; v < ng; v++) {
+                    double xe = tg[v] * pm12, we = wg[v] * pm12;
+                    for (int d = 0; d < 3; d++) {
+                        double x1 = xe + px1[d], x2 = xe + px2[d];
+                        for (int i = 0; i < V1; i++)
+                            for (int j = 0; j < V2; j++) {
+                                if (pass == 0) {
+                                    S1[d][i * V2 + j] += we * pow(x1, i) * pow(x2, j);
+                                } else {
+                                    double d1 = (i > 0 ? i * pow(x1, i - 1) : 0.0) - 2.0 * e1 * pow(x1, i + 1);
+                                    double d2 = (j > 0 ? j * pow(x2, j - 1) : 0.0) - 2.0 * e2 * pow(x2, j + 1);
+                                    T1[d][i * V2 + j] += we * d1 * d2;
+                                }
+                            }
+                    }
+                }
+            }
+            int off1 = b1->cart[i1], off2 = b2->cart[i2];
+            int c1 = 0;
+            for (int a = 0; a <= L1; a++)
+                for (int b = 0; b <= a; b++, c1++) {
+                    int l1 = L1 - a, m1 = a - b, n1 = b;
+                    int c2 = 0;
+                    for (int a2 = 0; a2 <= L2; a2++)
+                        for (int b2 = 0; b2 <= a2; b2++, c2++) {
+                            int l2 = L2 - a2, m2 = a2 - b2, n2 = b2;
+                            double sx = S1[0][l1 * V2 + l2], sy = S1[1][m1 * V2 + m2], sz = S1[2][n1 * V2 + n2];
+                            double sv = K12 * sx * sy * sz;
+                            double tv = K12 * 0.5 *
+                                        (T1[0][l1 * V2 + l2] * sy * sz + sx * T1[1][m1 * V2 + m2] * sz +
+                                         sx * sy * T1[2][n1 * V2 + n2]);
+                            size_t ij = (size_t)(c1 + off1) * nc2 + (c2 + off2);
+                            size_t ji = (size_t)(c2 + off2) * nc2 + (c1 + off1);
+                            int mirror = pl->symmetric && i1 != i2;
+                            if (S) {
+#pragma omp atomic
+                                S[ij] += sv;
+                                if (mirror) {
+#pragma omp atomic
+                                    S[ji] += sv;
+                                }
+                            }
+                            if (T) {
+#pragma omp atomic
+                                T[ij] += tv;
+                                if (mirror) {
+#pragma omp atomic
+                                    T[ji] += tv;
+                                }
+                            }
+                        }
+                }
+        }
+    }
+    return 0;
+}
+
+/* IntBox::potentialCPU (src/cpp/intbox/cpu/potential.cpp:14-147): V += sum_A int a b O(r_1A) (-c_A)
+ * for point charges xyzc (nA x 4: x, y, z, c), Cartesian, accumulated.  Per pair the Hermite
+ * integrals R_t(p d^2, P - A) by Rys quadrature with pref = alpha 2 pi / p (x sqrt(d^2) for erf
+ * operators, d^2 = omega^2/(p + omega^2)), then the bra Hermite -> Cartesian transform. */
+int oracle_potential(const ob_basis* b1, const ob_basis* b2, const ob_pairlist* pl, int nterm, const double* scales,
+                     const double* omegas, int nA, const double* xyzc, double* V)
+{
+    for (int cls = 0; cls < pl->ncls; cls++) {
+        int L = pl->cls_L1[cls] + pl->cls_L2[cls];
+        int H = hermite_count(L), nrys = L / 2 + 1;
+        int tx[35], ty[35], tz[35];
+        hermite_tuples(L, tx, ty, tz);
+        int s0 = pl->cls_start[cls], s1 = pl->cls_start[cls + 1];
+        double* dh = (double*)calloc((size_t)(pl->npair > 0 ? pl->npair : 1) * H, sizeof(double));
+#pragma omp parallel for schedule(dynamic)
+        for (int k = s0; k < s1; k++) {
+            int i1 = pl->p1[k], i2 = pl->p2[k];
+            const double* r1 = b1->xyz + 3 * i1;
+            const double* r2 = b2->xyz + 3 * i2;
+            double e1 = b1->e[i1], e2 = b2->e[i2], p = e1 + e2;
+            double P[3];
+            for (int d = 0; d < 3; d++) P[d] = (e1 * r1[d] + e2 * r2[d]) / p;
+            double* acc = dh + (size_t)k * H;
+            for (int it = 0; it < nterm; it++) {
+                double alpha = scales[it], omega = omegas[it];
+                if (alpha == 0.0 || omega == 0.0) continue;
+                double pref = alpha * 2.0 * M_PI / p, d2 = 1.0;
+                if (omega != -1.0) {
+                    d2 = omega * omega / (p + omega * omega);
+                    pref *= sqrt(d2);
+                }
+                for (int A = 0; A < nA; A++) {
+                    double X = P[0] - xyzc[4 * A], Y = P[1] - xyzc[4 * A + 1], Z = P[2] - xyzc[4 * A + 2];
+                    double cA = xyzc[4 * A + 3];
+                    double Tv = p * d2 * (X * X + Y * Y + Z * Z);
+                    double xr[9], wr[9];
+                    rys_eval(nrys, Tv, xr, wr);
+                    for (int i = 0; i < nrys; i++) {
+                        double U = -2.0 * p * d2 * xr[i];
+                        double Hx[2 * OB_MAXL + 1], Hy[2 * OB_MAXL + 1], Hz[2 * OB_MAXL + 1];
+                        Hx[0] = -pref * wr[i] * cA;
+                        Hy[0] = 1.0;
+                        Hz[0] = 1.0;
+                        if (L > 0) {
+                            Hx[1] = U * X * Hx[0];
+                            Hy[1] = U * Y;
+                            Hz[1] = U * Z;
+                        }
+                        for (int u = 1; u < L; u++) {
+                            Hx[u + 1] = U * (X * Hx[u] + u * Hx[u - 1]);
+                            Hy[u + 1] = U * (Y * Hy[u] + u * Hy[u - 1]);
+                            Hz[u + 1] = U * (Z * Hz[u] + u * Hz[u - 1]);
+                        }
+                        for (int t = 0; t < H; t++) acc[t] += Hx[tx[t]] * Hy[ty[t]] * Hz[tz[t]];
+                    }
+                }
+            }
+        }
+        hermite_transform(pl, cls, b1, b2, dh, V, -1, 0);
+        free(dh);
+    }
+    return 0;
+}

@@ -62,6 +62,11 @@ def lib():
             ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
             ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_rys.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_overlap_kinetic.argtypes = [ctypes.POINTER(_CBasis)] * 2 + [ctypes.POINTER(_CPairList),
+                                                                            ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_potential.argtypes = [ctypes.POINTER(_CBasis)] * 2 + [
+            ctypes.POINTER(_CPairList), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+            ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -308,3 +313,42 @@ def rys(n, T):
     w = np.zeros((T.size, n))
     lib().oracle_rys(n, T.size, T.ctypes.data, x.ctypes.data, w.ctypes.data)
     return x, w
+
+
+def _accumulate(b1, b2, Mc, M):
+    out = b1.transform() @ Mc @ b2.transform().T
+    if M is not None:
+        M = np.asarray(M)
+        if M.shape != (b1.nao, b2.nao):
+            raise RuntimeError("Tensor: shape error")
+        out = M + out
+    return out
+
+
+def overlap(pl, S=None):
+    """IntBox::overlapCPU (src/cpp/intbox/cpu/overlap.cpp:14-147): S += <a|b>."""
+    b1, b2 = pl.b1, pl.b2
+    Sc = np.zeros((b1.ncart, b2.ncart))
+    lib().oracle_overlap_kinetic(ctypes.byref(b1.cstruct), ctypes.byref(b2.cstruct), ctypes.byref(pl.c),
+                                 Sc.ctypes.data, None)
+    return _accumulate(b1, b2, Sc, S)
+
+
+def kinetic(pl, T=None):
+    """IntBox::kineticCPU (src/cpp/intbox/cpu/kinetic.cpp:14-175): T += <a|-1/2 nabla^2|b>."""
+    b1, b2 = pl.b1, pl.b2
+    Tc = np.zeros((b1.ncart, b2.ncart))
+    lib().oracle_overlap_kinetic(ctypes.byref(b1.cstruct), ctypes.byref(b2.cstruct), ctypes.byref(pl.c),
+                                 None, Tc.ctypes.data)
+    return _accumulate(b1, b2, Tc, T)
+
+
+def potential(ewald, pl, xyzc, V=None):
+    """IntBox::potentialCPU (src/cpp/intbox/cpu/potential.cpp:14-147): V += sum_A <a|O(r_A)|b> (-c_A)."""
+    b1, b2 = pl.b1, pl.b2
+    xyzc = np.ascontiguousarray(xyzc, dtype=np.float64).reshape(-1, 4)
+    sc, om = _ewald_arrays(ewald)
+    Vc = np.zeros((b1.ncart, b2.ncart))
+    lib().oracle_potential(ctypes.byref(b1.cstruct), ctypes.byref(b2.cstruct), ctypes.byref(pl.c), len(sc),
+                           sc.ctypes.data, om.ctypes.data, xyzc.shape[0], xyzc.ctypes.data, Vc.ctypes.data)
+    return _accumulate(b1, b2, Vc, V)

@@ -62,6 +62,22 @@ def prop_cases(backend, basname, symmetric):
     return out
 
 
+def one_electron_cases(backend, basname, symmetric):
+    """S, T, V of prop_int.py:53-65 (each computed twice to test accumulation; tol 1e-9; V from the
+    nuclear charges mol.xyzZ with the bare Coulomb operator)."""
+    mol = molecule("h2o")
+    bas = backend.make_basis(mol, basname)
+    z = np.load(os.path.join(DATA, "prop_h2o_%s.npz" % basname))
+    pl = backend.make_pairlist(bas, bas, symmetric, 1.0e-14)
+    S = backend.overlap(pl)
+    S = backend.overlap(pl, S)
+    T = backend.kinetic(pl)
+    T = backend.kinetic(pl, T)
+    V = backend.potential(COULOMB, pl, mol.xyzZ)
+    V = backend.potential(COULOMB, pl, mol.xyzZ, V)
+    return [("S", S, 2.0 * z["S"], 1.0e-9), ("T", T, 2.0 * z["T"], 1.0e-9), ("V", V, 2.0 * z["V"], 1.0e-9)]
+
+
 def ewald_cases(backend, basname, symmetric):
     mol = molecule("h2o")
     bas = backend.make_basis(mol, basname)
@@ -198,6 +214,15 @@ class OracleBackend:
     def exchange(self, ew, pl12, pl34, D, symm, thresp, K=None):
         return self.O.exchange(ew, pl12, pl34, D, symm, thresp, K)
 
+    def overlap(self, pl, S=None):
+        return self.O.overlap(pl, S)
+
+    def kinetic(self, pl, T=None):
+        return self.O.kinetic(pl, T)
+
+    def potential(self, ew, pl, xyzc, V=None):
+        return self.O.potential(ew, pl, xyzc, V)
+
 
 class ProductBackend:
     """lightspeed_amd's IntBox (HIP path)."""
@@ -226,4 +251,18 @@ class ProductBackend:
         ls = self.ls
         out = ls.IntBox.exchange(self.res, ls.Ewald(*ew), pl12, pl34, ls.Tensor.array(np.ascontiguousarray(D)),
                                  symm, thresp, thresp, None if K is None else ls.Tensor.array(K))
+        return np.array(out.np)
+
+    def overlap(self, pl, S=None):
+        ls = self.ls
+        return np.array(ls.IntBox.overlap(self.res, pl, None if S is None else ls.Tensor.array(S)).np)
+
+    def kinetic(self, pl, T=None):
+        ls = self.ls
+        return np.array(ls.IntBox.kinetic(self.res, pl, None if T is None else ls.Tensor.array(T)).np)
+
+    def potential(self, ew, pl, xyzc, V=None):
+        ls = self.ls
+        out = ls.IntBox.potential(self.res, ls.Ewald(*ew), pl, ls.Tensor.array(np.ascontiguousarray(xyzc)),
+                                  None if V is None else ls.Tensor.array(V))
         return np.array(out.np)

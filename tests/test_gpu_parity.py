@@ -30,6 +30,14 @@ def test_prop_fixtures_gpu(product_backend, basname, symmetric):
 
 @pytest.mark.parametrize("basname", FC.BASES)
 @pytest.mark.parametrize("symmetric", [True, False])
+def test_one_electron_fixtures_gpu(product_backend, basname, symmetric):
+    _say("one-electron %s sym=%s" % (basname, symmetric))
+    for key, val, ref, tol in FC.one_electron_cases(product_backend, basname, symmetric):
+        assert np.abs(val - ref).max() < tol, key
+
+
+@pytest.mark.parametrize("basname", FC.BASES)
+@pytest.mark.parametrize("symmetric", [True, False])
 def test_ewald_fixtures_gpu(product_backend, basname, symmetric):
     _say("ewald %s sym=%s" % (basname, symmetric))
     for key, val, ref, tol in FC.ewald_cases(product_backend, basname, symmetric):

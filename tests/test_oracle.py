@@ -19,6 +19,13 @@ def test_prop_fixtures(oracle_backend, basname, symmetric):
 
 @pytest.mark.parametrize("basname", FC.BASES)
 @pytest.mark.parametrize("symmetric", [True, False])
+def test_one_electron_fixtures(oracle_backend, basname, symmetric):
+    for key, val, ref, tol in FC.one_electron_cases(oracle_backend, basname, symmetric):
+        assert np.abs(val - ref).max() < tol, key
+
+
+@pytest.mark.parametrize("basname", FC.BASES)
+@pytest.mark.parametrize("symmetric", [True, False])
 def test_ewald_fixtures(oracle_backend, basname, symmetric):
     for key, val, ref, tol in FC.ewald_cases(oracle_backend, basname, symmetric):
         assert np.abs(val - ref).max() < tol, key

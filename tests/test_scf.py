@@ -1,0 +1,90 @@
+"""RHF harness (one-electron integrals + J/K in an SCF) against the reference's known answer.
+
+tests/coverage/rhf.py:607-646 of the reference quotes the PSI4 RHF energy of H2O/STO-3G
+(tests/golden/reference/h2o.xyz) as -74.9420799265908784 Eh with tolerance 1e-11.  The harness
+reaches it with the CPU oracle's integrals (pins the oracle's S/T/V/J/K chain and the driver) and
+with the HIP integrals (gpu).  The mixed FP32/FP64 route is checked for |dE_SCF| <= 1e-6 Eh over
+10 iterations from the core guess against the FP64 route.
+"""
+import numpy as np
+import pytest
+
+import fixture_cases as FC
+from lightspeed_amd import scf
+
+E_REF = -74.9420799265908784
+
+
+class OracleIntegrals:
+    """The same interface as scf.Integrals, from the CPU oracle (test infrastructure)."""
+
+    def __init__(self, O, molecule, basis_name):
+        from lightspeed_amd import loaders
+
+        class _B:
+            pass
+        b = _B()
+        b.primitives = loaders.basis_records_from_gbs_file(molecule, basis_name)[1]
+        self.O = O
+        self.b = O.OBasis(b)
+        self.pl = O.OPairList(self.b, self.b, True, 1.0e-14)
+
+    def overlap(self):
+        return self.O.overlap(self.pl)
+
+    def kinetic(self):
+        return self.O.kinetic(self.pl)
+
+    def potential(self, xyzc):
+        return self.O.potential(FC.COULOMB, self.pl, xyzc)
+
+    def coulomb(self, D, thresp, thredp):
+        return self.O.coulomb(FC.COULOMB, self.pl, self.pl, D, thresp)
+
+    def exchange(self, D, thresp, thredp):
+        return self.O.exchange(FC.COULOMB, self.pl, self.pl, D, True, thresp)
+
+
+def test_rhf_h2o_sto3g_oracle(oracle_backend):
+    mol = FC.molecule("h2o")
+    hf = scf.RHF(mol, OracleIntegrals(oracle_backend.O, mol, "sto-3g"))
+    E = hf.run()
+    assert hf.converged
+    assert abs(E - E_REF) < 1.0e-9, E
+
+
+def test_nuclear_repulsion_closed_form():
+    xyzZ = np.array([[0.0, 0.0, 0.0, 8.0], [0.0, 0.0, 2.0, 1.0], [0.0, 1.5, 0.0, 1.0]])
+    ref = 8.0 / 2.0 + 8.0 / 1.5 + 1.0 / np.sqrt(4.0 + 2.25)
+    assert abs(scf.nuclear_repulsion(xyzZ) - ref) < 1e-14
+
+
+@pytest.mark.gpu
+def test_rhf_h2o_sto3g_gpu(product_backend):
+    ls = product_backend.ls
+    mol = FC.molecule("h2o")
+    bas = ls.Basis.from_gbs_file(mol, "sto-3g")
+    hf = scf.RHF(mol, scf.Integrals(bas))
+    E = hf.run()
+    print("RHF H2O/STO-3G: E = %.12f after %d iterations" % (E, hf.iterations), flush=True)
+    assert hf.converged
+    assert abs(E - E_REF) < 1.0e-9, E
+
+
+@pytest.mark.gpu
+def test_rhf_mixed_route_energy_error(product_backend):
+    """(H2O)10/6-31G, 10 iterations from the core guess: the mixed route (thredp = 1e-6) stays within
+    1e-6 Eh of the FP64 route at every iteration."""
+    ls = product_backend.ls
+    from lightspeed_amd import workloads as W
+    mol, bname, _ = W.config("h2o10_631g")
+    bas = ls.Basis.from_gbs_file(mol, bname)
+    ints = scf.Integrals(bas, 1.0e-12)
+    e64 = scf.RHF(mol, ints, thresp=1.0e-12, thredp=1.0e-12, fixed_iterations=10)
+    e64.run()
+    emx = scf.RHF(mol, ints, thresp=1.0e-12, thredp=1.0e-6, fixed_iterations=10)
+    emx.run()
+    dE = max(abs(a[1] - b[1]) for a, b in zip(e64.history, emx.history))
+    print("(H2O)10/6-31G: E10 = %.10f, max |dE| mixed vs fp64 over 10 iterations = %.3e" % (e64.energy, dE),
+          flush=True)
+    assert dE <= 1.0e-6
