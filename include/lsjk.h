@@ -63,6 +63,9 @@ int lsjk_device_count(int* count);
 int lsjk_basis_create(int nprim, const double* c, const double* e, const double* xyz, const int* L,
                       const int* cart, int ncart, lsjk_basis** out);
 int lsjk_basis_destroy(lsjk_basis* b);
+/* Atom index of each primitive (reference Primitive::atomIdx, basis.hpp:136-250), natom atoms:
+ * needed by the gradient functions, which accumulate per-atom (natom x 3) results. */
+int lsjk_basis_set_atoms(lsjk_basis* b, int natom, const int* atom);
 
 /* A Schwarz pair list (reference PairList, pair_list.hpp:25-220) over (b1, b2): npair primitive
  * pairs (p1 index into b1, p2 into b2, float bound), grouped by (L1, L2) class in L1-major order
@@ -94,6 +97,14 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
 int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
                   int nterm, const double* scales, const double* omegas, float thresp, float thredp,
                   double* K13_dev);
+
+/* Coulomb gradient, bra-centre part (the reference's coulombGrad, intbox.hpp:954-1008 and
+ * cpu/coulomb_grad.cpp:19-563, is the sum of this with the densities in both roles):
+ *   G_dev (natom x 3) += sum_{ab,cd} Dbra_ab Dket_cd d/dR [bra centres a, b] (ab|O|cd),
+ * over a symmetric list (both sides), Cartesian densities, screening as lsjk_coulomb with
+ * Dket (thredp: FP64 route).  Needs lsjk_basis_set_atoms. */
+int lsjk_coulomb_grad(const lsjk_pairlist* pl, const double* Dbra_dev, const double* Dket_dev, int nterm,
+                      const double* scales, const double* omegas, float thresp, float thredp, double* G_dev);
 
 /* One-electron integrals of a pair list, accumulated into ncart1 x ncart2 device matrices
  * (Cartesian, as lsjk_coulomb).  A symmetric list fills both triangles.

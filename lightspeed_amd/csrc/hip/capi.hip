@@ -799,10 +799,25 @@ int lsjk_basis_create(int nprim, const double* c, const double* e, const double*
     });
 }
 
+int lsjk_basis_set_atoms(lsjk_basis* b, int natom, const int* atom)
+{
+    return guarded([&] {
+        require_init();
+        if (!b || natom < 0 || (b->nprim > 0 && !atom)) throw LsjkError(LSJK_E_INVALID, "lsjk_basis_set_atoms: null argument");
+        std::vector<int> a(atom, atom + b->nprim);
+        for (int v : a)
+            if (v < 0 || v >= natom) throw LsjkError(LSJK_E_INVALID, "lsjk_basis_set_atoms: atom index out of range");
+        dev_free(b->d_atom);
+        b->d_atom = dev_upload(a);
+        b->natom = natom;
+    });
+}
+
 int lsjk_basis_destroy(lsjk_basis* b)
 {
     return guarded([&] {
         if (!b) return;
+        dev_free(b->d_atom);
         dev_free(b->d_c);
         dev_free(b->d_e);
         dev_free(b->d_x);
@@ -1123,6 +1138,106 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
         HIPCHK(hipEventRecord(S.ev[2], st));
         HIPCHK(hipEventRecord(S.ev[3], st));
         finish_call(S.counting);
+    });
+}
+
+int lsjk_coulomb_grad(const lsjk_pairlist* pl, const double* Dbra_dev, const double* Dket_dev, int nterm,
+                      const double* scales, const double* omegas, float thresp, float thredp, double* G_dev)
+{
+    (void)thredp;  // FP64 route
+    return guarded([&] {
+        using namespace lsjk;
+        require_init();
+        if (!pl || !Dbra_dev || !Dket_dev || !G_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_coulomb_grad: null argument");
+        if (!pl->symmetric || pl->b1 != pl->b2)
+            throw LsjkError(LSJK_E_INVALID, "lsjk_coulomb_grad: the pair list must be symmetric");
+        if (!pl->b1->d_atom) throw LsjkError(LSJK_E_INVALID, "lsjk_coulomb_grad: basis atoms not set");
+        Ewald ew;
+        check_ewald(nterm, scales, omegas, ew);
+        hipStream_t st = S.stream;
+        const Basis* b = pl->b1;
+        // potential buffer at bra order L + 1: group L holds npair(L) x H(L + 1) doubles
+        size_t hoff2[2 * kMaxL + 2] = {0};
+        for (int L = 0; L <= 2 * kMaxL; L++)
+            hoff2[L + 1] = hoff2[L] + (size_t)(pl->hstart[L + 1] - pl->hstart[L]) * nherm(L + 1);
+        float* kb = nullptr;
+        double* kd = nullptr;
+        double* vp = nullptr;
+        Workspace wsp;
+        wsp.add(&kb, (size_t)pl->npair);
+        wsp.add(&kd, pl->hoff[2 * kMaxL + 1]);
+        wsp.add(&vp, hoff2[2 * kMaxL + 1]);
+        wsp.commit();
+        JSegments seg = get_jtasks(pl, pl);
+        launch_fill(S.d_sc, sizeof(CallScalars), st);
+        launch_fill(vp, std::max<size_t>(1, hoff2[2 * kMaxL + 1]) * sizeof(double), st);
+        JPrepArgs pa{};
+        pa.p1 = pl->d_p1;
+        pa.p2 = pl->d_p2;
+        pa.bound = pl->d_bound;
+        pa.hpos = pl->d_hpos;
+        pa.K = pl->d_K;
+        pa.eA = b->d_e; pa.xA = b->d_x; pa.yA = b->d_y; pa.zA = b->d_z; pa.cartA = b->d_cart;
+        pa.eB = b->d_e; pa.xB = b->d_x; pa.yB = b->d_y; pa.zB = b->d_z; pa.cartB = b->d_cart;
+        pa.D = Dket_dev;
+        pa.ldD = b->ncart;
+        pa.symmetric = 1;
+        pa.thre_ket = thresp / pl->maxbound;
+        pa.kb = kb;
+        pa.kd = kd;
+        pa.sc = S.d_sc;
+        for (size_t c = 0; c + 1 < pl->cls_start.size(); c++) {
+            int L = pl->cls_L1[c] + pl->cls_L2[c];
+            launch_j_ket_prep(pl->cls_L1[c], pl->cls_L2[c], pa, pl->cls_start[c], pl->cls_start[c + 1] - pl->cls_start[c],
+                              pl->hstart[L], pl->hoff[L], st);
+        }
+        if (ew.n > 0) {
+            for (int L12 = 0; L12 <= 2 * kMaxL; L12++)
+                for (int L34 = 0; L34 <= 2 * kMaxL; L34++) {
+                    if (seg.cnt[L12][L34] == 0) continue;
+                    JQuartetArgs qa{};
+                    qa.rys = S.rys;
+                    qa.ew = ew;
+                    qa.thresp = thresp;
+                    qa.thredp = thresp;
+                    qa.bra_b = pl->d_hb;
+                    qa.bra_geom = pl->d_hgeom;
+                    qa.bra_n = pl->npair;
+                    qa.bra_hstart = pl->hstart[L12];
+                    qa.jp = vp + hoff2[L12];
+                    qa.ket_b = kb;
+                    qa.ket_static_b = pl->d_hb;
+                    qa.ket_geom = pl->d_hgeom;
+                    qa.ket_n = pl->npair;
+                    qa.ket_hstart = pl->hstart[L34];
+                    qa.ket_d = kd + pl->hoff[L34];
+                    qa.sc = S.d_sc;
+                    launch_j_quartets_grad(L12 + 1, L34, qa, (const JTask*)seg.d_tasks + seg.off[L12][L34],
+                                           seg.cnt[L12][L34], &S.d_sc->queue[jcount_index(L12, L34)], st);
+                }
+            JGradArgs ga{};
+            ga.p1 = pl->d_p1;
+            ga.p2 = pl->d_p2;
+            ga.bound = pl->d_bound;
+            ga.hpos = pl->d_hpos;
+            ga.K = pl->d_K;
+            ga.eA = b->d_e; ga.xA = b->d_x; ga.yA = b->d_y; ga.zA = b->d_z; ga.cartA = b->d_cart; ga.atomA = b->d_atom;
+            ga.eB = b->d_e; ga.xB = b->d_x; ga.yB = b->d_y; ga.zB = b->d_z; ga.cartB = b->d_cart; ga.atomB = b->d_atom;
+            ga.vp = vp;
+            ga.D = Dbra_dev;
+            ga.ldD = b->ncart;
+            ga.symmetric = 1;
+            ga.thresp = thresp;
+            ga.G = G_dev;
+            ga.sc = S.d_sc;
+            for (size_t c = 0; c + 1 < pl->cls_start.size(); c++) {
+                int L = pl->cls_L1[c] + pl->cls_L2[c];
+                launch_j_grad_scatter(pl->cls_L1[c], pl->cls_L2[c], ga, pl->cls_start[c],
+                                      pl->cls_start[c + 1] - pl->cls_start[c], pl->hstart[L], hoff2[L], st);
+            }
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
     });
 }
 

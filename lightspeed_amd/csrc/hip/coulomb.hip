@@ -369,6 +369,92 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
     }
 }
 
+// ------------------------------------------------------------------ gradient (bra centres)
+// dE/dA_x = sum_ab Dsym_ab K_ab sum_t [2 alpha E^{a+1x,b}_tx - a_x E^{a-1x,b}_tx] E_ty E_tz V_t, the
+// derivative of the primitive, d phi_a / dA_x = 2 alpha phi_{a+1x} - a_x phi_{a-1x}, expanded in
+// Hermite Gaussians at P (t <= L12 + 1); likewise dE/dB with beta and b.  Dsym is the bra density
+// block symmetrised as in the J bra transform (both triangles, 1/2 for a primitive with itself).
+// The sum over both bra and ket roles gives the full gradient (reference coulomb_grad.cpp:19-563).
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void j_grad_scatter(JGradArgs a, int k0, int nk, int hstart, size_t hoff)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), L = LA + LB, H = nherm(L + 1);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nk) return;
+    const int k = k0 + i;
+    const float max34 = __uint_as_float(a.sc->max34_bits);
+    const float thre_bra = a.thresp / max34;
+    if (!(a.bound[k] >= thre_bra)) return;
+    const int ia = a.p1[k], ib = a.p2[k];
+    const double* vp = a.vp + hoff + (size_t)(a.hpos[k] - hstart) * H;
+    double V[H];
+#pragma unroll
+    for (int t = 0; t < H; t++) V[t] = vp[t];
+    const double eA = a.eA[ia], eB = a.eB[ib];
+    const double p = eA + eB, ip = 1.0 / p;
+    const double Ax = a.xA[ia], Ay = a.yA[ia], Az = a.zA[ia];
+    const double Bx = a.xB[ib], By = a.yB[ib], Bz = a.zB[ib];
+    const double Px = (eA * Ax + eB * Bx) * ip, Py = (eA * Ay + eB * By) * ip, Pz = (eA * Az + eB * Bz) * ip;
+    HermE<LA + 1, LB + 1> Ex, Ey, Ez;
+    hermite_E<LA + 1, LB + 1>(0.5 * ip, Px - Ax, Px - Bx, Ex);
+    hermite_E<LA + 1, LB + 1>(0.5 * ip, Py - Ay, Py - By, Ey);
+    hermite_E<LA + 1, LB + 1>(0.5 * ip, Pz - Az, Pz - Bz, Ez);
+    const double scale = (a.symmetric && ia == ib) ? 0.5 : 1.0;
+    const int ca = a.cartA[ia], cb = a.cartB[ib];
+    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+    // one-axis factor of the shifted function: sum over t of the axis, for exponents (i1, j1)
+    static_for<NA>([&](auto P_) {
+        constexpr int pa = decltype(P_)::value;
+        constexpr int ax = cart_lx(LA, pa), ay = cart_ly(LA, pa), az = cart_lz(LA, pa);
+        static_for<NB>([&](auto Q_) {
+            constexpr int qb = decltype(Q_)::value;
+            constexpr int bx = cart_lx(LB, qb), by = cart_ly(LB, qb), bz = cart_lz(LB, qb);
+            double Dv = a.D[(size_t)(ca + pa) * a.ldD + (cb + qb)];
+            if (a.symmetric) Dv = (Dv + a.D[(size_t)(cb + qb) * a.ldD + (ca + pa)]) * scale;
+            if (Dv == 0.0) return;
+            double dA[3] = {0.0, 0.0, 0.0}, dB[3] = {0.0, 0.0, 0.0};
+            static_for<H>([&](auto T_) {
+                constexpr int t = decltype(T_)::value;
+                constexpr int tx = herm_tx(t), ty = herm_ty(t), tz = herm_tz(t);
+                const double v = V[t];
+                // plain factors (t within reach of the unshifted exponents)
+                const double ex = (tx <= ax + bx) ? Ex.v[tx][ax][bx] : 0.0;
+                const double ey = (ty <= ay + by) ? Ey.v[ty][ay][by] : 0.0;
+                const double ez = (tz <= az + bz) ? Ez.v[tz][az][bz] : 0.0;
+                // shifted factors: 2 alpha E^{a+1,b} - a E^{a-1,b} (bra A) and 2 beta E^{a,b+1} - b E^{a,b-1}
+                auto shiftA = [&](const HermE<LA + 1, LB + 1>& E, int tt, int i1, int j1) __attribute__((always_inline)) {
+                    double s = (tt <= i1 + 1 + j1) ? 2.0 * eA * E.v[tt][i1 + 1][j1] : 0.0;
+                    if (i1 > 0 && tt <= i1 - 1 + j1) s -= i1 * E.v[tt][i1 - 1][j1];
+                    return s;
+                };
+                auto shiftB = [&](const HermE<LA + 1, LB + 1>& E, int tt, int i1, int j1) __attribute__((always_inline)) {
+                    double s = (tt <= i1 + j1 + 1) ? 2.0 * eB * E.v[tt][i1][j1 + 1] : 0.0;
+                    if (j1 > 0 && tt <= i1 + j1 - 1) s -= j1 * E.v[tt][i1][j1 - 1];
+                    return s;
+                };
+                dA[0] += shiftA(Ex, tx, ax, bx) * ey * ez * v;
+                dA[1] += ex * shiftA(Ey, ty, ay, by) * ez * v;
+                dA[2] += ex * ey * shiftA(Ez, tz, az, bz) * v;
+                dB[0] += shiftB(Ex, tx, ax, bx) * ey * ez * v;
+                dB[1] += ex * shiftB(Ey, ty, ay, by) * ez * v;
+                dB[2] += ex * ey * shiftB(Ez, tz, az, bz) * v;
+            });
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                gA[d] += Dv * dA[d];
+                gB[d] += Dv * dB[d];
+            }
+        });
+    });
+    const double Kab = a.K[k];
+    const int atA = a.atomA[ia], atB = a.atomB[ib];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        atomic_add(a.G + 3 * atA + d, Kab * gA[d]);
+        atomic_add(a.G + 3 * atB + d, Kab * gB[d]);
+    }
+}
+
 // ------------------------------------------------------------------ bra scatter
 template <int LA, int LB>
 __global__ __launch_bounds__(256) void j_bra_scatter(JScatterArgs a, int k0, int nk, int hstart, size_t hoff)
@@ -448,6 +534,20 @@ void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk,
 #undef CASE
 }
 
+void launch_j_grad_scatter(int LA, int LB, const JGradArgs& a, int k0, int nk, int hstart, size_t hoff,
+                           hipStream_t s)
+{
+    if (nk <= 0) return;
+    dim3 grid((nk + 255) / 256), block(256);
+#define CASE(X, Y)                                                                                  \
+    if (LA == X && LB == Y) {                                                                       \
+        hipLaunchKernelGGL((j_grad_scatter<X, Y>), grid, block, 0, s, a, k0, nk, hstart, hoff);     \
+        return;                                                                                     \
+    }
+    CASE(0, 0) CASE(0, 1) CASE(0, 2) CASE(1, 0) CASE(1, 1) CASE(1, 2) CASE(2, 0) CASE(2, 1) CASE(2, 2)
+#undef CASE
+}
+
 template <int L12, int L34, int ROUTE>
 static void launch_jq(const JQuartetArgs& a, const JTask* tasks, int ntask, unsigned int* qctr, hipStream_t s)
 {
@@ -477,6 +577,22 @@ void launch_j_quartets(int L12, int L34, int route, const JQuartetArgs& a, const
     }
 #define ROW(X) CASE(X, 0) CASE(X, 1) CASE(X, 2) CASE(X, 3) CASE(X, 4)
     ROW(0) ROW(1) ROW(2) ROW(3) ROW(4)
+#undef ROW
+#undef CASE
+}
+
+// Bra order L12 + 1 (gradients): FP64 route only.
+void launch_j_quartets_grad(int L12p1, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask,
+                            unsigned int* qctr, hipStream_t s)
+{
+    if (ntask <= 0) return;
+#define CASE(X, Y)                                                               \
+    if (L12p1 == X && L34 == Y) {                                                \
+        launch_jq<X, Y, kRoutePlain>(a, tasks, ntask, qctr, s);                 \
+        return;                                                                  \
+    }
+#define ROW(X) CASE(X, 0) CASE(X, 1) CASE(X, 2) CASE(X, 3) CASE(X, 4)
+    ROW(1) ROW(2) ROW(3) ROW(4) ROW(5)
 #undef ROW
 #undef CASE
 }

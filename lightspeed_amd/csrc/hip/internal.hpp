@@ -21,6 +21,8 @@ struct Basis {
     std::vector<int> h_L, h_cart, h_blk, h_blk_cart, h_blk_L;
     double *d_c = nullptr, *d_e = nullptr, *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
     int *d_L = nullptr, *d_cart = nullptr, *d_blk = nullptr, *d_blk_cart = nullptr, *d_blk_L = nullptr;
+    int natom = 0;            // atoms of the basis (lsjk_basis_set_atoms; 0 = not set)
+    int* d_atom = nullptr;    // atom index of each primitive
 };
 
 // ---------------------------------------------------------------- pair list

@@ -80,6 +80,29 @@ struct JScatterArgs {
     CallScalars* sc;
 };
 
+// Coulomb gradient, bra-centre part: per bra pair, dE/dA and dE/dB from the potential V_t of the
+// ket density at bra Hermite order L12 + 1 and the symmetrised bra density block.
+struct JGradArgs {
+    const int* p1;
+    const int* p2;
+    const float* bound;
+    const int* hpos;
+    const double* K;
+    const double *eA, *xA, *yA, *zA;
+    const int *cartA, *atomA;
+    const double *eB, *xB, *yB, *zB;
+    const int *cartB, *atomB;
+    const double* vp;  // V_t, Hermite order, H(L12+1) per pair of group L12
+    const double* D;   // bra density (Cartesian)
+    int ldD;
+    int symmetric;
+    float thresp;
+    double* G;  // natom x 3
+    CallScalars* sc;
+};
+void launch_j_grad_scatter(int LA, int LB, const JGradArgs& a, int k0, int nk, int hstart, size_t hoff,
+                           hipStream_t s);
+
 // One-electron integrals (overlap, kinetic, potential) of one pair list.
 struct OneEArgs {
     const int* p1;
@@ -194,6 +217,8 @@ void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk,
                           hipStream_t s);
 void launch_j_quartets(int L12, int L34, int route, const JQuartetArgs& a, const JTask* tasks, int ntask,
                        unsigned int* qctr, hipStream_t s);
+void launch_j_quartets_grad(int L12p1, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask,
+                            unsigned int* qctr, hipStream_t s);
 
 // Work-queue counters of one call (CallScalars::queue, zeroed with the call's scalars): J classes
 // at jcount_index.

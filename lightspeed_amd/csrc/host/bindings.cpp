@@ -238,6 +238,13 @@ PYBIND11_MODULE(_lightspeed, m)
                 IntBox::exchange_device(ew, a, b, (const double*)D, D24symm, thresp, thredp, (double*)K);
             },
             py::call_guard<py::gil_scoped_release>())
+        .def_static("coulombGrad", &IntBox::coulombGrad, py::arg("resources"), py::arg("ewald"), py::arg("pairlist"),
+                    py::arg("D12"), py::arg("D34"), py::arg("thresp"), py::arg("thredp"),
+                    py::arg("G") = std::shared_ptr<Tensor>(), py::call_guard<py::gil_scoped_release>())
+        .def_static("eriJ", &IntBox::eriJ, py::arg("resources"), py::arg("ewald"), py::arg("pairlist12"),
+                    py::arg("pairlist34"), py::arg("C1"), py::arg("C2"), py::arg("C3"), py::arg("C4"),
+                    py::arg("thresp"), py::arg("thredp"), py::arg("I") = std::shared_ptr<Tensor>(),
+                    py::call_guard<py::gil_scoped_release>())
         .def_static("overlap", &IntBox::overlap, py::arg("resources"), py::arg("pairlist"),
                     py::arg("S") = std::shared_ptr<Tensor>(), py::call_guard<py::gil_scoped_release>())
         .def_static("kinetic", &IntBox::kinetic, py::arg("resources"), py::arg("pairlist"),

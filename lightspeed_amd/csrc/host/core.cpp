@@ -162,6 +162,9 @@ lsjk_basis* Basis::device_handle() const
             cart[i] = p.cartIdx();
         }
         lsjk_check(lsjk_basis_create((int)n, c.data(), e.data(), xyz.data(), L.data(), cart.data(), (int)ncart_, &dev_));
+        std::vector<int> atom(n);
+        for (size_t i = 0; i < n; i++) atom[i] = primitives_[i].atomIdx();
+        lsjk_check(lsjk_basis_set_atoms(dev_, (int)natom(), atom.data()));
     }
     return dev_;
 }

@@ -2,6 +2,7 @@
 // src/lightspeed/intbox.hpp:721-828, dispatch src/cpp/intbox/intbox.cpp:213-329) routed to the HIP
 // library.  Pure <-> Cartesian transforms happen here on the host, as reference coulombCPU does
 // around its kernels (src/cpp/intbox/cpu/coulomb.cpp:26-35, 186-193).
+#include <algorithm>
 #include <cstring>
 
 #include "../../../include/lsjk.h"
@@ -168,6 +169,120 @@ std::shared_ptr<Tensor> IntBox::potential(const std::shared_ptr<ResourceList>& r
 {
     if (!xyzc) throw std::runtime_error("IntBox::potential: xyzc is required");
     return one_electron(OneE::V, "IntBox::potential", resources, ewald, pairlist, xyzc, V);
+}
+
+std::shared_ptr<Tensor> IntBox::coulombGrad(const std::shared_ptr<ResourceList>& resources,
+                                            const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl,
+                                            const std::shared_ptr<Tensor>& D12, const std::shared_ptr<Tensor>& D34,
+                                            float thresp, float thredp, const std::shared_ptr<Tensor>& G)
+{
+    if (!pl->is_symmetric()) throw std::runtime_error("IntBox::coulombGrad: pairlist must be symmetric");
+    auto b = pl->basis1();
+    D12->shape_error({b->nao(), b->nao()});
+    D34->shape_error({b->nao(), b->nao()});
+    const size_t natom = b->natom();
+    if (G) G->shape_error({natom, 3});
+    require_ready("IntBox::coulombGrad", resources, pl, pl);
+    auto Dc12 = PureTransform::pureToCart2(b, b, D12);
+    auto Dc34 = PureTransform::pureToCart2(b, b, D34);
+    const size_t nD = b->ncart() * b->ncart();
+    DevBuf d12(nD * sizeof(double)), d34(nD * sizeof(double)), dG(std::max<size_t>(1, 3 * natom) * sizeof(double));
+    check(lsjk_memcpy_htod(d12.p, Dc12->data().data(), nD * sizeof(double)));
+    check(lsjk_memcpy_htod(d34.p, Dc34->data().data(), nD * sizeof(double)));
+    check(lsjk_memset_zero(dG.p, std::max<size_t>(1, 3 * natom) * sizeof(double)));
+    std::vector<double> s, o;
+    ewald_arrays(ewald, s, o);
+    lsjk_pairlist* d = pl->device_handle();
+    // bra-centre derivatives with each density in the bra role (d/dR of (12|34) = bra part + ket
+    // part, and the ket part of (12|34) is the bra part of (34|12))
+    check(lsjk_coulomb_grad(d, d12.d(), d34.d(), (int)s.size(), s.data(), o.data(), thresp, thredp, dG.d()));
+    check(lsjk_coulomb_grad(d, d34.d(), d12.d(), (int)s.size(), s.data(), o.data(), thresp, thredp, dG.d()));
+    auto Gn = std::make_shared<Tensor>(std::vector<size_t>{natom, 3}, "G");
+    check(lsjk_memcpy_dtoh(Gn->data().data(), dG.p, 3 * natom * sizeof(double)));
+    if (!G) return Gn;
+    G->axpby(Gn, 1.0, 1.0);
+    return G;
+}
+
+std::shared_ptr<Tensor> IntBox::eriJ(const std::shared_ptr<ResourceList>& resources,
+                                     const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl12,
+                                     const std::shared_ptr<PairList>& pl34, const std::shared_ptr<Tensor>& C1,
+                                     const std::shared_ptr<Tensor>& C2, const std::shared_ptr<Tensor>& C3,
+                                     const std::shared_ptr<Tensor>& C4, float thresp, float thredp,
+                                     const std::shared_ptr<Tensor>& I)
+{
+    C1->ndim_error(2);
+    C2->ndim_error(2);
+    C3->ndim_error(2);
+    C4->ndim_error(2);
+    const size_t nao1 = pl12->basis1()->nao(), nao2 = pl12->basis2()->nao();
+    const size_t nao3 = pl34->basis1()->nao(), nao4 = pl34->basis2()->nao();
+    const size_t n1 = C1->shape()[1], n2 = C2->shape()[1], n3 = C3->shape()[1], n4 = C4->shape()[1];
+    C1->shape_error({nao1, n1});
+    C2->shape_error({nao2, n2});
+    C3->shape_error({nao3, n3});
+    C4->shape_error({nao4, n4});
+    std::shared_ptr<Tensor> I2 = I ? I : std::make_shared<Tensor>(std::vector<size_t>{n1, n2, n3, n4}, "I");
+    I2->shape_error({n1, n2, n3, n4});
+    double* Ip = I2->data().data();
+    const double *c1 = C1->data().data(), *c2 = C2->data().data(), *c3 = C3->data().data(), *c4 = C4->data().data();
+    const bool symm12 = (C1 == C2);
+    auto D = std::make_shared<Tensor>(std::vector<size_t>{nao1, nao2}, "D");
+    std::vector<double> T(nao3 * n4);
+    for (size_t i = 0; i < n1; i++) {
+        for (size_t j = 0; j < n2; j++) {
+            if (symm12 && i < j) continue;
+            double* Dp = D->data().data();
+            for (size_t p = 0; p < nao1; p++)
+                for (size_t q = 0; q < nao2; q++) Dp[p * nao2 + q] = c1[p * n1 + i] * c2[q * n2 + j];
+            // J over the (3,4) pairs from the (1,2) density: (pl34 | pl12) as in mo_eri.cpp:89-97
+            auto J = IntBox::coulomb(resources, ewald, pl34, pl12, D, thresp, thredp);
+            const double* Jp = J->data().data();
+            // L = C3^T J C4 (n3 x n4)
+            std::fill(T.begin(), T.end(), 0.0);
+            for (size_t r = 0; r < nao3; r++)
+                for (size_t s = 0; s < nao4; s++) {
+                    const double v = Jp[r * nao4 + s];
+                    if (v == 0.0) continue;
+                    for (size_t l = 0; l < n4; l++) T[r * n4 + l] += v * c4[s * n4 + l];
+                }
+            double* dst = Ip + i * (n2 * n3 * n4) + j * (n3 * n4);
+            for (size_t k = 0; k < n3; k++)
+                for (size_t l = 0; l < n4; l++) {
+                    double v = 0.0;
+                    for (size_t r = 0; r < nao3; r++) v += c3[r * n3 + k] * T[r * n4 + l];
+                    dst[k * n4 + l] = v;
+                }
+            if (symm12 && i != j) std::memcpy(Ip + j * (n2 * n3 * n4) + i * (n3 * n4), dst, n3 * n4 * sizeof(double));
+        }
+    }
+    // symmetrisations of mo_eri.cpp:120-155
+    if (C1 == C2)
+        for (size_t rs = 0; rs < n3 * n4; rs++)
+            for (size_t p = 0; p < n1; p++)
+                for (size_t q = 0; q < p; q++) {
+                    double& a = Ip[p * n2 * n3 * n4 + q * n3 * n4 + rs];
+                    double& b = Ip[q * n2 * n3 * n4 + p * n3 * n4 + rs];
+                    a = b = 0.5 * (a + b);
+                }
+    if (C3 == C4)
+        for (size_t pq = 0; pq < n1 * n2; pq++) {
+            double* I2p = Ip + pq * n3 * n4;
+            for (size_t r = 0; r < n3; r++)
+                for (size_t s = 0; s < r; s++) {
+                    double& a = I2p[r * n4 + s];
+                    double& b = I2p[s * n4 + r];
+                    a = b = 0.5 * (a + b);
+                }
+        }
+    if (C1 == C3 && C2 == C4)
+        for (size_t pq = 0; pq < n1 * n2; pq++)
+            for (size_t rs = 0; rs < pq; rs++) {
+                double& a = Ip[pq * n3 * n4 + rs];
+                double& b = Ip[rs * n3 * n4 + pq];
+                a = b = 0.5 * (a + b);
+            }
+    return I2;
 }
 
 void IntBox::coulomb_device(const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl12,

@@ -247,6 +247,25 @@ struct IntBox {
                                              const std::shared_ptr<PairList>& pairlist,
                                              const std::shared_ptr<Tensor>& xyzc,
                                              const std::shared_ptr<Tensor>& V = std::shared_ptr<Tensor>());
+    // MO-basis ERIs I[i,j,k,l] = (C1_i C2_j | C3_k C4_l) through one J build per (i, j) pair
+    // (reference intbox.hpp:1316-1363, mo_eri.cpp:10-157), with the reference's symmetrisations
+    // when C1 == C2, C3 == C4 or (C1, C2) == (C3, C4) (the same Tensor objects).  I is overwritten.
+    static std::shared_ptr<Tensor> eriJ(const std::shared_ptr<ResourceList>& resources,
+                                        const std::shared_ptr<Ewald>& ewald,
+                                        const std::shared_ptr<PairList>& pairlist12,
+                                        const std::shared_ptr<PairList>& pairlist34,
+                                        const std::shared_ptr<Tensor>& C1, const std::shared_ptr<Tensor>& C2,
+                                        const std::shared_ptr<Tensor>& C3, const std::shared_ptr<Tensor>& C4,
+                                        float thresp, float thredp,
+                                        const std::shared_ptr<Tensor>& I = std::shared_ptr<Tensor>());
+    // Coulomb gradient G (natom, 3) += d/dR sum D12_pq D34_rs (pq|rs) over a symmetric pair list
+    // (reference intbox.hpp:954-1008; dispatch intbox.cpp:424-496).
+    static std::shared_ptr<Tensor> coulombGrad(const std::shared_ptr<ResourceList>& resources,
+                                               const std::shared_ptr<Ewald>& ewald,
+                                               const std::shared_ptr<PairList>& pairlist,
+                                               const std::shared_ptr<Tensor>& D12, const std::shared_ptr<Tensor>& D34,
+                                               float thresp, float thredp,
+                                               const std::shared_ptr<Tensor>& G = std::shared_ptr<Tensor>());
     // Backend predicate, the counterpart of is_terachem_ready (reference intbox.cpp:12-26).
     static bool is_hip_ready(const std::shared_ptr<ResourceList>& resources,
                              const std::shared_ptr<PairList>& pairlist12, const std::shared_ptr<PairList>& pairlist34,

@@ -352,3 +352,31 @@ def potential(ewald, pl, xyzc, V=None):
     lib().oracle_potential(ctypes.byref(b1.cstruct), ctypes.byref(b2.cstruct), ctypes.byref(pl.c), len(sc),
                            sc.ctypes.data, om.ctypes.data, xyzc.shape[0], xyzc.ctypes.data, Vc.ctypes.data)
     return _accumulate(b1, b2, Vc, V)
+
+
+def eriJ(ewald, pl12, pl34, C1, C2, C3, C4, thresp, same12=False, same34=False, same1234=False):
+    """IntBox::eriJ (src/cpp/intbox/mo_eri.cpp:10-157): I[i,j,k,l] = (C1_i C2_j | C3_k C4_l) by one J
+    build over (pl34 | pl12) per (i, j) density C1_i C2_j^T, transformed with C3, C4; the flags stand
+    for the reference's pointer tests C1 == C2, C3 == C4, (C1, C2) == (C3, C4)."""
+    C1, C2, C3, C4 = (np.asarray(c, dtype=np.float64) for c in (C1, C2, C3, C4))
+    n1, n2, n3, n4 = C1.shape[1], C2.shape[1], C3.shape[1], C4.shape[1]
+    I = np.zeros((n1, n2, n3, n4))
+    for i in range(n1):
+        for j in range(n2):
+            if same12 and i < j:
+                continue
+            D = np.outer(C1[:, i], C2[:, j])
+            J = coulomb(ewald, pl34, pl12, D, thresp)
+            I[i, j] = C3.T @ J @ C4
+            if same12 and i != j:
+                I[j, i] = I[i, j]
+    if same12:
+        for p in range(n1):
+            for q in range(p):
+                I[p, q] = I[q, p] = 0.5 * (I[p, q] + I[q, p])
+    if same34:
+        I = 0.5 * (I + I.transpose(0, 1, 3, 2))
+    if same1234:
+        M = I.reshape(n1 * n2, n3 * n4)
+        I = (0.5 * (M + M.T)).reshape(n1, n2, n3, n4)
+    return I

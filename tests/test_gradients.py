@@ -1,0 +1,70 @@
+"""Coulomb nuclear gradient (IntBox::coulombGrad; reference intbox.hpp:954-1008,
+cpu/coulomb_grad.cpp:19-563) against the reference's stored results (prop_int.py:141-176: Jgr
+(D, D), Jgr2 (D2, D2), Jgr3 (D, D2), Jgr4 (D2, D), each computed twice to test accumulation,
+tol 1e-9, symmetric pair list).
+
+CPU: the fixtures' meaning is pinned by central finite differences of the oracle's Coulomb
+energy E = sum D12 J[D34] under rigid displacements of one atom's basis functions (D held fixed).
+GPU: the HIP gradient against the fixtures.
+"""
+import numpy as np
+import pytest
+
+import fixture_cases as FC
+
+
+def _fd_gradient(O, records, D12, D34, h=1.0e-4):
+    """dE/dR_A by central differences of the oracle's J energy, E = sum D12 * J[D34]."""
+    from types import SimpleNamespace
+    natom = 1 + max(r[12] for r in records)
+
+    def energy(shift_atom, d, step):
+        recs = []
+        for r in records:
+            r = list(r)
+            if r[12] == shift_atom:
+                r[2 + d] += step
+            recs.append(SimpleNamespace(c=r[0], e=r[1], x=r[2], y=r[3], z=r[4], c0=r[5], L=r[6], is_pure=r[7],
+                                        aoIdx=r[8], cartIdx=r[9], primIdx=r[10], shellIdx=r[11], atomIdx=r[12]))
+        b = SimpleNamespace(primitives=recs)
+        ob = O.OBasis(b)
+        pl = O.OPairList(ob, ob, True, 1.0e-14)
+        J = O.coulomb(FC.COULOMB, pl, pl, D34, 1.0e-14)
+        return float(np.sum(D12 * J))
+
+    G = np.zeros((natom, 3))
+    for A in range(natom):
+        for d in range(3):
+            G[A, d] = (energy(A, d, h) - energy(A, d, -h)) / (2 * h)
+    return G
+
+
+@pytest.mark.parametrize("basname", ["sto-3g", "cc-pvdz"])
+def test_coulomb_gradient_fixture_by_finite_differences(oracle_backend, basname):
+    from lightspeed_amd import loaders
+    import numpy as np
+    import os
+    mol = FC.molecule("h2o")
+    recs = [tuple(r) for r in loaders.basis_records_from_gbs_file(mol, basname)[1]]
+    z = np.load(os.path.join(FC.DATA, "prop_h2o_%s.npz" % basname))
+    G = _fd_gradient(oracle_backend.O, recs, z["D"], z["D"])
+    assert np.abs(G - z["Jgr"]).max() < 2.0e-6, np.abs(G - z["Jgr"]).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("basname", FC.BASES)
+def test_coulomb_gradient_fixtures_gpu(product_backend, basname):
+    import os
+    ls = product_backend.ls
+    mol = FC.molecule("h2o")
+    bas = ls.Basis.from_gbs_file(mol, basname)
+    pl = ls.PairList.build_schwarz(bas, bas, True, 1.0e-14)
+    z = np.load(os.path.join(FC.DATA, "prop_h2o_%s.npz" % basname))
+    res = ls.ResourceList.build()
+    ew = ls.Ewald([1.0], [-1.0])
+    D, D2 = ls.Tensor.array(z["D"]), ls.Tensor.array(z["D2"])
+    for key, (a, b) in {"Jgr": (D, D), "Jgr2": (D2, D2), "Jgr3": (D, D2), "Jgr4": (D2, D)}.items():
+        G = ls.IntBox.coulombGrad(res, ew, pl, a, b, 1.0e-14, 1.0e-14)
+        G = ls.IntBox.coulombGrad(res, ew, pl, a, b, 1.0e-14, 1.0e-14, G)
+        err = np.abs(np.array(G.np) - 2.0 * z[key]).max()
+        assert err < 1.0e-9, (basname, key, err)
