@@ -106,6 +106,14 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
 int lsjk_coulomb_grad(const lsjk_pairlist* pl, const double* Dbra_dev, const double* Dket_dev, int nterm,
                       const double* scales, const double* omegas, float thresp, float thredp, double* G_dev);
 
+/* Exchange gradient (the reference's exchangeGrad, intbox.hpp:1092-1160, cpu/exchange_grad.cpp):
+ *   G_dev (natom x 3) += d/dR sum_{abcd} D13_ac D24_bd (ab|O|cd)
+ * over a symmetric list (both sides), Cartesian densities, quartets with
+ * float(b12 b34) max|D24 block| >= thresp (as lsjk_exchange without the D24symm half-sieve; FP64).
+ * Needs lsjk_basis_set_atoms. */
+int lsjk_exchange_grad(const lsjk_pairlist* pl, const double* D13_dev, const double* D24_dev, int nterm,
+                       const double* scales, const double* omegas, float thresp, float thredp, double* G_dev);
+
 /* One-electron integrals of a pair list, accumulated into ncart1 x ncart2 device matrices
  * (Cartesian, as lsjk_coulomb).  A symmetric list fills both triangles.
  * lsjk_overlap   replaces IntBox::overlap   (reference src/cpp/intbox/intbox.cpp:30-40,

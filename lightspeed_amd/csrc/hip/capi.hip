@@ -1241,6 +1241,78 @@ int lsjk_coulomb_grad(const lsjk_pairlist* pl, const double* Dbra_dev, const dou
     });
 }
 
+int lsjk_exchange_grad(const lsjk_pairlist* pl, const double* D13_dev, const double* D24_dev, int nterm,
+                       const double* scales, const double* omegas, float thresp, float thredp, double* G_dev)
+{
+    (void)thredp;  // FP64 route
+    return guarded([&] {
+        using namespace lsjk;
+        require_init();
+        if (!pl || !D13_dev || !D24_dev || !G_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_exchange_grad: null argument");
+        if (!pl->symmetric || pl->b1 != pl->b2)
+            throw LsjkError(LSJK_E_INVALID, "lsjk_exchange_grad: the pair list must be symmetric");
+        const Basis* b = pl->b1;
+        if (!b->d_atom) throw LsjkError(LSJK_E_INVALID, "lsjk_exchange_grad: basis atoms not set");
+        Ewald ew;
+        check_ewald(nterm, scales, omegas, ew);
+        hipStream_t st = S.stream;
+        double* Dsh = nullptr;
+        Workspace wsp;
+        wsp.add(&Dsh, (size_t)b->nblk * b->nblk);
+        wsp.commit();
+        launch_fill(S.d_sc, sizeof(CallScalars), st);
+        launch_k_blockmax(D24_dev, b->ncart, b->d_blk_cart, b->d_blk_L, b->nblk, b->d_blk_cart, b->d_blk_L, b->nblk, Dsh,
+                          S.d_sc, st);
+        if (ew.n > 0) {
+            for (int L1 = 0; L1 <= b->maxL; L1++)
+                for (int L2 = 0; L2 <= b->maxL; L2++)
+                    for (int L3 = 0; L3 <= b->maxL; L3++)
+                        for (int L4 = 0; L4 <= b->maxL; L4++) {
+                            if (pl->h_rows[L1][L2].empty() || pl->h_rows[L3][L4].empty()) continue;
+                            const auto units = get_kunits(pl, pl, L1, L2, L3, L4, false, nullptr);
+                            if (units.second == 0) continue;
+                            KArgs ka{};
+                            ka.rys = S.rys;
+                            ka.ew = ew;
+                            ka.thresp = thresp;
+                            ka.thredp = thresp;
+                            KSide* ks[2] = {&ka.s12, &ka.s34};
+                            const int Ls[2] = {L1, L3}, Lps[2] = {L2, L4};
+                            for (int q = 0; q < 2; q++) {
+                                KSide& s = *ks[q];
+                                s.row_start = pl->d_row_start;
+                                s.maxLp = pl->kmaxL2;
+                                s.eb = pl->d_ebound;
+                                s.ecart = pl->d_ecart;
+                                s.eblk = pl->d_eblk;
+                                s.eg = pl->d_egeom;
+                                s.eg32 = pl->d_egeom32;
+                                s.nent = pl->nent;
+                                s.rows = pl->d_rows[Ls[q]][Lps[q]];
+                                s.nrows = (int)pl->h_rows[Ls[q]][Lps[q]].size();
+                                s.cart = b->d_cart;
+                            }
+                            ka.D = D24_dev;
+                            ka.ldD = b->ncart;
+                            ka.Dsh = Dsh;
+                            ka.ldDsh = b->nblk;
+                            ka.maxb12 = pl->maxbound;
+                            ka.maxb34 = pl->maxbound;
+                            ka.sc = S.d_sc;
+                            ka.D13 = D13_dev;
+                            ka.ldD13 = b->ncart;
+                            ka.G = G_dev;
+                            ka.exp1 = ka.exp2 = ka.exp3 = ka.exp4 = b->d_e;
+                            ka.atom1 = ka.atom2 = ka.atom3 = ka.atom4 = b->d_atom;
+                            ka.eprim12 = ka.eprim34 = pl->d_eprim;
+                            launch_k_quartets(L1, L2, L3, L4, kRouteGrad, ka, units.first, units.second, st);
+                        }
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
 // One-electron integrals of one pair list (accumulated, Cartesian): S and/or T.
 static void one_e_st(const lsjk_pairlist* pl, double* Sout, double* Tout)
 {

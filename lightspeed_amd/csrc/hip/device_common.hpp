@@ -43,6 +43,12 @@ __host__ __device__ constexpr int cart_lz(int L, int k)
     return 0;
 }
 
+// Exponent of axis k (0 = x, 1 = y, 2 = z) of Cartesian component idx of shell L.
+__host__ __device__ constexpr int cart_lk(int L, int idx, int k)
+{
+    return k == 0 ? cart_lx(L, idx) : (k == 1 ? cart_ly(L, idx) : cart_lz(L, idx));
+}
+
 // Hermite index t of total order <= L in the nested (L, lexical) order: components.
 __host__ __device__ constexpr int herm_tx(int t)
 {

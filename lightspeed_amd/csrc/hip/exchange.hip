@@ -12,6 +12,15 @@ extern template void launch_k_quartets_l1<0>(int, int, int, int, const KArgs&, c
 extern template void launch_k_quartets_l1<1>(int, int, int, int, const KArgs&, const KUnit*, int, hipStream_t);
 extern template void launch_k_quartets_l1<2>(int, int, int, int, const KArgs&, const KUnit*, int, hipStream_t);
 
+template <int L1, int L2>
+void launch_k_grad_l12(int L3, int L4, const KArgs& a, const KUnit* units, int nunits, hipStream_t s);
+#define LSJK_GRAD_EXTERN(X, Y) \
+    extern template void launch_k_grad_l12<X, Y>(int, int, const KArgs&, const KUnit*, int, hipStream_t);
+LSJK_GRAD_EXTERN(0, 0) LSJK_GRAD_EXTERN(0, 1) LSJK_GRAD_EXTERN(0, 2)
+LSJK_GRAD_EXTERN(1, 0) LSJK_GRAD_EXTERN(1, 1) LSJK_GRAD_EXTERN(1, 2)
+LSJK_GRAD_EXTERN(2, 0) LSJK_GRAD_EXTERN(2, 1) LSJK_GRAD_EXTERN(2, 2)
+#undef LSJK_GRAD_EXTERN
+
 // Block maxima of |D24| per (block2, block4) and the global maximum.
 __global__ void k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2, int nb2, const int* bc4,
                            const int* bL4, int nb4, double* Dsh, CallScalars* sc)
@@ -74,6 +83,13 @@ void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a
                        hipStream_t s)
 {
     if (nunits <= 0) return;
+    if (route == kRouteGrad) {
+#define G(X, Y) \
+    if (L1 == X && L2 == Y) return launch_k_grad_l12<X, Y>(L3, L4, a, units, nunits, s);
+        G(0, 0) G(0, 1) G(0, 2) G(1, 0) G(1, 1) G(1, 2) G(2, 0) G(2, 1) G(2, 2)
+#undef G
+        return;
+    }
     if (L1 == 0)
         launch_k_quartets_l1<0>(L2, L3, L4, route, a, units, nunits, s);
     else if (L1 == 1)

@@ -236,6 +236,105 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const Ewald& ew, 
     }
 }
 
+// Derivatives of one primitive quartet with respect to the centres A, B, C (D by translational
+// invariance, dE/dD = -(dE/dA + dE/dB + dE/dC)), contracted with D13[a][c] D24[b][d]:
+//   d/dA_x (ab|cd) = 2 alpha (a+1x b|cd) - a_x (a-1x b|cd), likewise B (beta), C (gamma),
+// from 2-D Rys integrals with L1, L2, L3 raised by one (reference exchange_grad.cpp).
+template <int L1, int L2, int L3, int L4>
+__device__ __forceinline__ void k_quartet_grad(const RysTable& tab, const Ewald& ew, const KEnt<double>& e12,
+                                               const KEnt<double>& e34, double alpha, double beta, double gamma,
+                                               const double (&D13)[ncart(L1)][ncart(L3)],
+                                               const double (&D24)[ncart(L2)][ncart(L4)], double (&gA)[3],
+                                               double (&gB)[3], double (&gC)[3])
+{
+    constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4);
+    constexpr int NR = (L1 + L2 + L3 + L4 + 1) / 2 + 1;
+    const double p = e12.p, q = e34.p;
+    const double pq = p + q;
+    const double ipq = 1.0 / pq;
+    const double rho = p * q * ipq;
+    const double PQx = e12.Px - e34.Px, PQy = e12.Py - e34.Py, PQz = e12.Pz - e34.Pz;
+    const double r2 = PQx * PQx + PQy * PQy + PQz * PQz;
+    const double base = 2.0 * 17.493418327624862 / (p * q * sqrt(pq)) * e12.K * e34.K;
+    const double qf = q * ipq, pf = p * ipq;
+    const double i2p = 0.5 / p, i2q = 0.5 / q;
+    for (int it = 0; it < ew.n; it++) {
+        const double om = ew.omega[it];
+        double d2 = 1.0;
+        double pref = ew.alpha[it] * base;
+        if (om != -1.0) {
+            d2 = om * om / (rho + om * om);
+            pref *= sqrt(d2);
+        }
+        double x[NR], w[NR];
+        rys_roots<NR>(tab, rho * d2 * r2, x, w);
+        // roots one after another (not unrolled): keeps the derivative kernels' code size and
+        // compile time bounded
+#pragma unroll 1
+        for (int i = 0; i < NR; i++) {
+            const double u = x[i] * d2;
+            const double B00 = 0.5 * u * ipq;
+            const double B10 = i2p * (1.0 - u * qf);
+            const double B01 = i2q * (1.0 - u * pf);
+            const double cq = u * qf, cp = u * pf;
+            Axis4<double, L1 + 1, L2 + 1, L3 + 1, L4> I[3];
+            rys_axis<double, L1 + 1, L2 + 1, L3 + 1, L4>(pref * w[i], e12.PAx - cq * PQx, e34.PAx + cp * PQx, B10, B01,
+                                                         B00, e12.ABx, e34.ABx, I[0]);
+            rys_axis<double, L1 + 1, L2 + 1, L3 + 1, L4>(1.0, e12.PAy - cq * PQy, e34.PAy + cp * PQy, B10, B01, B00,
+                                                         e12.ABy, e34.ABy, I[1]);
+            rys_axis<double, L1 + 1, L2 + 1, L3 + 1, L4>(1.0, e12.PAz - cq * PQz, e34.PAz + cp * PQz, B10, B01, B00,
+                                                         e12.ABz, e34.ABz, I[2]);
+            // (a, c) run as ordinary loops (the 2-D integrals are then indexed at run time), the
+            // partner components (b, d) and the three axes are expanded at compile time
+#pragma unroll 1
+            for (int a = 0; a < N1; a++) {
+                int an[3];
+                for (int k = 0; k < 3; k++) an[k] = cart_lk(L1, a, k);
+#pragma unroll 1
+                for (int c = 0; c < N3; c++) {
+                    int cn[3];
+                    for (int k = 0; k < 3; k++) cn[k] = cart_lk(L3, c, k);
+                    const double d13 = D13[a][c];
+                    if (d13 == 0.0) continue;
+                    static_for<N2>([&](auto B_) {
+                        constexpr int b = decltype(B_)::value;
+                        static_for<N4>([&](auto D_) {
+                            constexpr int d = decltype(D_)::value;
+                            const double wgt = d13 * D24[b][d];
+                            double base3[3], dA[3], dB[3], dC[3];
+                            static_for<3>([&](auto K_) {
+                                constexpr int k = decltype(K_)::value;
+                                constexpr int ib = cart_lk(L2, b, k), id = cart_lk(L4, d, k);
+                                const int ia = an[k], ic = cn[k];
+                                const auto& V = I[k].v;
+                                base3[k] = V[ia][ib][ic][id];
+                                double va = 2.0 * alpha * V[ia + 1][ib][ic][id];
+                                if (ia > 0) va -= ia * V[ia - 1][ib][ic][id];
+                                double vb = 2.0 * beta * V[ia][ib + 1][ic][id];
+                                if constexpr (ib > 0) vb -= ib * V[ia][ib - 1][ic][id];
+                                double vc = 2.0 * gamma * V[ia][ib][ic + 1][id];
+                                if (ic > 0) vc -= ic * V[ia][ib][ic - 1][id];
+                                dA[k] = va;
+                                dB[k] = vb;
+                                dC[k] = vc;
+                            });
+                            gA[0] += wgt * dA[0] * base3[1] * base3[2];
+                            gA[1] += wgt * base3[0] * dA[1] * base3[2];
+                            gA[2] += wgt * base3[0] * base3[1] * dA[2];
+                            gB[0] += wgt * dB[0] * base3[1] * base3[2];
+                            gB[1] += wgt * base3[0] * dB[1] * base3[2];
+                            gB[2] += wgt * base3[0] * base3[1] * dB[2];
+                            gC[0] += wgt * dC[0] * base3[1] * base3[2];
+                            gC[1] += wgt * base3[0] * dC[1] * base3[2];
+                            gC[2] += wgt * base3[0] * base3[1] * dC[2];
+                        });
+                    });
+                }
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -255,6 +354,7 @@ template <int L1, int L2, int L3, int L4, int ROUTE>
 __global__ __launch_bounds__(64) void k_tiles(KArgs a, const KUnit* __restrict__ units)
 {
     constexpr bool LO = ROUTE == kRouteLo;
+    constexpr bool GRAD = ROUTE == kRouteGrad;
     using Real = typename std::conditional<LO, float, double>::type;
     constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4), N13 = N1 * N3;
     __shared__ double s_acc[kKTile * N13];
@@ -300,6 +400,29 @@ __global__ __launch_bounds__(64) void k_tiles(KArgs a, const KUnit* __restrict__
         for (int b = 0; b < N2; b++)
 #pragma unroll
             for (int d = 0; d < N4; d++) Db[b][d] = (Real)a.D[(size_t)(E12.cart + b) * a.ldD + (E34.cart + d)];
+        if constexpr (GRAD) {
+            const int r3 = a.s34.rows[j3a + tt];
+            const int p2 = a.eprim12[e2], p4 = a.eprim34[e4];
+            const int o1 = a.s12.cart[r1], o3 = a.s34.cart[r3];
+            double D13[N1][N3];
+#pragma unroll
+            for (int i = 0; i < N1; i++)
+#pragma unroll
+                for (int j = 0; j < N3; j++) D13[i][j] = a.D13[(size_t)(o1 + i) * a.ldD13 + (o3 + j)];
+            double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
+            k_quartet_grad<L1, L2, L3, L4>(a.rys, a.ew, E12, E34, a.exp1[r1], a.exp2[p2], a.exp3[r3], D13, Db, gA, gB,
+                                           gC);
+            const int at1 = a.atom1[r1], at2 = a.atom2[p2], at3 = a.atom3[r3], at4 = a.atom4[p4];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                atomic_add(a.G + 3 * at1 + k, gA[k]);
+                atomic_add(a.G + 3 * at2 + k, gB[k]);
+                atomic_add(a.G + 3 * at3 + k, gC[k]);
+                atomic_add(a.G + 3 * at4 + k, -(gA[k] + gB[k] + gC[k]));
+            }
+            nq++;
+            return;
+        }
         Real acc[N1][N3];
 #pragma unroll
         for (int i = 0; i < N1; i++)
@@ -382,6 +505,7 @@ __global__ __launch_bounds__(64) void k_tiles(KArgs a, const KUnit* __restrict__
     __syncthreads();
     if (lane < qn) evaluate(lane);
     __syncthreads();
+    if constexpr (GRAD) return;  // gradients were added per quartet
     // flush the tile accumulator
     const int o1 = a.s12.cart[r1];
     for (int q = lane; q < (j3b - j3a) * N13; q += 64) {
@@ -424,6 +548,20 @@ void launch_k_quartets_l1(int L2, int L3, int L4, int route, const KArgs& a, con
     C3(0) C3(1) C3(2)
 #undef C3
 #undef C4
+#undef CASE
+}
+
+// Launch of one exchange-gradient class (instantiated per (L1, L2) in exchange_grad_<L1><L2>.hip).
+template <int L1, int L2>
+void launch_k_grad_l12(int L3, int L4, const KArgs& a, const KUnit* units, int nunits, hipStream_t s)
+{
+    dim3 grid((unsigned)nunits), block(64);
+#define CASE(C, D)                                                                                  \
+    if (L3 == C && L4 == D) {                                                                       \
+        hipLaunchKernelGGL((k_tiles<L1, L2, C, D, kRouteGrad>), grid, block, 0, s, a, units);      \
+        return;                                                                                     \
+    }
+    CASE(0, 0) CASE(0, 1) CASE(0, 2) CASE(1, 0) CASE(1, 1) CASE(1, 2) CASE(2, 0) CASE(2, 1) CASE(2, 2)
 #undef CASE
 }
 

@@ -16,6 +16,7 @@ __host__ __device__ constexpr int kcount_index(int L1, int L2, int L3, int L4)
 constexpr int kRoutePlain = 0;  // est >= thresp, FP64 (thredp <= thresp)
 constexpr int kRouteHi = 1;     // est >= thredp, FP64
 constexpr int kRouteLo = 2;     // thresp <= est < thredp, FP32 arithmetic, FP64 accumulation
+constexpr int kRouteGrad = 3;   // exchange gradient: est >= thresp, FP64 derivative integrals
 
 struct JPrepArgs {
     const int* p1;
@@ -176,6 +177,13 @@ struct KArgs {
     int ldK;
     float maxb12, maxb34;  // max_bound of the two lists
     CallScalars* sc;
+    // exchange gradient (kRouteGrad): G (natom x 3) += d/dR sum D13_ac D24_bd (ab|cd)
+    const double* D13;
+    int ldD13;
+    double* G;
+    const double *exp1, *exp2, *exp3, *exp4;  // primitive exponents of the four bases
+    const int *atom1, *atom2, *atom3, *atom4; // primitive atoms
+    const int *eprim12, *eprim34;             // partner primitive of each entry
 };
 
 struct CountJArgs {

@@ -204,6 +204,40 @@ std::shared_ptr<Tensor> IntBox::coulombGrad(const std::shared_ptr<ResourceList>&
     return G;
 }
 
+std::shared_ptr<Tensor> IntBox::exchangeGrad(const std::shared_ptr<ResourceList>& resources,
+                                             const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl,
+                                             const std::shared_ptr<Tensor>& D13, const std::shared_ptr<Tensor>& D24,
+                                             bool D13symm, bool D24symm, bool Dsame, float thresp, float thredp,
+                                             const std::shared_ptr<Tensor>& G)
+{
+    (void)D13symm;
+    (void)D24symm;
+    (void)Dsame;
+    if (!pl->is_symmetric()) throw std::runtime_error("IntBox::exchangeGrad: pairlist must be symmetric");
+    auto b = pl->basis1();
+    D13->shape_error({b->nao(), b->nao()});
+    D24->shape_error({b->nao(), b->nao()});
+    const size_t natom = b->natom();
+    if (G) G->shape_error({natom, 3});
+    require_ready("IntBox::exchangeGrad", resources, pl, pl);
+    auto Dc13 = PureTransform::pureToCart2(b, b, D13);
+    auto Dc24 = PureTransform::pureToCart2(b, b, D24);
+    const size_t nD = b->ncart() * b->ncart();
+    DevBuf d13(nD * sizeof(double)), d24(nD * sizeof(double)), dG(std::max<size_t>(1, 3 * natom) * sizeof(double));
+    check(lsjk_memcpy_htod(d13.p, Dc13->data().data(), nD * sizeof(double)));
+    check(lsjk_memcpy_htod(d24.p, Dc24->data().data(), nD * sizeof(double)));
+    check(lsjk_memset_zero(dG.p, std::max<size_t>(1, 3 * natom) * sizeof(double)));
+    std::vector<double> s, o;
+    ewald_arrays(ewald, s, o);
+    check(lsjk_exchange_grad(pl->device_handle(), d13.d(), d24.d(), (int)s.size(), s.data(), o.data(), thresp, thredp,
+                             dG.d()));
+    auto Gn = std::make_shared<Tensor>(std::vector<size_t>{natom, 3}, "G");
+    check(lsjk_memcpy_dtoh(Gn->data().data(), dG.p, 3 * natom * sizeof(double)));
+    if (!G) return Gn;
+    G->axpby(Gn, 1.0, 1.0);
+    return G;
+}
+
 std::shared_ptr<Tensor> IntBox::eriJ(const std::shared_ptr<ResourceList>& resources,
                                      const std::shared_ptr<Ewald>& ewald, const std::shared_ptr<PairList>& pl12,
                                      const std::shared_ptr<PairList>& pl34, const std::shared_ptr<Tensor>& C1,

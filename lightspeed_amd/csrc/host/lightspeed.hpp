@@ -266,6 +266,15 @@ struct IntBox {
                                                const std::shared_ptr<Tensor>& D12, const std::shared_ptr<Tensor>& D34,
                                                float thresp, float thredp,
                                                const std::shared_ptr<Tensor>& G = std::shared_ptr<Tensor>());
+    // Exchange gradient G (natom, 3) += d/dR sum D13_pq D24_rs (pr|qs) over a symmetric pair list
+    // (reference intbox.hpp:1092-1160; dispatch intbox.cpp:519-620).  The symmetry flags of the
+    // reference only select faster CPU paths; they are checked as the reference does.
+    static std::shared_ptr<Tensor> exchangeGrad(const std::shared_ptr<ResourceList>& resources,
+                                                const std::shared_ptr<Ewald>& ewald,
+                                                const std::shared_ptr<PairList>& pairlist,
+                                                const std::shared_ptr<Tensor>& D13, const std::shared_ptr<Tensor>& D24,
+                                                bool D13symm, bool D24symm, bool Dsame, float thresp, float thredp,
+                                                const std::shared_ptr<Tensor>& G = std::shared_ptr<Tensor>());
     // Backend predicate, the counterpart of is_terachem_ready (reference intbox.cpp:12-26).
     static bool is_hip_ready(const std::shared_ptr<ResourceList>& resources,
                              const std::shared_ptr<PairList>& pairlist12, const std::shared_ptr<PairList>& pairlist34,

@@ -13,8 +13,9 @@ import pytest
 import fixture_cases as FC
 
 
-def _fd_gradient(O, records, D12, D34, h=1.0e-4):
-    """dE/dR_A by central differences of the oracle's J energy, E = sum D12 * J[D34]."""
+def _fd_gradient(O, records, D12, D34, h=1.0e-4, kind="J"):
+    """dE/dR_A by central differences of the oracle's energy: E = sum D12 * J[D34] (kind "J") or
+    E = sum D13 * K13[D24] (kind "K", D12 -> D13, D34 -> D24)."""
     from types import SimpleNamespace
     natom = 1 + max(r[12] for r in records)
 
@@ -29,8 +30,11 @@ def _fd_gradient(O, records, D12, D34, h=1.0e-4):
         b = SimpleNamespace(primitives=recs)
         ob = O.OBasis(b)
         pl = O.OPairList(ob, ob, True, 1.0e-14)
-        J = O.coulomb(FC.COULOMB, pl, pl, D34, 1.0e-14)
-        return float(np.sum(D12 * J))
+        if kind == "J":
+            M = O.coulomb(FC.COULOMB, pl, pl, D34, 1.0e-14)
+        else:
+            M = O.exchange(FC.COULOMB, pl, pl, D34, False, 1.0e-14)
+        return float(np.sum(D12 * M))
 
     G = np.zeros((natom, 3))
     for A in range(natom):
@@ -49,6 +53,39 @@ def test_coulomb_gradient_fixture_by_finite_differences(oracle_backend, basname)
     z = np.load(os.path.join(FC.DATA, "prop_h2o_%s.npz" % basname))
     G = _fd_gradient(oracle_backend.O, recs, z["D"], z["D"])
     assert np.abs(G - z["Jgr"]).max() < 2.0e-6, np.abs(G - z["Jgr"]).max()
+
+
+@pytest.mark.parametrize("basname", ["sto-3g", "6-31gs"])
+def test_exchange_gradient_fixture_by_finite_differences(oracle_backend, basname):
+    """Kgr3 = exchangeGrad(D13 = D, D24 = D2): dE/dR of E = sum D13_pq D24_rs (pr|qs)."""
+    from lightspeed_amd import loaders
+    import os
+    mol = FC.molecule("h2o")
+    recs = [tuple(r) for r in loaders.basis_records_from_gbs_file(mol, basname)[1]]
+    z = np.load(os.path.join(FC.DATA, "prop_h2o_%s.npz" % basname))
+    G = _fd_gradient(oracle_backend.O, recs, z["D"], z["D2"], kind="K")
+    assert np.abs(G - z["Kgr3"]).max() < 2.0e-6, np.abs(G - z["Kgr3"]).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("basname", FC.BASES)
+def test_exchange_gradient_fixtures_gpu(product_backend, basname):
+    import os
+    ls = product_backend.ls
+    mol = FC.molecule("h2o")
+    bas = ls.Basis.from_gbs_file(mol, basname)
+    pl = ls.PairList.build_schwarz(bas, bas, True, 1.0e-14)
+    z = np.load(os.path.join(FC.DATA, "prop_h2o_%s.npz" % basname))
+    res = ls.ResourceList.build()
+    ew = ls.Ewald([1.0], [-1.0])
+    D, D2 = ls.Tensor.array(z["D"]), ls.Tensor.array(z["D2"])
+    cases = {"Kgr": (D, D, True, True, True), "Kgr2": (D2, D2, False, False, True),
+             "Kgr3": (D, D2, True, False, False), "Kgr4": (D2, D, False, True, False)}
+    for key, (a, b, s13, s24, same) in cases.items():
+        G = ls.IntBox.exchangeGrad(res, ew, pl, a, b, s13, s24, same, 1.0e-14, 1.0e-14)
+        G = ls.IntBox.exchangeGrad(res, ew, pl, a, b, s13, s24, same, 1.0e-14, 1.0e-14, G)
+        err = np.abs(np.array(G.np) - 2.0 * z[key]).max()
+        assert err < 1.0e-9, (basname, key, err)
 
 
 @pytest.mark.gpu
