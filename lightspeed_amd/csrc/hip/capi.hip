@@ -466,11 +466,12 @@ struct JSegments {
     int cnt[5][5] = {{0}};
 };
 
-// K work units of one class, heaviest first.  A unit is (row1, block of kKTile row3s, chunk of at
-// most kKChunk2 of row1's partners); its cost estimate is (chunk partners) x (partners of the
-// block's row3s) from the bound-screened lists alone (D only removes quartets, it never adds).
-// In the triangle mode row3s below row1 are not counted and units without work are dropped.
-// Dispatching heaviest first keeps long units from finishing after the rest of the launch.
+// K work units of one class.  A unit is (row1, block of kKTile row3s, chunk of at most kKChunk2 of
+// row1's partners); its cost estimate is (chunk partners) x (partners of the block's row3s) from
+// the bound-screened lists alone (D only removes quartets, it never adds).  In the triangle mode
+// row3s below row1 are not counted and units without work are dropped.  Heavy units are
+// dispatched first (so that long units do not finish after the rest of the launch), in an order
+// that keeps each XCD on neighbouring rows (see below).
 std::pair<const lsjk::KUnit*, int> get_kunits(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34, int L1, int L2,
                                               int L3, int L4, bool tri, double* cost_out)
 {
@@ -507,10 +508,28 @@ std::pair<const lsjk::KUnit*, int> get_kunits(const lsjk_pairlist* pl12c, const 
                 }
             }
         }
-        std::stable_sort(v.begin(), v.end(), [](const std::pair<long long, KUnit>& a,
-                                                const std::pair<long long, KUnit>& b) { return a.first > b.first; });
-        std::vector<KUnit> u(v.size());
-        for (size_t i = 0; i < v.size(); i++) u[i] = v[i].second;
+        // Dispatch order: coarse cost buckets (powers of two), heaviest bucket first; inside a bucket
+        // the units stay in row order (v is built row-major) and are split into 8 contiguous ranges
+        // that are interleaved, so that with the round-robin workgroup-to-XCD dispatch each XCD (one
+        // L2) works through neighbouring rows of D and K.
+        auto bucket = [](long long c) { return c > 0 ? 63 - __builtin_clzll((unsigned long long)c) : 0; };
+        std::stable_sort(v.begin(), v.end(), [&](const std::pair<long long, KUnit>& a,
+                                                 const std::pair<long long, KUnit>& b) {
+            return bucket(a.first) > bucket(b.first);
+        });
+        std::vector<KUnit> u;
+        u.reserve(v.size());
+        for (size_t s0 = 0; s0 < v.size();) {
+            size_t s1 = s0;
+            while (s1 < v.size() && bucket(v[s1].first) == bucket(v[s0].first)) s1++;
+            const size_t n = s1 - s0, chunk = (n + 7) / 8;
+            for (size_t k = 0; k < chunk; k++)
+                for (size_t x = 0; x < 8; x++) {
+                    const size_t src = s0 + x * chunk + k;
+                    if (x * chunk + k < n) u.push_back(v[src].second);
+                }
+            s0 = s1;
+        }
         if (u.size() > (size_t)std::numeric_limits<int>::max())
             throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk: too many exchange work units");
         KUnit* d = u.empty() ? nullptr : dev_upload(u);
