@@ -261,7 +261,7 @@ def main():
         route = ", %d" % (2 if lo else (1 if head == "mixed" else 0))
         if kind == "J":
             return "j_quartets<%d, %d%s>" % (idx // 5, idx % 5, route)
-        return "k_tiles<%d, %d, %d, %d%s>" % (cls_of(idx - 256) + (route,))
+        return "k_ket<%d, %d, %d, %d%s>" % (cls_of(idx - 256) + (route,))
 
     means = {k: float(np.mean(v)) for k, v in launch_ms.items()}
     for k in sorted(means, key=lambda k: -means[k])[:24]:

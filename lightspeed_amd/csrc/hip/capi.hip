@@ -322,7 +322,9 @@ void build_pairlist(lsjk_pairlist* pl)
         eblk[q] = b2->h_blk[s];
         double* g = eg.data() + (size_t)kKEntDoubles * q;
         g[0] = p;
-        g[1] = b1->h_c[r] * b2->h_c[s] * std::exp(-er * es * ip * R2);
+        // K_AB sqrt(2 pi^(5/2)) / p: the product of the two sides' values times (p + q)^(-1/2) is the
+        // quartet prefactor 2 pi^(5/2) K_AB K_CD / (p q sqrt(p + q))
+        g[1] = b1->h_c[r] * b2->h_c[s] * std::exp(-er * es * ip * R2) * kKPrefRoot * ip;
         for (int d = 0; d < 3; d++) {
             const double A = b1->h_xyz[3 * r + d], B = b2->h_xyz[3 * s + d];
             const double P = (er * A + es * B) * ip;
@@ -332,15 +334,16 @@ void build_pairlist(lsjk_pairlist* pl)
         }
         const int ids[2] = {ecart[q], eblk[q]};
         std::memcpy(&g[11], ids, sizeof(ids));
+        g[12] = 0.5 * ip;
         // FP32-route copy: P in FP64, the rest rounded to FP32
         double* h = eg32.data() + (size_t)kKEnt32Doubles * q;
         h[0] = g[2];
         h[1] = g[3];
         h[2] = g[4];
-        const float f[8] = {(float)g[0], (float)g[1], (float)g[5], (float)g[6],
-                            (float)g[7], (float)g[8], (float)g[9], (float)g[10]};
+        const float f[10] = {(float)g[0], (float)g[1], (float)g[5], (float)g[6], (float)g[7],
+                             (float)g[8], (float)g[9], (float)g[10], (float)g[12], 0.0f};
         std::memcpy(&h[3], f, sizeof(f));
-        std::memcpy(&h[7], ids, sizeof(ids));
+        std::memcpy(&h[8], ids, sizeof(ids));
     }
     for (int L = 0; L <= kMaxL; L++)
         for (int Lp = 0; Lp <= kMaxL; Lp++) pl->h_rows[L][Lp].clear();
@@ -379,6 +382,7 @@ void build_pairlist(lsjk_pairlist* pl)
     pl->d_hgeom = dev_upload(hgeom);
     pl->d_row_start = dev_upload(pl->h_row_start);
     pl->d_ebound = dev_upload(eb);
+    pl->h_eb = eb;
     pl->d_eprim = dev_upload(eprim);
     pl->d_ecart = dev_upload(ecart);
     pl->d_eblk = dev_upload(eblk);
@@ -452,6 +456,17 @@ void free_pairlist(lsjk_pairlist* pl)
     for (auto& kv : pl->kunits) dev_free(kv.second.first);
     pl->kunits.clear();
     pl->kcost.clear();
+    for (auto& kv : pl->kunits_g) dev_free(kv.second.first);
+    pl->kunits_g.clear();
+    pl->kcost_g.clear();
+    for (auto& kv : pl->kgroups) {
+        dev_free(kv.second.d_slot);
+        dev_free(kv.second.d_grp_slot);
+        dev_free(kv.second.d_grp_rows);
+        dev_free(kv.second.d_grp_rank);
+        dev_free(kv.second.d_chunk_b);
+    }
+    pl->kgroups.clear();
 }
 
 // J task list for (bra list, ket list): per (L12, L34) group pair, tiles of 64 bra pairs x chunks of
@@ -538,6 +553,179 @@ std::pair<const lsjk::KUnit*, int> get_kunits(const lsjk_pairlist* pl12c, const 
     }
     if (cost_out) *cost_out = pl12->kcost[key];
     return {static_cast<const KUnit*>(it->second.first), it->second.second};
+}
+
+// Spatial rank of every basis-1 primitive of a list: primitives sorted by the Morton code of
+// their centre (10 bits per axis over the bounding box), ties by index.
+const std::vector<int>& get_mrank(lsjk_pairlist* pl)
+{
+    if (!pl->h_mrank.empty() || pl->b1->nprim == 0) return pl->h_mrank;
+    const lsjk::Basis* b = pl->b1;
+    const int n = b->nprim;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], b->h_xyz[3 * i + d]);
+            hi[d] = std::max(hi[d], b->h_xyz[3 * i + d]);
+        }
+    auto spread = [](unsigned int v) {  // 10 bits -> every third bit of 30
+        unsigned long long x = v & 1023u, r = 0;
+        for (int k = 0; k < 10; k++) r |= ((x >> k) & 1ull) << (3 * k);
+        return r;
+    };
+    std::vector<std::pair<unsigned long long, int>> key(n);
+    for (int i = 0; i < n; i++) {
+        unsigned long long m = 0;
+        for (int d = 0; d < 3; d++) {
+            const double ext = hi[d] - lo[d];
+            const unsigned int q = ext > 0.0 ? (unsigned int)std::min(1023.0, (b->h_xyz[3 * i + d] - lo[d]) / ext * 1023.0) : 0u;
+            m |= spread(q) << d;
+        }
+        key[i] = {m, i};
+    }
+    std::sort(key.begin(), key.end());
+    pl->h_mrank.assign(n, 0);
+    for (int r = 0; r < n; r++) pl->h_mrank[key[r].second] = r;
+    return pl->h_mrank;
+}
+
+// Ket groups of the (row L, partner Lp) class of a list (see KGroups): the class's rows in spatial
+// order, kKGroup per group, each group's partner entries merged by bound (descending, ties by
+// entry index) and padded to a multiple of 64 slots.
+const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp)
+{
+    using lsjk::kKGroup;
+    const int key = L * 8 + Lp;
+    auto it = pl->kgroups.find(key);
+    if (it != pl->kgroups.end()) return it->second;
+    lsjk_pairlist::KGroupData G;
+    const auto& rows = pl->h_rows[L][Lp];
+    const auto& mrank = get_mrank(pl);
+    const int nr = (int)rows.size();
+    G.order.resize(nr);
+    for (int i = 0; i < nr; i++) G.order[i] = i;
+    std::sort(G.order.begin(), G.order.end(), [&](int a, int b) { return mrank[rows[a]] < mrank[rows[b]]; });
+    G.ngroup = (nr + kKGroup - 1) / kKGroup;
+    const int m = pl->kmaxL2 + 1;
+    std::vector<int2> slot;
+    std::vector<int> grp_rows((size_t)G.ngroup * kKGroup, -1);
+    G.grp_rank.assign((size_t)G.ngroup * kKGroup, std::numeric_limits<int>::max());
+    G.grp_slot.assign(G.ngroup + 1, 0);
+    for (int g = 0; g < G.ngroup; g++) {
+        std::vector<int2> v;
+        for (int t = 0; t < kKGroup && g * kKGroup + t < nr; t++) {
+            const int r = rows[G.order[g * kKGroup + t]];
+            grp_rows[(size_t)g * kKGroup + t] = r;
+            G.grp_rank[(size_t)g * kKGroup + t] = mrank[r];
+            for (int e = pl->h_row_start[(size_t)r * m + Lp]; e < pl->h_row_start[(size_t)r * m + Lp + 1]; e++)
+                v.push_back(make_int2(e, t));
+        }
+        std::stable_sort(v.begin(), v.end(), [&](const int2& a, const int2& b) { return pl->h_eb[a.x] > pl->h_eb[b.x]; });
+        while (v.size() % 64) v.push_back(make_int2(-1, 0));
+        for (size_t c = 0; c < v.size(); c += 64) G.chunk_b.push_back(pl->h_eb[v[c].x]);
+        slot.insert(slot.end(), v.begin(), v.end());
+        G.grp_slot[g + 1] = (int)slot.size();
+    }
+    G.d_slot = dev_upload(slot);
+    G.d_grp_slot = dev_upload(G.grp_slot);
+    G.d_grp_rows = dev_upload(grp_rows);
+    G.d_grp_rank = dev_upload(G.grp_rank);
+    G.d_chunk_b = dev_upload(G.chunk_b);
+    return pl->kgroups.emplace(key, std::move(G)).first->second;
+}
+
+// K build units of one class: (bra row, ket group) pairs.  Rows are taken in spatial order and,
+// in the triangle mode, groups lying wholly before the bra row in that order are dropped.  The
+// cost estimate of a unit is the number of (bra partner, ket chunk) steps that pass the bound
+// test with |D| <= 1 (an ordering heuristic only; the kernel screens with the actual D).  Units
+// are dispatched in coarse cost buckets, heaviest first; inside a bucket in row order, split into
+// 8 contiguous ranges that are interleaved, so that with the round-robin workgroup-to-XCD
+// dispatch each XCD (one L2) works through neighbouring rows of D and K.
+std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34c, int L1,
+                                                 int L2, int L3, int L4, bool tri, float thresp, double* cost_out)
+{
+    using lsjk::kKGroup;
+    using lsjk::KUnitG;
+    lsjk_pairlist* pl12 = const_cast<lsjk_pairlist*>(pl12c);
+    lsjk_pairlist* pl34 = const_cast<lsjk_pairlist*>(pl34c);
+    unsigned int tb;
+    std::memcpy(&tb, &thresp, sizeof(tb));
+    const long long cls = ((long long)tb << 8) | (((((L1 * 3 + L2) * 3 + L3) * 3 + L4) << 1) | (tri ? 1 : 0));
+    const auto key = std::make_pair(pl34->uid, cls);
+    auto it = pl12->kunits_g.find(key);
+    if (it == pl12->kunits_g.end()) {
+        const auto& rows1 = pl12->h_rows[L1][L2];
+        const auto& G34 = get_kgroups(pl34, L3, L4);
+        const auto& G12 = get_kgroups(pl12, L1, L2);
+        const auto& mrank = get_mrank(pl12);
+        const int m12 = pl12->kmaxL2 + 1;
+        // chunk bounds per group, and the smallest rank of each group (triangle mode)
+        std::vector<std::pair<long long, KUnitG>> v;
+        double total = 0.0;
+        for (int i1 : G12.order) {
+            const int r1 = rows1[i1];
+            const int a0 = pl12->h_row_start[(size_t)r1 * m12 + L2], a1 = pl12->h_row_start[(size_t)r1 * m12 + L2 + 1];
+            // bounds are sorted descending: the partners with b12 >= x are a prefix
+            auto npass = [&](double x) {
+                int lo = a0, hi = a1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) / 2;
+                    if ((double)pl12->h_eb[mid] >= x) lo = mid + 1; else hi = mid;
+                }
+                return (long long)(lo - a0);
+            };
+            const int rank1 = tri ? mrank[r1] : -1;
+            for (int g = 0; g < G34.ngroup; g++) {
+                if (tri) {
+                    int rmax = -1;
+                    for (int t = 0; t < kKGroup; t++) {
+                        const int rk = G34.grp_rank[(size_t)g * kKGroup + t];
+                        if (rk != std::numeric_limits<int>::max()) rmax = std::max(rmax, rk);
+                    }
+                    if (rmax < rank1) continue;
+                }
+                long long c = 0;
+                for (int s = G34.grp_slot[g]; s < G34.grp_slot[g + 1]; s += 64)
+                    c += npass((double)thresp / (double)G34.chunk_b[s / 64]);
+                v.push_back({c, KUnitG{i1, g, rank1, 0}});
+                total += (double)c;
+            }
+        }
+        auto bucket = [](long long c) { return c > 0 ? 64 - __builtin_clzll((unsigned long long)c) : 0; };
+        std::stable_sort(v.begin(), v.end(), [&](const std::pair<long long, KUnitG>& a,
+                                                 const std::pair<long long, KUnitG>& b) {
+            return bucket(a.first) > bucket(b.first);
+        });
+        std::vector<KUnitG> u;
+        u.reserve(v.size());
+        for (size_t s0 = 0; s0 < v.size();) {
+            size_t s1 = s0;
+            while (s1 < v.size() && bucket(v[s1].first) == bucket(v[s0].first)) s1++;
+            const size_t n = s1 - s0, chunk = (n + 7) / 8;
+            for (size_t k = 0; k < chunk; k++)
+                for (size_t x = 0; x < 8; x++)
+                    if (x * chunk + k < n) u.push_back(v[s0 + x * chunk + k].second);
+            s0 = s1;
+        }
+        if (u.size() > (size_t)std::numeric_limits<int>::max() / lsjk::kKWaves)
+            throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk: too many exchange work units");
+        KUnitG* d = u.empty() ? nullptr : dev_upload(u);
+        it = pl12->kunits_g.insert({key, {(void*)d, (int)u.size()}}).first;
+        pl12->kcost_g[key] = total;
+    }
+    if (cost_out) *cost_out = pl12->kcost_g[key];
+    return {static_cast<const KUnitG*>(it->second.first), it->second.second};
+}
+
+lsjk::KGroups kgroups_args(const lsjk_pairlist::KGroupData& G)
+{
+    lsjk::KGroups k;
+    k.slot = G.d_slot;
+    k.grp_slot = G.d_grp_slot;
+    k.grp_rows = G.d_grp_rows;
+    k.grp_rank = G.d_grp_rank;
+    k.chunk_b = G.d_chunk_b;
+    return k;
 }
 
 JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
@@ -1069,10 +1257,11 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
             fo.begin();
             struct KL {
                 int L1, L2, L3, L4;
-                const KUnit* units;
+                const KUnitG* units;
                 int nunits;
                 double cost;
                 KArgs ka;
+                KGroups kg;
             };
             std::vector<KL> kls;
             for (int L1 = 0; L1 <= b1->maxL; L1++)
@@ -1087,8 +1276,9 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                             // are evaluated
                             const bool tri = symmode && (L1 == L3);
                             double cost = 0.0;
-                            const auto units = get_kunits(pl12, pl34, L1, L2, L3, L4, tri, &cost);
+                            const auto units = get_kunits_g(pl12, pl34, L1, L2, L3, L4, tri, thresp, &cost);
                             if (units.second == 0) continue;
+                            const KGroups kg = kgroups_args(get_kgroups(const_cast<lsjk_pairlist*>(pl34), L3, L4));
                             KArgs ka{};
                             ka.rys = S.rys;
                             ka.ew = ew;
@@ -1134,7 +1324,7 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                             const double Pi = (double)ncart(L1) * ncart(L2) * ncart(L3) * ncart(L4);
                             const double G = (double)(L1 + L2 + 1) * (L3 + L4 + 1);
                             const double F = 40.0 + (Lt / 2 + 1) * (30.0 + 9.0 * G + 3.0 * Pi) + 2.0 * Pi;
-                            kls.push_back({L1, L2, L3, L4, units.first, units.second, cost * F, ka});
+                            kls.push_back({L1, L2, L3, L4, units.first, units.second, cost * F, ka, kg});
                         }
             // heaviest classes first (estimated cost)
             std::stable_sort(kls.begin(), kls.end(), [](const KL& a, const KL& b) { return a.cost > b.cost; });
@@ -1142,12 +1332,12 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                 const int idx = kcount_index(k.L1, k.L2, k.L3, k.L4);
                 hipStream_t ss = fo.pick();
                 lt.begin(idx, ss);
-                launch_k_quartets(k.L1, k.L2, k.L3, k.L4, mixed ? kRouteHi : kRoutePlain, k.ka, k.units, k.nunits, ss);
+                launch_k_ket(k.L1, k.L2, k.L3, k.L4, mixed ? kRouteHi : kRoutePlain, k.ka, k.kg, k.units, k.nunits, ss);
                 lt.end();
                 if (mixed) {
                     hipStream_t sl = fo.pick();
                     lt.begin(idx + kLoRouteId, sl);
-                    launch_k_quartets(k.L1, k.L2, k.L3, k.L4, kRouteLo, k.ka, k.units, k.nunits, sl);
+                    launch_k_ket(k.L1, k.L2, k.L3, k.L4, kRouteLo, k.ka, k.kg, k.units, k.nunits, sl);
                     lt.end();
                 }
             }

@@ -121,7 +121,8 @@ struct RysTable {
 // polynomial on each bin reaches the FP32 rounding floor (1.2e-7 relative to F0).
 constexpr int kRysNcoef32 = 6;
 
-// Evaluate the nrys-root rule: x_i = t_i^2, w_i.  Same arithmetic as the oracle's rys_eval.
+// Evaluate the nrys-root rule: x_i = t_i^2, w_i.  The oracle's rys_eval evaluates the same rule
+// (same table, same asymptotic formula; here 1/T and T^(-1/2) come from one rsqrt).
 // A non-finite T takes the asymptotic branch (no table access), and the bin is clamped, so the
 // table is never read out of bounds.
 template <int NRYS>
@@ -129,8 +130,8 @@ __device__ __forceinline__ void rys_roots(const RysTable& tab, double T, double*
 {
     constexpr int NC = 12;  // LSJK_RYS_NCOEF
     if (!(T < tab.Tc[NRYS])) {
-        const double iT = 1.0 / T;
-        const double s = sqrt(iT);
+        const double s = rsqrt(T);
+        const double iT = s * s;
 #pragma unroll
         for (int i = 0; i < NRYS; i++) {
             x[i] = tab.asym_r2[NRYS * (NRYS - 1) / 2 + i] * iT;
@@ -192,8 +193,8 @@ __device__ __forceinline__ void rys_roots_lds(const RysTable& tab, const double*
 {
     constexpr int NC = 12;
     if (!(T < tab.Tc[NRYS])) {
-        const double iT = 1.0 / T;
-        const double s = sqrt(iT);
+        const double s = rsqrt(T);
+        const double iT = s * s;
 #pragma unroll
         for (int i = 0; i < NRYS; i++) {
             x[i] = tab.asym_r2[NRYS * (NRYS - 1) / 2 + i] * iT;
@@ -239,8 +240,8 @@ __device__ __forceinline__ void rys_roots_f32(const RysTable& tab, const float* 
 {
     constexpr int NC = kRysNcoef32;
     if (!(T < tab.Tc[NRYS])) {
-        const double iT = 1.0 / T;
-        const double s = sqrt(iT);
+        const double s = rsqrt(T);
+        const double iT = s * s;
 #pragma unroll
         for (int i = 0; i < NRYS; i++) {
             x[i] = (float)(tab.asym_r2[NRYS * (NRYS - 1) / 2 + i] * iT);
@@ -270,6 +271,9 @@ __device__ __forceinline__ void rys_roots_f32(const RysTable& tab, const float* 
 
 // Floats of the FP32 table block of one nrys.
 __host__ __device__ constexpr size_t rys32_lds_floats(int nbin, int nrys) { return (size_t)nbin * nrys * 2 * kRysNcoef32; }
+
+// Kernels of the FP32 route stage the FP32 block in LDS for nrys <= 5 (at most 14.2 KB).
+__host__ __device__ constexpr bool rys32_staged(int nrys) { return nrys <= 5; }
 
 // Cooperative copy of the FP32 nrys block into LDS (the block is a multiple of 12 floats and
 // starts at a multiple of 12 floats: 16-byte copies stay aligned when the base is).

@@ -6,11 +6,13 @@
 namespace lsjk {
 
 template <int L1>
-void launch_k_quartets_l1(int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,
-                          hipStream_t s);
-extern template void launch_k_quartets_l1<0>(int, int, int, int, const KArgs&, const KUnit*, int, hipStream_t);
-extern template void launch_k_quartets_l1<1>(int, int, int, int, const KArgs&, const KUnit*, int, hipStream_t);
-extern template void launch_k_quartets_l1<2>(int, int, int, int, const KArgs&, const KUnit*, int, hipStream_t);
+void launch_k_ket_l1(int L2, int L3, int L4, int route, const KArgs& a, const KGroups& kg, const KUnitG* units,
+                     int nunits, hipStream_t s);
+#define LSJK_KET_EXTERN(X)                                                                                   \
+    extern template void launch_k_ket_l1<X>(int, int, int, int, const KArgs&, const KGroups&, const KUnitG*, int, \
+                                            hipStream_t);
+LSJK_KET_EXTERN(0) LSJK_KET_EXTERN(1) LSJK_KET_EXTERN(2)
+#undef LSJK_KET_EXTERN
 
 template <int L1, int L2>
 void launch_k_grad_l12(int L3, int L4, const KArgs& a, const KUnit* units, int nunits, hipStream_t s);
@@ -83,19 +85,23 @@ void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a
                        hipStream_t s)
 {
     if (nunits <= 0) return;
-    if (route == kRouteGrad) {
+    if (route != kRouteGrad) throw std::runtime_error("lsjk: tile kernels serve the gradient route only");
 #define G(X, Y) \
     if (L1 == X && L2 == Y) return launch_k_grad_l12<X, Y>(L3, L4, a, units, nunits, s);
-        G(0, 0) G(0, 1) G(0, 2) G(1, 0) G(1, 1) G(1, 2) G(2, 0) G(2, 1) G(2, 2)
+    G(0, 0) G(0, 1) G(0, 2) G(1, 0) G(1, 1) G(1, 2) G(2, 0) G(2, 1) G(2, 2)
 #undef G
-        return;
-    }
+}
+
+void launch_k_ket(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KGroups& kg, const KUnitG* units,
+                  int nunits, hipStream_t s)
+{
+    if (nunits <= 0) return;
     if (L1 == 0)
-        launch_k_quartets_l1<0>(L2, L3, L4, route, a, units, nunits, s);
+        launch_k_ket_l1<0>(L2, L3, L4, route, a, kg, units, nunits, s);
     else if (L1 == 1)
-        launch_k_quartets_l1<1>(L2, L3, L4, route, a, units, nunits, s);
+        launch_k_ket_l1<1>(L2, L3, L4, route, a, kg, units, nunits, s);
     else
-        launch_k_quartets_l1<2>(L2, L3, L4, route, a, units, nunits, s);
+        launch_k_ket_l1<2>(L2, L3, L4, route, a, kg, units, nunits, s);
 }
 
 }  // namespace lsjk

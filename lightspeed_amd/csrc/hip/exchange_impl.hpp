@@ -107,7 +107,7 @@ __device__ __forceinline__ void rys_axis(Real g00, Real C00, Real D00, Real B10,
 template <typename Real>
 struct KEnt {
     double Px, Py, Pz;
-    Real p, K, PAx, PAy, PAz, ABx, ABy, ABz;
+    Real p, K, PAx, PAy, PAz, ABx, ABy, ABz, i2p;  // K: Kp of the packed entry
     int cart, blk;
 };
 
@@ -130,11 +130,12 @@ __device__ __forceinline__ KEnt<double> load_ent(const double* __restrict__ eg, 
     const int* ids = reinterpret_cast<const int*>(g + 11);
     e.cart = ids[0];
     e.blk = ids[1];
+    e.i2p = g[12];
     return e;
 }
 
-// One packed FP32-route entry: a 64-byte record (P as 3 doubles; p, K, P - A, A - B as 8 floats;
-// {cart, blk}).
+// One packed FP32-route entry: an 80-byte record (P as 3 doubles; p, Kp, P - A, A - B, 1/(2p) as
+// floats; {cart, blk}).
 __device__ __forceinline__ KEnt<float> load_ent32(const double* __restrict__ eg32, int i)
 {
     const double* g = static_cast<const double*>(__builtin_assume_aligned(eg32 + (size_t)i * kKEnt32Doubles, 16));
@@ -151,7 +152,8 @@ __device__ __forceinline__ KEnt<float> load_ent32(const double* __restrict__ eg3
     e.ABx = f[5];
     e.ABy = f[6];
     e.ABz = f[7];
-    const int* ids = reinterpret_cast<const int*>(g + 7);
+    e.i2p = f[8];
+    const int* ids = reinterpret_cast<const int*>(g + 8);
     e.cart = ids[0];
     e.blk = ids[1];
     return e;
@@ -166,28 +168,41 @@ __device__ __forceinline__ Real kdiv(Real a, Real b)
         return a / b;
 }
 
+template <typename Real>
+__device__ __forceinline__ Real krsqrt(Real x)
+{
+    if constexpr (std::is_same<Real, float>::value)
+        return __frsqrt_rn(x);
+    else
+        return rsqrt(x);
+}
+
 // One primitive quartet: acc[a][c] += sum_{b,d} (ab|cd) Db[b][d], in the route's arithmetic.
-template <typename Real, int L1, int L2, int L3, int L4>
-__device__ __forceinline__ void k_quartet(const RysTable& tab, const Ewald& ew, const KEnt<Real>& e12,
-                                          const KEnt<Real>& e34, const Real (&Db)[ncart(L2)][ncart(L4)],
-                                          Real (&acc)[ncart(L1)][ncart(L3)])
+// STAGED: the table block of this nrys was copied to LDS at s_tab (FP64 block on the FP64 route,
+// FP32 block on the FP32 route); otherwise the global table is read.
+template <typename Real, int L1, int L2, int L3, int L4, bool STAGED>
+__device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab, const Ewald& ew,
+                                          const KEnt<Real>& e12, const KEnt<Real>& e34,
+                                          const Real (&Db)[ncart(L2)][ncart(L4)], Real (&acc)[ncart(L1)][ncart(L3)])
 {
     constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4);
     constexpr int NR = (L1 + L2 + L3 + L4) / 2 + 1;
     const Real p = e12.p, q = e34.p;
     const Real pq = p + q;
-    const Real ipq = kdiv(Real(1), pq);
+    const Real rpq = krsqrt(pq);
+    const Real ipq = rpq * rpq;
     const Real rho = p * q * ipq;
     const double PQxd = e12.Px - e34.Px, PQyd = e12.Py - e34.Py, PQzd = e12.Pz - e34.Pz;
     const double r2 = PQxd * PQxd + PQyd * PQyd + PQzd * PQzd;
     const Real PQx = (Real)PQxd, PQy = (Real)PQyd, PQz = (Real)PQzd;
-    const Real base = kdiv(Real(2.0 * 17.493418327624862), p * q * sqrt(pq)) * e12.K * e34.K;
+    // 2 pi^(5/2) K_AB K_CD / (p q sqrt(p + q))
+    const Real base = e12.K * e34.K * rpq;
     const Real PAx = e12.PAx, PAy = e12.PAy, PAz = e12.PAz;
     const Real QCx = e34.PAx, QCy = e34.PAy, QCz = e34.PAz;
     const Real ABx = e12.ABx, ABy = e12.ABy, ABz = e12.ABz;
     const Real CDx = e34.ABx, CDy = e34.ABy, CDz = e34.ABz;
     const Real qf = q * ipq, pf = p * ipq;  // rho/p, rho/q
-    const Real i2p = kdiv(Real(0.5), p), i2q = kdiv(Real(0.5), q);
+    const Real i2p = e12.i2p, i2q = e34.i2p;
     for (int it = 0; it < ew.n; it++) {
         const double om = ew.omega[it];
         Real d2 = Real(1);
@@ -199,7 +214,10 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const Ewald& ew, 
         }
         Real x[NR], w[NR];
         if constexpr (std::is_same<Real, float>::value)
-            rys_roots_f32<NR>(tab, tab.coef32 + tab.offset32[NR], (double)(rho * d2) * r2, x, w);
+            rys_roots_f32<NR>(tab, STAGED ? static_cast<const float*>(s_tab) : tab.coef32 + tab.offset32[NR],
+                              (double)(rho * d2) * r2, x, w);
+        else if constexpr (STAGED)
+            rys_roots_lds<NR>(tab, static_cast<const double*>(s_tab), rho * d2 * r2, x, w);
         else
             rys_roots<NR>(tab, rho * d2 * r2, x, w);
 #pragma unroll
@@ -251,13 +269,14 @@ __device__ __forceinline__ void k_quartet_grad(const RysTable& tab, const Ewald&
     constexpr int NR = (L1 + L2 + L3 + L4 + 1) / 2 + 1;
     const double p = e12.p, q = e34.p;
     const double pq = p + q;
-    const double ipq = 1.0 / pq;
+    const double rpq = rsqrt(pq);
+    const double ipq = rpq * rpq;
     const double rho = p * q * ipq;
     const double PQx = e12.Px - e34.Px, PQy = e12.Py - e34.Py, PQz = e12.Pz - e34.Pz;
     const double r2 = PQx * PQx + PQy * PQy + PQz * PQz;
-    const double base = 2.0 * 17.493418327624862 / (p * q * sqrt(pq)) * e12.K * e34.K;
+    const double base = e12.K * e34.K * rpq;  // 2 pi^(5/2) K_AB K_CD / (p q sqrt(p + q))
     const double qf = q * ipq, pf = p * ipq;
-    const double i2p = 0.5 / p, i2q = 0.5 / q;
+    const double i2p = e12.i2p, i2q = e34.i2p;
     for (int it = 0; it < ew.n; it++) {
         const double om = ew.omega[it];
         double d2 = 1.0;
@@ -344,31 +363,22 @@ __device__ __forceinline__ double wave_sum(double v)
 
 constexpr int kKQueue = 128;
 
-// One wave per tile = (row1, block of kKTile row3s) of a class (L1 L2 | L3 L4).  For each unit
-// (row1, row3) of the tile the lanes test the sieve over the candidate rectangle (partners that can
-// pass with the global D maximum), compact the passing (2,4) combinations into an LDS queue shared
-// by all units of the tile, and evaluate the quartets 64 at a time; each quartet's N1 x N3 block is
-// added to the tile's LDS accumulator, which is flushed to K once at the end (plus the transposed
-// block in the symmetric mode).
-template <int L1, int L2, int L3, int L4, int ROUTE>
-__global__ __launch_bounds__(64) void k_tiles(KArgs a, const KUnit* __restrict__ units)
+// Exchange gradient.  One wave per tile = (row1, block of kKTile row3s) of a class (L1 L2 | L3 L4).
+// For each (row1, row3) of the tile the lanes test the sieve over the candidate rectangle
+// (partners that can pass with the global D maximum), compact the passing (2,4) combinations into
+// an LDS queue shared by all pairs of the tile, and evaluate the quartets 64 at a time; each
+// quartet's derivative contributions go to the four atoms' gradients with atomics.
+template <int L1, int L2, int L3, int L4>
+__global__ __launch_bounds__(64) void k_tiles_grad(KArgs a, const KUnit* __restrict__ units)
 {
-    constexpr bool LO = ROUTE == kRouteLo;
-    constexpr bool GRAD = ROUTE == kRouteGrad;
-    using Real = typename std::conditional<LO, float, double>::type;
-    constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4), N13 = N1 * N3;
-    __shared__ double s_acc[kKTile * N13];
     __shared__ int s_e2[kKQueue], s_e4[kKQueue], s_t[kKQueue];
     const int lane = threadIdx.x;
     const KUnit unit = units[blockIdx.x];
     const int r1 = a.s12.rows[unit.i1];
     const int j3a = unit.ib * kKTile;
     const int j3b = min(j3a + kKTile, a.s34.nrows);
-    for (int q = lane; q < kKTile * N13; q += 64) s_acc[q] = 0.0;
 
-    // quartets with est at or above `thresp` here (and, on the FP32 route, below thredp) are evaluated
-    const float thresp = (ROUTE == kRouteHi) ? a.thredp : a.thresp;
-    const double thredp = (double)a.thredp;
+    const float thresp = a.thresp;
     const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
     const float thre12 = (float)((double)thresp / ((double)a.maxb34 * dmax));
     const float thre34 = (float)((double)thresp / ((double)a.maxb12 * dmax));
@@ -382,64 +392,39 @@ __global__ __launch_bounds__(64) void k_tiles(KArgs a, const KUnit* __restrict__
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) n2max += __shfl_xor(n2max, o);
 
-    unsigned long long nq = 0;
     int qn = 0;  // queue length (wave-uniform)
     // inlined at both call sites: an outlined copy would take the address of the kernel arguments
     auto evaluate = [&](int slot) __attribute__((always_inline)) {
+        constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4);
         const int e2 = s_e2[slot], e4 = s_e4[slot], tt = s_t[slot];
-        KEnt<Real> E12, E34;
-        if constexpr (LO) {
-            E12 = load_ent32(a.s12.eg32, e2);
-            E34 = load_ent32(a.s34.eg32, e4);
-        } else {
-            E12 = load_ent(a.s12.eg, e2);
-            E34 = load_ent(a.s34.eg, e4);
-        }
-        Real Db[N2][N4];
+        const KEnt<double> E12 = load_ent(a.s12.eg, e2), E34 = load_ent(a.s34.eg, e4);
+        double Db[N2][N4];
 #pragma unroll
         for (int b = 0; b < N2; b++)
 #pragma unroll
-            for (int d = 0; d < N4; d++) Db[b][d] = (Real)a.D[(size_t)(E12.cart + b) * a.ldD + (E34.cart + d)];
-        if constexpr (GRAD) {
-            const int r3 = a.s34.rows[j3a + tt];
-            const int p2 = a.eprim12[e2], p4 = a.eprim34[e4];
-            const int o1 = a.s12.cart[r1], o3 = a.s34.cart[r3];
-            double D13[N1][N3];
+            for (int d = 0; d < N4; d++) Db[b][d] = a.D[(size_t)(E12.cart + b) * a.ldD + (E34.cart + d)];
+        const int r3 = a.s34.rows[j3a + tt];
+        const int p2 = a.eprim12[e2], p4 = a.eprim34[e4];
+        const int o1 = a.s12.cart[r1], o3 = a.s34.cart[r3];
+        double D13[N1][N3];
 #pragma unroll
-            for (int i = 0; i < N1; i++)
+        for (int i = 0; i < N1; i++)
 #pragma unroll
-                for (int j = 0; j < N3; j++) D13[i][j] = a.D13[(size_t)(o1 + i) * a.ldD13 + (o3 + j)];
-            double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
-            k_quartet_grad<L1, L2, L3, L4>(a.rys, a.ew, E12, E34, a.exp1[r1], a.exp2[p2], a.exp3[r3], D13, Db, gA, gB,
-                                           gC);
-            const int at1 = a.atom1[r1], at2 = a.atom2[p2], at3 = a.atom3[r3], at4 = a.atom4[p4];
+            for (int j = 0; j < N3; j++) D13[i][j] = a.D13[(size_t)(o1 + i) * a.ldD13 + (o3 + j)];
+        double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
+        k_quartet_grad<L1, L2, L3, L4>(a.rys, a.ew, E12, E34, a.exp1[r1], a.exp2[p2], a.exp3[r3], D13, Db, gA, gB, gC);
+        const int at1 = a.atom1[r1], at2 = a.atom2[p2], at3 = a.atom3[r3], at4 = a.atom4[p4];
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                atomic_add(a.G + 3 * at1 + k, gA[k]);
-                atomic_add(a.G + 3 * at2 + k, gB[k]);
-                atomic_add(a.G + 3 * at3 + k, gC[k]);
-                atomic_add(a.G + 3 * at4 + k, -(gA[k] + gB[k] + gC[k]));
-            }
-            nq++;
-            return;
+        for (int k = 0; k < 3; k++) {
+            atomic_add(a.G + 3 * at1 + k, gA[k]);
+            atomic_add(a.G + 3 * at2 + k, gB[k]);
+            atomic_add(a.G + 3 * at3 + k, gC[k]);
+            atomic_add(a.G + 3 * at4 + k, -(gA[k] + gB[k] + gC[k]));
         }
-        Real acc[N1][N3];
-#pragma unroll
-        for (int i = 0; i < N1; i++)
-#pragma unroll
-            for (int j = 0; j < N3; j++) acc[i][j] = Real(0);
-        k_quartet<Real, L1, L2, L3, L4>(a.rys, a.ew, E12, E34, Db, acc);
-        // FP64 accumulation on both routes
-#pragma unroll
-        for (int i = 0; i < N1; i++)
-#pragma unroll
-            for (int j = 0; j < N3; j++) atomicAdd(&s_acc[tt * N13 + i * N3 + j], (double)acc[i][j]);
-        nq++;
     };
 
     for (int t = j3a; t < j3b && n2max > 0; t++) {
         const int r3 = a.s34.rows[t];
-        if (a.tri && r3 < r1) continue;
         const int c0 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4];
         const int c1 = a.s34.row_start[r3 * (a.s34.maxLp + 1) + L4 + 1];
         const float b34top = a.s34.eb[c0];
@@ -470,7 +455,6 @@ __global__ __launch_bounds__(64) void k_tiles(KArgs a, const KUnit* __restrict__
                     const double dblk = a.Dsh[(size_t)a.s12.eblk[e2] * a.ldDsh + a.s34.eblk[e4]];
                     const double est = (double)(b12 * b34) * dblk;
                     pass = est >= (double)thresp;
-                    if constexpr (LO) pass = pass && !(est >= thredp);
                 }
             }
             const unsigned long long m = __ballot(pass);
@@ -504,18 +488,198 @@ __global__ __launch_bounds__(64) void k_tiles(KArgs a, const KUnit* __restrict__
     }
     __syncthreads();
     if (lane < qn) evaluate(lane);
+}
+
+// One packed FP64 entry at a wave-uniform index, through scalar loads.
+__device__ __forceinline__ KEnt<double> load_ent_uniform(const double* __restrict__ eg, int i)
+{
+    const double* g = eg + (size_t)i * kKEntDoubles;
+    KEnt<double> e;
+    e.p = ld_const(g + 0);
+    e.K = ld_const(g + 1);
+    e.Px = ld_const(g + 2);
+    e.Py = ld_const(g + 3);
+    e.Pz = ld_const(g + 4);
+    e.PAx = ld_const(g + 5);
+    e.PAy = ld_const(g + 6);
+    e.PAz = ld_const(g + 7);
+    e.ABx = ld_const(g + 8);
+    e.ABy = ld_const(g + 9);
+    e.ABz = ld_const(g + 10);
+    const int* ids = reinterpret_cast<const int*>(g + 11);
+    e.cart = ld_const(ids);
+    e.blk = ld_const(ids + 1);
+    e.i2p = ld_const(g + 12);
+    return e;
+}
+
+// One packed FP32-route entry at a wave-uniform index, through scalar loads.
+__device__ __forceinline__ KEnt<float> load_ent32_uniform(const double* __restrict__ eg32, int i)
+{
+    const double* g = eg32 + (size_t)i * kKEnt32Doubles;
+    KEnt<float> e;
+    e.Px = ld_const(g + 0);
+    e.Py = ld_const(g + 1);
+    e.Pz = ld_const(g + 2);
+    const float* f = reinterpret_cast<const float*>(g + 3);
+    e.p = ld_const(f + 0);
+    e.K = ld_const(f + 1);
+    e.PAx = ld_const(f + 2);
+    e.PAy = ld_const(f + 3);
+    e.PAz = ld_const(f + 4);
+    e.ABx = ld_const(f + 5);
+    e.ABy = ld_const(f + 6);
+    e.ABz = ld_const(f + 7);
+    e.i2p = ld_const(f + 8);
+    const int* ids = reinterpret_cast<const int*>(g + 8);
+    e.cart = ld_const(ids);
+    e.blk = ld_const(ids + 1);
+    return e;
+}
+
+// LDS of one k_ket workgroup: kKWaves group accumulators (kKGroup x N1 x N3 doubles each), then
+// the route's table block when it is staged.
+template <int L1, int L2, int L3, int L4, int ROUTE>
+struct KKetLds {
+    static constexpr int NR = (L1 + L2 + L3 + L4) / 2 + 1;
+    static constexpr bool LO = ROUTE == kRouteLo;
+    static constexpr bool STAGED = LO ? rys32_staged(NR) : rys_staged(NR);
+    static constexpr int ACC = kKGroup * ncart(L1) * ncart(L3);  // doubles per wave
+    static size_t bytes(const RysTable& tab)
+    {
+        size_t b = (size_t)kKWaves * ACC * sizeof(double);
+        if (STAGED)
+            b += LO ? rys32_lds_floats(tab.nbin[NR], NR) * sizeof(float) : rys_lds_doubles(tab.nbin[NR], NR) * sizeof(double);
+        return b;
+    }
+};
+
+// K build over ket groups.  Each wave of the workgroup takes one unit (bra row r1, ket group g);
+// for each 64-slot chunk of the group, lane l holds ket pair (r3, 4) of slot l in registers and the
+// wave walks r1's partners 2 in bound order (wave-uniform, scalar loads), stopping where even the
+// chunk's largest ket bound cannot pass with the largest D block.  Each lane tests the reference's
+// sieve float(b12 b34) max|D24 block| >= thresp for its (2, 4) and, where it passes, evaluates the
+// quartet into register accumulators for (r1, r3) (reference exchange.cpp:145-152, 186).  At the
+// end of a chunk the lanes add their accumulators into the wave's LDS block of the group (row
+// position t); at the end of the unit the block is added to K with one atomic per element, plus
+// the transposed block in the symmetric mode (K = K^T) for r3 != r1.
+template <int L1, int L2, int L3, int L4, int ROUTE>
+__global__ __launch_bounds__(64 * kKWaves) void k_ket(KArgs a, KGroups kg, const KUnitG* __restrict__ units, int nunits)
+{
+    using Lds = KKetLds<L1, L2, L3, L4, ROUTE>;
+    constexpr bool LO = Lds::LO;
+    constexpr bool STAGED = Lds::STAGED;
+    constexpr int NR = Lds::NR;
+    using Real = typename std::conditional<LO, float, double>::type;
+    constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4), N13 = N1 * N3;
+    extern __shared__ double s_dyn[];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double* s_acc = s_dyn + w * Lds::ACC;
+    void* s_tab = s_dyn + kKWaves * Lds::ACC;
+    if constexpr (STAGED) {
+        if constexpr (LO)
+            rys32_stage_lds<NR>(a.rys, static_cast<float*>(s_tab));
+        else
+            rys_stage_lds<NR>(a.rys, static_cast<double*>(s_tab));
+    }
+    for (int q = lane; q < Lds::ACC; q += 64) s_acc[q] = 0.0;
     __syncthreads();
-    if constexpr (GRAD) return;  // gradients were added per quartet
-    // flush the tile accumulator
-    const int o1 = a.s12.cart[r1];
-    for (int q = lane; q < (j3b - j3a) * N13; q += 64) {
-        const double v = s_acc[q];
-        if (v == 0.0) continue;
-        const int tt = q / N13, ij = q - tt * N13, i = ij / N3, j = ij - i * N3;
-        const int r3 = a.s34.rows[j3a + tt];
-        const int o3 = a.s34.cart[r3];
-        atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
-        if (a.symmetric && r1 != r3) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
+    // no workgroup barrier after this point: the waves are independent
+    const int u = blockIdx.x * kKWaves + w;
+    if (u >= nunits) return;
+    const int i1 = ld_const(&units[u].i1), g = ld_const(&units[u].g), rank1 = ld_const(&units[u].rank1);
+    const int r1 = ld_const(a.s12.rows + i1);
+    const int a0 = ld_const(a.s12.row_start + r1 * (a.s12.maxLp + 1) + L2);
+    const int a1 = ld_const(a.s12.row_start + r1 * (a.s12.maxLp + 1) + L2 + 1);
+    const int s0 = ld_const(kg.grp_slot + g), s1 = ld_const(kg.grp_slot + g + 1);
+
+    // quartets with est at or above `thresp` here (and, on the FP32 route, below thredp) are evaluated
+    const float thresp = (ROUTE == kRouteHi) ? a.thredp : a.thresp;
+    const double thr = (double)thresp;
+    const double thredp = (double)a.thredp;
+    const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
+    const float thre12 = (float)(thr / ((double)a.maxb34 * dmax));
+    const float thre34 = (float)(thr / ((double)a.maxb12 * dmax));
+    const float b12top = ld_const(a.s12.eb + a0);
+
+    unsigned long long nq = 0;
+    bool any = false;
+    for (int c0 = s0; c0 < s1; c0 += 64) {
+        const float b34max = ld_const(kg.chunk_b + (c0 >> 6));
+        // no quartet of this chunk can pass, and the later chunks of the group hold smaller bounds
+        if (!(b34max >= thre34) || (double)(b12top * b34max) * dmax < thr) break;
+        const int2 sl = kg.slot[c0 + lane];
+        const int e4 = max(sl.x, 0);
+        const float b34 = a.s34.eb[e4];
+        const bool valid = sl.x >= 0 && kg.grp_rank[g * kKGroup + sl.y] >= rank1 && b34 >= thre34;
+        const int blk4 = a.s34.eblk[e4];
+        KEnt<Real> E34;
+        if constexpr (LO)
+            E34 = load_ent32(a.s34.eg32, e4);
+        else
+            E34 = load_ent(a.s34.eg, e4);
+        const double* __restrict__ dsh = a.Dsh + blk4;
+        const double* __restrict__ dcol = a.D + E34.cart;
+        Real acc[N1][N3];
+#pragma unroll
+        for (int i = 0; i < N1; i++)
+#pragma unroll
+            for (int j = 0; j < N3; j++) acc[i][j] = Real(0);
+        bool hit = false;
+        // the D block maximum of the next partner is loaded one step ahead
+        double dnext = valid ? dsh[(size_t)ld_const(a.s12.eblk + a0) * a.ldDsh] : 0.0;
+        for (int k = a0; k < a1; k++) {
+            const float b12 = ld_const(a.s12.eb + k);
+            if (!(b12 >= thre12) || (double)(b12 * b34max) * dmax < thr) break;
+            const double dblk = dnext;
+            if (k + 1 < a1 && valid) dnext = dsh[(size_t)ld_const(a.s12.eblk + k + 1) * a.ldDsh];
+            bool pass = false;
+            if (valid) {
+                const double est = (double)(b12 * b34) * dblk;
+                pass = est >= thr;
+                if constexpr (LO) pass = pass && !(est >= thredp);
+            }
+            if (__ballot(pass) == 0ull) continue;
+            KEnt<Real> E12;
+            if constexpr (LO)
+                E12 = load_ent32_uniform(a.s12.eg32, k);
+            else
+                E12 = load_ent_uniform(a.s12.eg, k);
+            if (pass) {
+                Real Db[N2][N4];
+#pragma unroll
+                for (int b = 0; b < N2; b++)
+#pragma unroll
+                    for (int d = 0; d < N4; d++) Db[b][d] = (Real)dcol[(size_t)(E12.cart + b) * a.ldD + d];
+                k_quartet<Real, L1, L2, L3, L4, STAGED>(a.rys, s_tab, a.ew, E12, E34, Db, acc);
+                nq++;
+                hit = true;
+            }
+        }
+        if (hit) {
+#pragma unroll
+            for (int i = 0; i < N1; i++)
+#pragma unroll
+                for (int j = 0; j < N3; j++) atomicAdd(&s_acc[sl.y * N13 + i * N3 + j], (double)acc[i][j]);
+        }
+        any = any || (__ballot(hit) != 0ull);
+    }
+    if (any) {
+        // the lanes' LDS additions above are complete before any lane reads the block
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int o1 = ld_const(a.s12.cart + r1);
+        for (int q = lane; q < Lds::ACC; q += 64) {
+            const double v = s_acc[q];
+            if (v == 0.0) continue;
+            const int t = q / N13, ij = q - t * N13, i = ij / N3, j = ij - i * N3;
+            const int r3 = kg.grp_rows[g * kKGroup + t];
+            const int o3 = a.s34.cart[r3];
+            atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
+            if (a.symmetric && r1 != r3) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
+        }
     }
     if (a.count) {
 #pragma unroll
@@ -527,20 +691,26 @@ __global__ __launch_bounds__(64) void k_tiles(KArgs a, const KUnit* __restrict__
     }
 }
 
-// Launch of one class at row angular momentum L1 (instantiated in exchange_l<L1>.hip).
+// Launch of one K build class at row angular momentum L1 (instantiated in exchange_l<L1>.hip).
 template <int L1>
-void launch_k_quartets_l1(int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,
-                          hipStream_t s)
+void launch_k_ket_l1(int L2, int L3, int L4, int route, const KArgs& a, const KGroups& kg, const KUnitG* units,
+                     int nunits, hipStream_t s)
 {
-    dim3 grid((unsigned)nunits), block(64);
+    dim3 grid((unsigned)((nunits + kKWaves - 1) / kKWaves)), block(64 * kKWaves);
+#define ONE(B, C, D, R)                                                                             \
+    {                                                                                               \
+        const size_t lds = KKetLds<L1, B, C, D, R>::bytes(a.rys);                                   \
+        if (lds > kMaxLdsPerGroup) throw std::runtime_error("lsjk: k_ket LDS request too large");  \
+        hipLaunchKernelGGL((k_ket<L1, B, C, D, R>), grid, block, lds, s, a, kg, units, nunits);     \
+    }
 #define CASE(B, C, D)                                                                               \
     if (L2 == B && L3 == C && L4 == D) {                                                            \
         if (route == kRouteLo)                                                                      \
-            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRouteLo>), grid, block, 0, s, a, units);       \
+            ONE(B, C, D, kRouteLo)                                                                  \
         else if (route == kRouteHi)                                                                 \
-            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRouteHi>), grid, block, 0, s, a, units);       \
+            ONE(B, C, D, kRouteHi)                                                                  \
         else                                                                                        \
-            hipLaunchKernelGGL((k_tiles<L1, B, C, D, kRoutePlain>), grid, block, 0, s, a, units);    \
+            ONE(B, C, D, kRoutePlain)                                                               \
         return;                                                                                     \
     }
 #define C4(B, C) CASE(B, C, 0) CASE(B, C, 1) CASE(B, C, 2)
@@ -549,6 +719,7 @@ void launch_k_quartets_l1(int L2, int L3, int L4, int route, const KArgs& a, con
 #undef C3
 #undef C4
 #undef CASE
+#undef ONE
 }
 
 // Launch of one exchange-gradient class (instantiated per (L1, L2) in exchange_grad_<L1><L2>.hip).
@@ -558,7 +729,7 @@ void launch_k_grad_l12(int L3, int L4, const KArgs& a, const KUnit* units, int n
     dim3 grid((unsigned)nunits), block(64);
 #define CASE(C, D)                                                                                  \
     if (L3 == C && L4 == D) {                                                                       \
-        hipLaunchKernelGGL((k_tiles<L1, L2, C, D, kRouteGrad>), grid, block, 0, s, a, units);      \
+        hipLaunchKernelGGL((k_tiles_grad<L1, L2, C, D>), grid, block, 0, s, a, units);             \
         return;                                                                                     \
     }
     CASE(0, 0) CASE(0, 1) CASE(0, 2) CASE(1, 0) CASE(1, 1) CASE(1, 2) CASE(2, 0) CASE(2, 1) CASE(2, 2)

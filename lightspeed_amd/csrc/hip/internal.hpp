@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <map>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -83,10 +84,33 @@ struct PairList {
     unsigned long long uid = 0;
     // cached J task lists keyed by the ket list's uid
     std::map<unsigned long long, std::pair<int*, int>> jtasks;
-    // cached K unit lists (device pointer, count) keyed by (ket list uid, class, triangle flag)
+    // cached K gradient unit lists (device pointer, count) keyed by (ket list uid, class, triangle flag)
     std::map<std::pair<unsigned long long, int>, std::pair<void*, int>> kunits;
     // cost of the class's units (sum over units of partner-count products), same key
     std::map<std::pair<unsigned long long, int>, double> kcost;
+    // host copy of the entry bounds (K view order)
+    std::vector<float> h_eb;
+    // spatial (Morton) rank of each basis-1 primitive: the order of the ket groups, and the total
+    // order of the triangle mode; built on first use
+    std::vector<int> h_mrank;
+    // K ket groups per (row L, partner L) (kernels.hpp KGroups), built on first use
+    struct KGroupData {
+        int2* d_slot = nullptr;
+        int* d_grp_slot = nullptr;
+        int* d_grp_rows = nullptr;
+        int* d_grp_rank = nullptr;
+        float* d_chunk_b = nullptr;
+        int ngroup = 0;
+        std::vector<int> grp_slot;  // host copies
+        std::vector<int> grp_rank;
+        std::vector<float> chunk_b;
+        std::vector<int> order;     // indices into h_rows[L][Lp] in group order
+    };
+    std::map<int, KGroupData> kgroups;
+    // cached K build unit lists (device pointer, count) keyed by (ket list uid, class and
+    // triangle flag, thresp bits), and their estimated cost
+    std::map<std::pair<unsigned long long, long long>, std::pair<void*, int>> kunits_g;
+    std::map<std::pair<unsigned long long, long long>, double> kcost_g;
 };
 
 // ---------------------------------------------------------------- per-call argument blocks

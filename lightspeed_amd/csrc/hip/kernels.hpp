@@ -128,21 +128,45 @@ struct OneEArgs {
 void launch_one_e_st(int LA, int LB, const OneEArgs& a, int k0, int nk, hipStream_t s);
 void launch_one_e_v(int LA, int LB, const OneEArgs& a, int k0, int nk, int hstart, size_t hoff, hipStream_t s);
 
-// K: one wave per tile = (row1, kKTile consecutive row3s) of fixed angular momenta.
+// K gradient: one wave per tile = (row1, kKTile consecutive row3s) of fixed angular momenta.
 constexpr int kKTile = 16;
-// Row-1 partners per work unit: a tile's partner range is split into chunks of this size so that
-// no unit is much heavier than the rest.
+// Row-1 partners per gradient work unit: a tile's partner range is split into chunks of this size
+// so that no unit is much heavier than the rest.
 constexpr int kKChunk2 = 64;
-// A K work unit: row1 = rows1[i1], row3 block ib (rows3[ib*kKTile ...]), row1's partners
+// A K gradient work unit: row1 = rows1[i1], row3 block ib (rows3[ib*kKTile ...]), row1's partners
 // [e2off, e2off + e2cnt) of the class's partner angular momentum.
 struct KUnit {
     int i1, ib, e2off, e2cnt;
 };
-// A packed partner entry (row primitive A, partner primitive B): p, K_AB, P (3), P - A (3),
-// A - B (3), then the partner's Cartesian offset and block id as two ints in the last double.
-constexpr int kKEntDoubles = 12;
-// The FP32-route copy: P (3 doubles), p, K, P - A, A - B (8 floats), {cart, blk}: 64 bytes.
-constexpr int kKEnt32Doubles = 8;
+
+// K build: ket groups.  The rows of one (row L, partner L) class of the ket list are put in
+// spatial (Morton) order and cut into groups of kKGroup consecutive rows; the partner entries of a
+// group's rows are merged, sorted by bound (descending) and padded to a multiple of 64 slots.  A
+// wave holds 64 slots (one ket pair per lane) in registers and streams the bra row's partners
+// through scalar loads.
+constexpr int kKGroup = 16;
+constexpr int kKWaves = 4;  // waves (independent work units) per workgroup
+struct KGroups {
+    const int2* slot;      // [nslot] {entry index or -1 (padding), position of the row in the group}
+    const int* grp_slot;   // [ngroup + 1] first slot of each group
+    const int* grp_rows;   // [ngroup * kKGroup] row primitive at each position (-1: none)
+    const int* grp_rank;   // [ngroup * kKGroup] spatial rank of that row (INT_MAX: none)
+    const float* chunk_b;  // [nslot / 64] largest bound of each 64-slot chunk
+};
+// A K work unit: bra row rows1[i1] against ket group g.  rank1: in the triangle mode (one list,
+// symmetric D24) only ket rows of spatial rank >= rank1 (the bra row's) take part; -1 otherwise.
+struct KUnitG {
+    int i1, g, rank1, pad;
+};
+// A packed partner entry (row primitive A, partner primitive B), 128 bytes: p, Kp, P (3),
+// P - A (3), A - B (3), {the partner's Cartesian offset, its block id} (two ints), 1/(2p), 3 zero
+// pads.  Kp = K_AB sqrt(2 pi^(5/2)) / p with K_AB = c_A c_B exp(-a b / p |A - B|^2).
+constexpr int kKEntDoubles = 16;
+// sqrt(2 pi^(5/2))
+constexpr double kKPrefRoot = 5.914967172795613;
+// The FP32-route copy, 80 bytes: P (3 doubles), p, Kp, P - A, A - B, 1/(2p), 0 (10 floats),
+// {cart, blk}, pad.
+constexpr int kKEnt32Doubles = 10;
 struct KSide {
     const int* row_start;  // [nrow * (maxLp+1) + 1]
     int maxLp;
@@ -239,8 +263,12 @@ int persistent_grid(const void* kernel, int block, size_t lds, long long nwork_g
 void launch_k_blockmax(const double* D, int ldD, const int* blk_cart2, const int* blk_L2, int nblk2,
                        const int* blk_cart4, const int* blk_L4, int nblk4, double* Dsh, CallScalars* sc,
                        hipStream_t s);
+// Exchange gradient classes (route kRouteGrad) over KUnit tiles.
 void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,
                        hipStream_t s);
+// K build classes (routes plain, FP64 part, FP32 part) over ket-group units.
+void launch_k_ket(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KGroups& g, const KUnitG* units,
+                  int nunits, hipStream_t s);
 void launch_fill(void* p, size_t bytes, hipStream_t s);
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
