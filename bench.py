@@ -22,9 +22,11 @@ events around each launch on the stream it runs on, measured in `--attribution-s
 steps after the timed region with the class launches serialised, so each duration is the
 kernel's own): algorithmic FLOPs per launch = sum over its classes of N(class) x F(class),
 F = 40 + nrys (30 + 9 G + 3 Pi) + 2 Pi (Pi = N1 N2 N3 N4, G = (L1+L2+1)(L3+L4+1),
-nrys = (L1+L2+L3+L4)/2 + 1), against the FP64 peak 78.6 TFLOP/s (FP32-route launches: the FP32
-peak 157.3 TFLOP/s).  For a J launch the FP32 share of its classes' quartets is the launch's
-evaluated FP32 fraction.  jk_frac = [sum_FP64 N F / 78.6e12 + sum_FP32 N F / 157.3e12] / t_JK.
+nrys = (L1+L2+L3+L4)/2 + 1), against the peak of the launch's mix: its FP64 FLOPs at the FP64
+peak 78.6 TFLOP/s and its FP32 FLOPs at the FP32 peak 157.3 TFLOP/s (on the mixed route one
+launch per class evaluates both parts).  For a J launch the FP32 share of its classes' quartets
+is the launch's evaluated FP32 fraction.
+jk_frac = [sum_FP64 N F / 78.6e12 + sum_FP32 N F / 157.3e12] / t_JK.
 cpu_baseline: the reference's own CPU IntBox::coulombCPU + IntBox::exchangeCPU, compiled from
 its sources into oracle/_ref/ref_jk, timed on a bounded sample of the same workload (the bra list
 keeps every `--cpu-stride`-th pair of each class, the ket list is whole), all threads of the host.
@@ -48,7 +50,6 @@ sys.path.insert(0, ROOT)
 METRIC = "J+K build time (s) & ERI quartets/s, mixed vs FP64 GPU at |ΔE|≤1e-6 Eh"
 FP64_PEAK = 78.6e12
 FP32_PEAK = 157.3e12
-LO_ID = 4096  # launch ids of FP32-route launches are offset by this
 
 
 def log(msg):
@@ -242,35 +243,39 @@ def main():
     cj, cj32, ck, ck32 = cnt[head]
 
     def launch_flops(kind, lid):
-        lo = lid >= LO_ID
-        idx = lid - LO_ID if lo else lid
+        """(FP64 FLOPs, FP32 FLOPs) of one launch."""
+        idx = lid
+        # one launch per class evaluates both parts of the mixed route
         if kind == "J":
             L12, L34 = idx // 5, idx % 5
             full = sum(NJ[c] * cls_flops(c) for c in range(625)
                        if sum(cls_of(c)[:2]) == L12 and sum(cls_of(c)[2:]) == L34)
             f32 = cj32[idx] / cj[idx] if cj[idx] else 0.0
-            return full * (f32 if lo else 1.0 - f32), lo
+            return full * (1.0 - f32), full * f32
         c = idx - 256
-        n = ck32[idx] if lo else ck[idx] - ck32[idx]
-        return n * cls_flops(c), lo
+        return (ck[idx] - ck32[idx]) * cls_flops(c), ck32[idx] * cls_flops(c)
 
     def launch_name(kind, lid):
-        lo = lid >= LO_ID
-        idx = lid - LO_ID if lo else lid
-        # template arguments as the profiler prints them: route 0 plain FP64, 1 FP64 part, 2 FP32 part
-        route = ", %d" % (2 if lo else (1 if head == "mixed" else 0))
+        idx = lid
+        # template arguments as the profiler prints them: route 0 plain FP64, 4 mixed (FP64 and FP32
+        # parts in one launch)
         if kind == "J":
-            return "j_quartets<%d, %d%s>" % (idx // 5, idx % 5, route)
-        return "k_ket<%d, %d, %d, %d%s>" % (cls_of(idx - 256) + (route,))
+            return "j_quartets<%d, %d, %d>" % (idx // 5, idx % 5, 4 if head == "mixed" else 0)
+        return "k_ket<%d, %d, %d, %d, %d>" % (cls_of(idx - 256) + (4 if head == "mixed" else 0,))
 
     means = {k: float(np.mean(v)) for k, v in launch_ms.items()}
+    log("serialised quartet kernels per step: J %.3f ms, K %.3f ms" % (
+        sum(v for k, v in means.items() if k[0] == "J"), sum(v for k, v in means.items() if k[0] == "K")))
     for k in sorted(means, key=lambda k: -means[k])[:24]:
-        fl, lo = launch_flops(*k)
+        fl = sum(launch_flops(*k))
         log("  launch %-36s %9.3f ms  %8.3f TFLOP/s" % (launch_name(*k), means[k], fl / (means[k] * 1e-3) / 1e12))
     dom = max(means, key=lambda k: means[k])
-    dom_fl, dom_lo = launch_flops(*dom)
-    peak = FP32_PEAK if dom_lo else FP64_PEAK
+    dom64, dom32 = launch_flops(*dom)
+    dom_fl = dom64 + dom32
+    # peak of the launch's mix: FP64 FLOPs at the FP64 peak, FP32 FLOPs at the FP32 peak
+    peak = dom_fl / (dom64 / FP64_PEAK + dom32 / FP32_PEAK) if dom_fl > 0 else FP64_PEAK
     achieved = dom_fl / (means[dom] * 1e-3)
+    dom_lo = dom32 > 0.0
     # whole-step fraction: sum_FP64 N F / 78.6e12 + sum_FP32 N F / 157.3e12, over the step time
     n32 = {c: 0.0 for c in range(625)}
     for c in range(625):

@@ -41,9 +41,6 @@ struct LsjkError : public std::runtime_error {
 thread_local std::string g_err;
 std::recursive_mutex g_mu;
 
-// Per-launch ids of FP32-route launches are the class's counter index plus this offset.
-constexpr int kLoRouteId = 4096;
-
 struct Session {
     bool init = false;
     int device = 0;
@@ -634,6 +631,9 @@ const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp)
     return pl->kgroups.emplace(key, std::move(G)).first->second;
 }
 
+// Work units per K class below which the groups' chunk ranges are split.
+constexpr long long kKUnitsTarget = 12288;
+
 // K build units of one class: (bra row, ket group) pairs.  Rows are taken in spatial order and,
 // in the triangle mode, groups lying wholly before the bra row in that order are dropped.  The
 // cost estimate of a unit is the number of (bra partner, ket chunk) steps that pass the bound
@@ -690,6 +690,42 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
                 v.push_back({c, KUnitG{i1, g, rank1, 0}});
                 total += (double)c;
             }
+        }
+        // classes with few (row, group) pairs: split the groups' chunk ranges so that the launch
+        // has enough waves to fill the device
+        long long nchunk = 0;
+        for (const auto& x : v) nchunk += (G34.grp_slot[x.second.g + 1] - G34.grp_slot[x.second.g]) / 64;
+        const int per = (int)std::max<long long>(1, std::min<long long>(
+            (long long)(v.size() >= kKUnitsTarget ? 1 << 15 : (nchunk + kKUnitsTarget - 1) / kKUnitsTarget), 1 << 15));
+        {
+            std::vector<std::pair<long long, KUnitG>> w;
+            w.reserve(v.size());
+            for (const auto& x : v) {
+                const int g = x.second.g;
+                const int nch = (G34.grp_slot[g + 1] - G34.grp_slot[g]) / 64;
+                if (nch >= (1 << 15)) throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk: exchange ket group too large");
+                const int r1 = rows1[x.second.i1];
+                const int a0 = pl12->h_row_start[(size_t)r1 * m12 + L2], a1 = pl12->h_row_start[(size_t)r1 * m12 + L2 + 1];
+                // equal pieces of at most `per` chunks
+                const int npiece = (nch + per - 1) / per, step = (nch + npiece - 1) / npiece;
+                for (int c0 = 0; c0 < nch; c0 += step) {
+                    const int nc = std::min(step, nch - c0);
+                    long long c = 0;
+                    if (step < nch)
+                        for (int k = 0; k < nc; k++) {
+                            const double x0 = (double)thresp / (double)G34.chunk_b[G34.grp_slot[g] / 64 + c0 + k];
+                            c += (long long)(std::partition_point(pl12->h_eb.begin() + a0, pl12->h_eb.begin() + a1,
+                                                                  [&](float b) { return (double)b >= x0; }) -
+                                             (pl12->h_eb.begin() + a0));
+                        }
+                    else
+                        c = x.first;
+                    KUnitG uu = x.second;
+                    uu.crange = c0 | (nc << 16);
+                    w.push_back({c, uu});
+                }
+            }
+            v.swap(w);
         }
         auto bucket = [](long long c) { return c > 0 ? 64 - __builtin_clzll((unsigned long long)c) : 0; };
         std::stable_sort(v.begin(), v.end(), [&](const std::pair<long long, KUnitG>& a,
@@ -1180,16 +1216,9 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
                     const int idx = jcount_index(L12, L34);
                     hipStream_t ss = fo.pick();
                     lt.begin(idx, ss);
-                    launch_j_quartets(L12, L34, mixed ? kRouteHi : kRoutePlain, qa, tk, seg.cnt[L12][L34],
+                    launch_j_quartets(L12, L34, mixed ? kRouteMixed : kRoutePlain, qa, tk, seg.cnt[L12][L34],
                                       &S.d_sc->queue[idx], ss);
                     lt.end();
-                    if (mixed) {
-                        // the FP32 launch of the class: launch id and queue counter offset by kLoRouteId
-                        hipStream_t sl = fo.pick();
-                        lt.begin(idx + kLoRouteId, sl);
-                        launch_j_quartets(L12, L34, kRouteLo, qa, tk, seg.cnt[L12][L34], &S.d_sc->queue[idx + 32], sl);
-                        lt.end();
-                    }
                 }
             }
             fo.end();
@@ -1332,14 +1361,8 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                 const int idx = kcount_index(k.L1, k.L2, k.L3, k.L4);
                 hipStream_t ss = fo.pick();
                 lt.begin(idx, ss);
-                launch_k_ket(k.L1, k.L2, k.L3, k.L4, mixed ? kRouteHi : kRoutePlain, k.ka, k.kg, k.units, k.nunits, ss);
+                launch_k_ket(k.L1, k.L2, k.L3, k.L4, mixed ? kRouteMixed : kRoutePlain, k.ka, k.kg, k.units, k.nunits, ss);
                 lt.end();
-                if (mixed) {
-                    hipStream_t sl = fo.pick();
-                    lt.begin(idx + kLoRouteId, sl);
-                    launch_k_ket(k.L1, k.L2, k.L3, k.L4, kRouteLo, k.ka, k.kg, k.units, k.nunits, sl);
-                    lt.end();
-                }
             }
             fo.end();
             HIPCHK(hipGetLastError());

@@ -244,26 +244,44 @@ __device__ __forceinline__ void hermite_contract_factorized(const RysTable& tab,
     });
 }
 
+// LDS of one j_quartets workgroup: the FP64 table block (routes with FP64 quartets, nrys <= 3),
+// then the FP32 block (routes with FP32 quartets).
+template <int L12, int L34, int ROUTE>
+struct JLds {
+    static constexpr int NR = (L12 + L34) / 2 + 1;
+    static constexpr bool HAS32 = ROUTE == kRouteLo || ROUTE == kRouteMixed;
+    static constexpr bool STAGED64 = ROUTE != kRouteLo && rys_staged(NR);
+    static __host__ __device__ size_t bytes64(const RysTable& tab)
+    {
+        return STAGED64 ? rys_lds_doubles(tab.nbin[NR], NR) * sizeof(double) : 0;
+    }
+    static size_t bytes(const RysTable& tab)
+    {
+        return bytes64(tab) + (HAS32 ? rys32_lds_floats(tab.nbin[NR], NR) * sizeof(float) : 0);
+    }
+};
+
 // Persistent kernel: each wave takes (64 bra pairs x ket range) tasks from a queue until it is
 // empty.  The workgroup's waves share an LDS copy of the Rys table (FP64 rule for nrys <= 3 on the
-// FP64 routes, the FP32 rule on the FP32 route).  Lanes own bra pairs; the kets are read at
-// wave-uniform addresses (scalar loads), in bound order.  ROUTE selects the quartets by
-// est = B12 * B34' (kRoutePlain / kRouteHi / kRouteLo).
+// routes with FP64 quartets, the FP32 rule on the routes with FP32 quartets).  Lanes own bra
+// pairs; the kets are read at wave-uniform addresses (scalar loads), in bound order.  ROUTE
+// selects the quartets by est = B12 * B34' (kRoutePlain / kRouteHi / kRouteLo), or evaluates
+// the mixed route in one launch (kRouteMixed): a ket step with any quartet at or above thredp is
+// evaluated in FP64, the others in FP32.
 template <int L12, int L34, int ROUTE>
 __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* __restrict__ tasks, int ntask,
                                                   unsigned int* qctr)
 {
+    using Lds = JLds<L12, L34, ROUTE>;
     constexpr int H12 = nherm(L12), H34 = nherm(L34), L = L12 + L34, HL = nherm(L), NR = L / 2 + 1;
     constexpr bool LO = ROUTE == kRouteLo;
-    using Real = typename std::conditional<LO, float, double>::type;
-    extern __shared__ double s_tab[];
-    if constexpr (LO) {
-        rys32_stage_lds<NR>(a.rys, reinterpret_cast<float*>(s_tab));
-        __syncthreads();
-    } else if constexpr (rys_staged(NR)) {
-        rys_stage_lds<NR>(a.rys, s_tab);
-        __syncthreads();
-    }
+    constexpr bool MIXED = ROUTE == kRouteMixed;
+    extern __shared__ double s_dyn[];
+    double* s_tab64 = s_dyn;
+    float* s_tab32 = reinterpret_cast<float*>(reinterpret_cast<char*>(s_dyn) + Lds::bytes64(a.rys));
+    if constexpr (Lds::STAGED64) rys_stage_lds<NR>(a.rys, s_tab64);
+    if constexpr (Lds::HAS32) rys32_stage_lds<NR>(a.rys, s_tab32);
+    if constexpr (Lds::STAGED64 || Lds::HAS32) __syncthreads();
 
     const int lane = threadIdx.x & 63;
     const float thresp = a.thresp;
@@ -272,7 +290,7 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
     const float max34 = __uint_as_float(a.sc->max34_bits);
     const float thre_bra = thresp / max34;
     const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
-    unsigned long long nq = 0;
+    unsigned long long nq = 0, nq32 = 0;
 
     for (;;) {
         const int t = (int)wave_take(qctr);
@@ -308,27 +326,28 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
             const float est = B12 * ld_const(a.ket_b + k);
             bool take = bra_ok && est >= thre_lo;
             if constexpr (LO) take = take && !(est >= a.thredp);
+            bool step32 = LO;
+            if constexpr (MIXED) step32 = __ballot(take && est >= a.thredp) == 0ull;
             if (!take) continue;
             nq++;
+            if (step32) nq32++;
             const double q = ld_const(a.ket_geom + k);
             const double X = Px - ld_const(a.ket_geom + a.ket_n + k);
             const double Y = Py - ld_const(a.ket_geom + 2 * a.ket_n + k);
             const double Z = Pz - ld_const(a.ket_geom + 3 * a.ket_n + k);
             const size_t dofs = (size_t)(k - a.ket_hstart) * H34;
-            Real dreg[H34];
+            // one quartet in the arithmetic of `acc`'s element type, reading the ket density of
+            // that precision and the matching table block
+            auto contract = [&](auto& acc, const auto* dsrc, const void* tab) __attribute__((always_inline)) {
+                using Real = typename std::remove_reference<decltype(acc[0])>::type;
+                Real dreg[H34];
 #pragma unroll
-            for (int u = 0; u < H34; u++) {
-                if constexpr (LO)
-                    dreg[u] = ld_const(a.ket_d32 + dofs + u);
-                else
-                    dreg[u] = ld_const(a.ket_d + dofs + u);
-            }
-            auto contract = [&](auto& acc) __attribute__((always_inline)) {
+                for (int u = 0; u < H34; u++) dreg[u] = ld_const(dsrc + dofs + u);
                 if constexpr (L >= 5) {
-                    hermite_contract_factorized<Real, L12, L34>(a.rys, s_tab, a.ew, p12, q, X, Y, Z, dreg, acc);
+                    hermite_contract_factorized<Real, L12, L34>(a.rys, tab, a.ew, p12, q, X, Y, Z, dreg, acc);
                 } else {
                     Real R[HL];
-                    hermite_R<Real, L>(a.rys, s_tab, a.ew, p12, q, X, Y, Z, R);
+                    hermite_R<Real, L>(a.rys, tab, a.ew, p12, q, X, Y, Z, R);
                     static_for<H34>([&](auto U_) {
                         constexpr int u = decltype(U_)::value;
                         constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
@@ -340,16 +359,24 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
                     });
                 }
             };
-            if constexpr (LO) {
+            auto fp32 = [&]() __attribute__((always_inline)) {
                 // FP32 quartet, FP64 accumulation
                 float jq[H12];
 #pragma unroll
                 for (int u = 0; u < H12; u++) jq[u] = 0.0f;
-                contract(jq);
+                contract(jq, a.ket_d32, s_tab32);
 #pragma unroll
                 for (int u = 0; u < H12; u++) jp[u] += (double)jq[u];
+            };
+            if constexpr (LO) {
+                fp32();
+            } else if constexpr (MIXED) {
+                if (step32)
+                    fp32();
+                else
+                    contract(jp, a.ket_d, s_tab64);
             } else {
-                contract(jp);
+                contract(jp, a.ket_d, s_tab64);
             }
         }
         if (bra_ok) {
@@ -360,11 +387,14 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
         }
     }
     if (a.count) {
-        // per-wave reduction of the evaluated-quartet count
-        for (int o = 32; o > 0; o >>= 1) nq += __shfl_xor(nq, o);
+        // per-wave reduction of the evaluated-quartet counts
+        for (int o = 32; o > 0; o >>= 1) {
+            nq += __shfl_xor(nq, o);
+            nq32 += __shfl_xor(nq32, o);
+        }
         if (lane == 0 && nq) {
             atomicAdd(&a.sc->counts[a.count_cls], nq);
-            if constexpr (LO) atomicAdd(&a.sc->counts32[a.count_cls], nq);
+            if (nq32) atomicAdd(&a.sc->counts32[a.count_cls], nq32);
         }
     }
 }
@@ -551,12 +581,7 @@ void launch_j_grad_scatter(int LA, int LB, const JGradArgs& a, int k0, int nk, i
 template <int L12, int L34, int ROUTE>
 static void launch_jq(const JQuartetArgs& a, const JTask* tasks, int ntask, unsigned int* qctr, hipStream_t s)
 {
-    constexpr int NR = (L12 + L34) / 2 + 1;
-    size_t lds = 0;
-    if (ROUTE == kRouteLo)
-        lds = rys32_lds_floats(a.rys.nbin[NR], NR) * sizeof(float);
-    else if (rys_staged(NR))
-        lds = rys_lds_doubles(a.rys.nbin[NR], NR) * sizeof(double);
+    const size_t lds = JLds<L12, L34, ROUTE>::bytes(a.rys);
     const int grid = persistent_grid((const void*)j_quartets<L12, L34, ROUTE>, 256, lds, ((long long)ntask + 3) / 4);
     hipLaunchKernelGGL((j_quartets<L12, L34, ROUTE>), dim3(grid), dim3(256), lds, s, a, tasks, ntask, qctr);
 }
@@ -567,10 +592,8 @@ void launch_j_quartets(int L12, int L34, int route, const JQuartetArgs& a, const
     if (ntask <= 0) return;
 #define CASE(X, Y)                                                                     \
     if (L12 == X && L34 == Y) {                                                        \
-        if (route == kRouteLo)                                                         \
-            launch_jq<X, Y, kRouteLo>(a, tasks, ntask, qctr, s);                      \
-        else if (route == kRouteHi)                                                    \
-            launch_jq<X, Y, kRouteHi>(a, tasks, ntask, qctr, s);                      \
+        if (route == kRouteMixed)                                                      \
+            launch_jq<X, Y, kRouteMixed>(a, tasks, ntask, qctr, s);                   \
         else                                                                           \
             launch_jq<X, Y, kRoutePlain>(a, tasks, ntask, qctr, s);                   \
         return;                                                                        \

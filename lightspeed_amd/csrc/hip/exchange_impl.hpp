@@ -177,13 +177,14 @@ __device__ __forceinline__ Real krsqrt(Real x)
         return rsqrt(x);
 }
 
-// One primitive quartet: acc[a][c] += sum_{b,d} (ab|cd) Db[b][d], in the route's arithmetic.
-// STAGED: the table block of this nrys was copied to LDS at s_tab (FP64 block on the FP64 route,
-// FP32 block on the FP32 route); otherwise the global table is read.
+// One primitive quartet: acc[a][c] += sum_{b,d} (ab|cd) Db[b][d], evaluated in the route's
+// arithmetic (Real) and added to the FP64 accumulator.  STAGED: the table block of this nrys was
+// copied to LDS at s_tab (FP64 block for Real = double, FP32 block for Real = float); otherwise
+// the global table is read.
 template <typename Real, int L1, int L2, int L3, int L4, bool STAGED>
 __device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab, const Ewald& ew,
                                           const KEnt<Real>& e12, const KEnt<Real>& e34,
-                                          const Real (&Db)[ncart(L2)][ncart(L4)], Real (&acc)[ncart(L1)][ncart(L3)])
+                                          const Real (&Db)[ncart(L2)][ncart(L4)], double (&acc)[ncart(L1)][ncart(L3)])
 {
     constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4);
     constexpr int NR = (L1 + L2 + L3 + L4) / 2 + 1;
@@ -203,6 +204,15 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab
     const Real CDx = e34.ABx, CDy = e34.ABy, CDz = e34.ABz;
     const Real qf = q * ipq, pf = p * ipq;  // rho/p, rho/q
     const Real i2p = e12.i2p, i2q = e34.i2p;
+    // FP32 quartets: the roots' sum is formed in FP32 and added to the FP64 accumulator once
+    constexpr bool FP32 = std::is_same<Real, float>::value;
+    Real qacc[N1][N3];
+    if constexpr (FP32) {
+#pragma unroll
+        for (int a = 0; a < N1; a++)
+#pragma unroll
+            for (int c = 0; c < N3; c++) qacc[a][c] = Real(0);
+    }
     for (int it = 0; it < ew.n; it++) {
         const double om = ew.omega[it];
         Real d2 = Real(1);
@@ -247,10 +257,19 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab
                             s = fma(Ix.v[ax][bx][cx][dx] * Iy.v[ay][by][cy][dy] * Iz.v[az][bz][cz][dz], Db[b][d], s);
                         });
                     });
-                    acc[a][c] += s;
+                    if constexpr (FP32)
+                        qacc[a][c] += s;
+                    else
+                        acc[a][c] += s;
                 });
             });
         }
+    }
+    if constexpr (FP32) {
+#pragma unroll
+        for (int a = 0; a < N1; a++)
+#pragma unroll
+            for (int c = 0; c < N3; c++) acc[a][c] += (double)qacc[a][c];
     }
 }
 
@@ -538,61 +557,89 @@ __device__ __forceinline__ KEnt<float> load_ent32_uniform(const double* __restri
 }
 
 // LDS of one k_ket workgroup: kKWaves group accumulators (kKGroup x N1 x N3 doubles each), then
-// the route's table block when it is staged.
+// the staged table blocks: FP64 (routes with FP64 quartets, nrys <= 3), then FP32 (routes with
+// FP32 quartets, nrys <= 5).
 template <int L1, int L2, int L3, int L4, int ROUTE>
 struct KKetLds {
     static constexpr int NR = (L1 + L2 + L3 + L4) / 2 + 1;
-    static constexpr bool LO = ROUTE == kRouteLo;
-    static constexpr bool STAGED = LO ? rys32_staged(NR) : rys_staged(NR);
+    static constexpr bool HAS32 = ROUTE == kRouteLo || ROUTE == kRouteMixed;
+    static constexpr bool HAS64 = ROUTE != kRouteLo;
+    static constexpr bool STAGED64 = HAS64 && rys_staged(NR);
+    static constexpr bool STAGED32 = HAS32 && rys32_staged(NR);
     static constexpr int ACC = kKGroup * ncart(L1) * ncart(L3);  // doubles per wave
+    static __host__ __device__ size_t bytes64(const RysTable& tab)
+    {
+        return STAGED64 ? rys_lds_doubles(tab.nbin[NR], NR) * sizeof(double) : 0;
+    }
     static size_t bytes(const RysTable& tab)
     {
-        size_t b = (size_t)kKWaves * ACC * sizeof(double);
-        if (STAGED)
-            b += LO ? rys32_lds_floats(tab.nbin[NR], NR) * sizeof(float) : rys_lds_doubles(tab.nbin[NR], NR) * sizeof(double);
+        size_t b = (size_t)kKWaves * ACC * sizeof(double) + bytes64(tab);
+        if (STAGED32) b += rys32_lds_floats(tab.nbin[NR], NR) * sizeof(float);
         return b;
     }
 };
 
-// K build over ket groups.  Each wave of the workgroup takes one unit (bra row r1, ket group g);
-// for each 64-slot chunk of the group, lane l holds ket pair (r3, 4) of slot l in registers and the
+// An FP64 entry rounded to the FP32 route's arithmetic (the values of its FP32 record).
+__device__ __forceinline__ KEnt<float> ent_to_f32(const KEnt<double>& e)
+{
+    KEnt<float> f;
+    f.Px = e.Px;
+    f.Py = e.Py;
+    f.Pz = e.Pz;
+    f.p = (float)e.p;
+    f.K = (float)e.K;
+    f.PAx = (float)e.PAx;
+    f.PAy = (float)e.PAy;
+    f.PAz = (float)e.PAz;
+    f.ABx = (float)e.ABx;
+    f.ABy = (float)e.ABy;
+    f.ABz = (float)e.ABz;
+    f.i2p = (float)e.i2p;
+    f.cart = e.cart;
+    f.blk = e.blk;
+    return f;
+}
+
+// K build over ket groups.  Each wave of the workgroup takes one unit (bra row r1, chunks of ket
+// group g); for each 64-slot chunk, lane l holds ket pair (r3, 4) of slot l in registers and the
 // wave walks r1's partners 2 in bound order (wave-uniform, scalar loads), stopping where even the
 // chunk's largest ket bound cannot pass with the largest D block.  Each lane tests the reference's
 // sieve float(b12 b34) max|D24 block| >= thresp for its (2, 4) and, where it passes, evaluates the
-// quartet into register accumulators for (r1, r3) (reference exchange.cpp:145-152, 186).  At the
-// end of a chunk the lanes add their accumulators into the wave's LDS block of the group (row
-// position t); at the end of the unit the block is added to K with one atomic per element, plus
-// the transposed block in the symmetric mode (K = K^T) for r3 != r1.
+// quartet into FP64 register accumulators for (r1, r3) (reference exchange.cpp:145-152, 186).  On
+// the mixed route (one launch) quartets with est >= thredp are evaluated in FP64 and the others in
+// FP32, except that a wave step holding any FP64 quartet evaluates all of its quartets in FP64
+// (never less precision than the route asks for, and one evaluation per step).  At the end of a chunk the lanes add their
+// accumulators into the wave's LDS block of the group (row position t); at the end of the unit
+// the block is added to K with one atomic per element, plus the transposed block in the symmetric
+// mode (K = K^T) for r3 != r1.
 template <int L1, int L2, int L3, int L4, int ROUTE>
 __global__ __launch_bounds__(64 * kKWaves) void k_ket(KArgs a, KGroups kg, const KUnitG* __restrict__ units, int nunits)
 {
     using Lds = KKetLds<L1, L2, L3, L4, ROUTE>;
-    constexpr bool LO = Lds::LO;
-    constexpr bool STAGED = Lds::STAGED;
+    constexpr bool LO = ROUTE == kRouteLo;        // FP32 only
+    constexpr bool SPLIT = ROUTE == kRouteMixed;  // FP64 and FP32 in one launch
     constexpr int NR = Lds::NR;
-    using Real = typename std::conditional<LO, float, double>::type;
     constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4), N13 = N1 * N3;
     extern __shared__ double s_dyn[];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     double* s_acc = s_dyn + w * Lds::ACC;
-    void* s_tab = s_dyn + kKWaves * Lds::ACC;
-    if constexpr (STAGED) {
-        if constexpr (LO)
-            rys32_stage_lds<NR>(a.rys, static_cast<float*>(s_tab));
-        else
-            rys_stage_lds<NR>(a.rys, static_cast<double*>(s_tab));
-    }
+    double* s_tab64 = s_dyn + kKWaves * Lds::ACC;
+    float* s_tab32 = reinterpret_cast<float*>(reinterpret_cast<char*>(s_tab64) + Lds::bytes64(a.rys));
+    if constexpr (Lds::STAGED64) rys_stage_lds<NR>(a.rys, s_tab64);
+    if constexpr (Lds::STAGED32) rys32_stage_lds<NR>(a.rys, s_tab32);
     for (int q = lane; q < Lds::ACC; q += 64) s_acc[q] = 0.0;
     __syncthreads();
     // no workgroup barrier after this point: the waves are independent
     const int u = blockIdx.x * kKWaves + w;
     if (u >= nunits) return;
     const int i1 = ld_const(&units[u].i1), g = ld_const(&units[u].g), rank1 = ld_const(&units[u].rank1);
+    const int crange = ld_const(&units[u].crange);
     const int r1 = ld_const(a.s12.rows + i1);
     const int a0 = ld_const(a.s12.row_start + r1 * (a.s12.maxLp + 1) + L2);
     const int a1 = ld_const(a.s12.row_start + r1 * (a.s12.maxLp + 1) + L2 + 1);
-    const int s0 = ld_const(kg.grp_slot + g), s1 = ld_const(kg.grp_slot + g + 1);
+    const int s0 = ld_const(kg.grp_slot + g) + 64 * (crange & 0xffff);
+    const int s1 = min(ld_const(kg.grp_slot + g + 1), s0 + 64 * (crange >> 16));
 
     // quartets with est at or above `thresp` here (and, on the FP32 route, below thredp) are evaluated
     const float thresp = (ROUTE == kRouteHi) ? a.thredp : a.thresp;
@@ -603,7 +650,7 @@ __global__ __launch_bounds__(64 * kKWaves) void k_ket(KArgs a, KGroups kg, const
     const float thre34 = (float)(thr / ((double)a.maxb12 * dmax));
     const float b12top = ld_const(a.s12.eb + a0);
 
-    unsigned long long nq = 0;
+    unsigned long long nq = 0, nq32 = 0;
     bool any = false;
     for (int c0 = s0; c0 < s1; c0 += 64) {
         const float b34max = ld_const(kg.chunk_b + (c0 >> 6));
@@ -614,18 +661,22 @@ __global__ __launch_bounds__(64 * kKWaves) void k_ket(KArgs a, KGroups kg, const
         const float b34 = a.s34.eb[e4];
         const bool valid = sl.x >= 0 && kg.grp_rank[g * kKGroup + sl.y] >= rank1 && b34 >= thre34;
         const int blk4 = a.s34.eblk[e4];
-        KEnt<Real> E34;
-        if constexpr (LO)
-            E34 = load_ent32(a.s34.eg32, e4);
-        else
+        KEnt<double> E34;
+        KEnt<float> E34f;
+        if constexpr (LO) {
+            E34f = load_ent32(a.s34.eg32, e4);
+        } else {
             E34 = load_ent(a.s34.eg, e4);
+            if constexpr (SPLIT) E34f = ent_to_f32(E34);
+        }
+        const int cart4 = LO ? E34f.cart : E34.cart;
         const double* __restrict__ dsh = a.Dsh + blk4;
-        const double* __restrict__ dcol = a.D + E34.cart;
-        Real acc[N1][N3];
+        const double* __restrict__ dcol = a.D + cart4;
+        double acc[N1][N3];
 #pragma unroll
         for (int i = 0; i < N1; i++)
 #pragma unroll
-            for (int j = 0; j < N3; j++) acc[i][j] = Real(0);
+            for (int j = 0; j < N3; j++) acc[i][j] = 0.0;
         bool hit = false;
         // the D block maximum of the next partner is loaded one step ahead
         double dnext = valid ? dsh[(size_t)ld_const(a.s12.eblk + a0) * a.ldDsh] : 0.0;
@@ -634,26 +685,56 @@ __global__ __launch_bounds__(64 * kKWaves) void k_ket(KArgs a, KGroups kg, const
             if (!(b12 >= thre12) || (double)(b12 * b34max) * dmax < thr) break;
             const double dblk = dnext;
             if (k + 1 < a1 && valid) dnext = dsh[(size_t)ld_const(a.s12.eblk + k + 1) * a.ldDsh];
-            bool pass = false;
+            bool pass = false, low = LO;
             if (valid) {
                 const double est = (double)(b12 * b34) * dblk;
                 pass = est >= thr;
                 if constexpr (LO) pass = pass && !(est >= thredp);
+                if constexpr (SPLIT) low = !(est >= thredp);
             }
-            if (__ballot(pass) == 0ull) continue;
-            KEnt<Real> E12;
-            if constexpr (LO)
-                E12 = load_ent32_uniform(a.s12.eg32, k);
-            else
-                E12 = load_ent_uniform(a.s12.eg, k);
+            const unsigned long long m = __ballot(pass);
+            if (m == 0ull) continue;
+            if constexpr (ROUTE == kRoutePlain || ROUTE == kRouteHi) {
+                const KEnt<double> E12 = load_ent_uniform(a.s12.eg, k);
+                if (pass) {
+                    double Db[N2][N4];
+#pragma unroll
+                    for (int b = 0; b < N2; b++)
+#pragma unroll
+                        for (int d = 0; d < N4; d++) Db[b][d] = dcol[(size_t)(E12.cart + b) * a.ldD + d];
+                    k_quartet<double, L1, L2, L3, L4, Lds::STAGED64>(a.rys, s_tab64, a.ew, E12, E34, Db, acc);
+                    nq++;
+                    hit = true;
+                }
+                continue;
+            }
+            // mixed route: a wave step with any quartet at or above thredp evaluates all its
+            // quartets in FP64 (at least the precision the route asks for); the other steps in FP32
+            const bool step32 = LO || (SPLIT && __ballot(pass && !low) == 0ull);
+            if (!step32) {
+                const KEnt<double> E12 = load_ent_uniform(a.s12.eg, k);
+                if (pass) {
+                    double Db[N2][N4];
+#pragma unroll
+                    for (int b = 0; b < N2; b++)
+#pragma unroll
+                        for (int d = 0; d < N4; d++) Db[b][d] = dcol[(size_t)(E12.cart + b) * a.ldD + d];
+                    k_quartet<double, L1, L2, L3, L4, Lds::STAGED64>(a.rys, s_tab64, a.ew, E12, E34, Db, acc);
+                }
+            } else {
+                const KEnt<float> E12f = load_ent32_uniform(a.s12.eg32, k);
+                if (pass) {
+                    float Dbf[N2][N4];
+#pragma unroll
+                    for (int b = 0; b < N2; b++)
+#pragma unroll
+                        for (int d = 0; d < N4; d++) Dbf[b][d] = (float)dcol[(size_t)(E12f.cart + b) * a.ldD + d];
+                    k_quartet<float, L1, L2, L3, L4, Lds::STAGED32>(a.rys, s_tab32, a.ew, E12f, E34f, Dbf, acc);
+                }
+            }
             if (pass) {
-                Real Db[N2][N4];
-#pragma unroll
-                for (int b = 0; b < N2; b++)
-#pragma unroll
-                    for (int d = 0; d < N4; d++) Db[b][d] = (Real)dcol[(size_t)(E12.cart + b) * a.ldD + d];
-                k_quartet<Real, L1, L2, L3, L4, STAGED>(a.rys, s_tab, a.ew, E12, E34, Db, acc);
                 nq++;
+                if (step32) nq32++;
                 hit = true;
             }
         }
@@ -661,7 +742,7 @@ __global__ __launch_bounds__(64 * kKWaves) void k_ket(KArgs a, KGroups kg, const
 #pragma unroll
             for (int i = 0; i < N1; i++)
 #pragma unroll
-                for (int j = 0; j < N3; j++) atomicAdd(&s_acc[sl.y * N13 + i * N3 + j], (double)acc[i][j]);
+                for (int j = 0; j < N3; j++) atomicAdd(&s_acc[sl.y * N13 + i * N3 + j], acc[i][j]);
         }
         any = any || (__ballot(hit) != 0ull);
     }
@@ -683,10 +764,13 @@ __global__ __launch_bounds__(64 * kKWaves) void k_ket(KArgs a, KGroups kg, const
     }
     if (a.count) {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) nq += __shfl_xor(nq, o);
+        for (int o = 32; o > 0; o >>= 1) {
+            nq += __shfl_xor(nq, o);
+            nq32 += __shfl_xor(nq32, o);
+        }
         if (lane == 0 && nq) {
             atomicAdd(&a.sc->counts[kcount_index(L1, L2, L3, L4)], nq);
-            if constexpr (LO) atomicAdd(&a.sc->counts32[kcount_index(L1, L2, L3, L4)], nq);
+            if (nq32) atomicAdd(&a.sc->counts32[kcount_index(L1, L2, L3, L4)], nq32);
         }
     }
 }
@@ -705,12 +789,12 @@ void launch_k_ket_l1(int L2, int L3, int L4, int route, const KArgs& a, const KG
     }
 #define CASE(B, C, D)                                                                               \
     if (L2 == B && L3 == C && L4 == D) {                                                            \
-        if (route == kRouteLo)                                                                      \
-            ONE(B, C, D, kRouteLo)                                                                  \
-        else if (route == kRouteHi)                                                                 \
-            ONE(B, C, D, kRouteHi)                                                                  \
-        else                                                                                        \
+        if (route == kRouteMixed)                                                                   \
+            ONE(B, C, D, kRouteMixed)                                                               \
+        else if (route == kRoutePlain)                                                              \
             ONE(B, C, D, kRoutePlain)                                                               \
+        else                                                                                        \
+            throw std::runtime_error("lsjk: k_ket route not instantiated");                         \
         return;                                                                                     \
     }
 #define C4(B, C) CASE(B, C, 0) CASE(B, C, 1) CASE(B, C, 2)

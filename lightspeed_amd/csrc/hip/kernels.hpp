@@ -17,6 +17,7 @@ constexpr int kRoutePlain = 0;  // est >= thresp, FP64 (thredp <= thresp)
 constexpr int kRouteHi = 1;     // est >= thredp, FP64
 constexpr int kRouteLo = 2;     // thresp <= est < thredp, FP32 arithmetic, FP64 accumulation
 constexpr int kRouteGrad = 3;   // exchange gradient: est >= thresp, FP64 derivative integrals
+constexpr int kRouteMixed = 4;  // K build: est >= thredp in FP64 and thresp <= est < thredp in FP32, one launch
 
 struct JPrepArgs {
     const int* p1;
@@ -153,10 +154,12 @@ struct KGroups {
     const int* grp_rank;   // [ngroup * kKGroup] spatial rank of that row (INT_MAX: none)
     const float* chunk_b;  // [nslot / 64] largest bound of each 64-slot chunk
 };
-// A K work unit: bra row rows1[i1] against ket group g.  rank1: in the triangle mode (one list,
-// symmetric D24) only ket rows of spatial rank >= rank1 (the bra row's) take part; -1 otherwise.
+// A K work unit: bra row rows1[i1] against the 64-slot chunks [c0, c0 + nc) of ket group g
+// (crange = c0 | nc << 16, chunks counted from the group's first slot).  rank1: in the triangle
+// mode (one list, symmetric D24) only ket rows of spatial rank >= rank1 (the bra row's) take
+// part; -1 otherwise.
 struct KUnitG {
-    int i1, g, rank1, pad;
+    int i1, g, rank1, crange;
 };
 // A packed partner entry (row primitive A, partner primitive B), 128 bytes: p, Kp, P (3),
 // P - A (3), A - B (3), {the partner's Cartesian offset, its block id} (two ints), 1/(2p), 3 zero

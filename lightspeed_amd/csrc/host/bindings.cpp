@@ -195,7 +195,19 @@ PYBIND11_MODULE(_lightspeed, m)
         .def_property_readonly("thre", &PairList::thre)
         .def_property_readonly("pairlists", &PairList::pairlists)
         .def_property_readonly("npair", &PairList::npair)
-        .def("max_bound", [](const std::shared_ptr<PairList>& p) { return PairListUtil::max_bound(p); });
+        .def("max_bound", [](const std::shared_ptr<PairList>& p) { return PairListUtil::max_bound(p); })
+        // every stride-th pair of each class, order kept (a bounded sample of a list for tests
+        // against the CPU oracle at full-size workloads)
+        .def("subsample", [](const std::shared_ptr<PairList>& p, int stride) {
+            if (stride < 1) throw std::runtime_error("PairList.subsample: stride must be >= 1");
+            std::vector<PairListL> ls;
+            for (const auto& l : p->pairlists()) {
+                std::vector<Pair> v;
+                for (size_t k = 0; k < l.pairs().size(); k += (size_t)stride) v.push_back(l.pairs()[k]);
+                ls.emplace_back(l.is_symmetric(), l.L1(), l.L2(), v);
+            }
+            return std::make_shared<PairList>(p->basis1(), p->basis2(), p->is_symmetric(), p->thre(), ls);
+        }, py::arg("stride"));
 
     py::class_<ResourceList, std::shared_ptr<ResourceList>>(m, "ResourceList")
         .def_static("build", &ResourceList::build, py::arg("cpu_bytes") = 0, py::arg("gpu_bytes") = 0)
