@@ -196,6 +196,11 @@ def main():
     # closed-shell two-electron energy with D = C C^T: E2 = 2 tr(D J) - tr(D K)
     dE = float(abs(2.0 * np.sum(Dc * (Jmx - J64)) - np.sum(Dc * (Kmx - K64))))
     log("mixed vs fp64: max|dJ|=%.3e max|dK|=%.3e |dE2|=%.3e" % (dJmax, dKmax, dE))
+    # share of the mixed route's evaluated quartets that ran in FP32
+    cjm, cjm32, ckm, ckm32 = cnt["mixed"]
+    mixed_fp32 = float((cjm32.sum() + ckm32.sum()) / max(1, cjm.sum() + ckm.sum()))
+    log("mixed route: FP32 share of evaluated quartets %.3f (J %.3f, K %.3f)" % (
+        mixed_fp32, cjm32.sum() / max(1, cjm.sum()), ckm32.sum() / max(1, ckm.sum())))
 
     for w in range(args.warmup):
         for name, thredp in routes.items():
@@ -320,6 +325,7 @@ def main():
                       "fp64_ms_per_step": 1000 * timing["fp64"], "fp64_quartets_per_s": quartets / timing["fp64"],
                       "mixed_ms_per_step": 1000 * timing["mixed"],
                       "mixed_vs_fp64_speedup": timing["fp64"] / timing["mixed"],
+                      "mixed_fp32_quartet_share": mixed_fp32,
                       "mixed_meets_accuracy": bool(mixed_ok), "mixed_max_abs_dJ": dJmax, "mixed_max_abs_dK": dKmax, "mixed_abs_dE2": dE,
                       "pairlist_s": t_pl},
            "roofline": roofline}

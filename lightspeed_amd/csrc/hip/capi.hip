@@ -259,7 +259,7 @@ void build_pairlist(lsjk_pairlist* pl)
     }
     pl->hstart[2 * kMaxL + 1] = (int)order.size();
     pl->hoff[2 * kMaxL + 1] = hoff;
-    std::vector<double> hgeom(4 * (size_t)n);
+    std::vector<double> hgeom(5 * (size_t)n);
     pl->h_hb.assign(n, 0.0f);
     for (int h = 0; h < n; h++) {
         int k = order[h];
@@ -267,6 +267,7 @@ void build_pairlist(lsjk_pairlist* pl)
         double e1 = b1->h_e[i], e2 = b2->h_e[j], p = e1 + e2, ip = 1.0 / p;
         hgeom[h] = p;
         for (int d = 0; d < 3; d++) hgeom[(size_t)(d + 1) * n + h] = (e1 * b1->h_xyz[3 * i + d] + e2 * b2->h_xyz[3 * j + d]) * ip;
+        hgeom[(size_t)4 * n + h] = lsjk::kKPrefRoot * ip;
         pl->h_hb[h] = pl->h_bound[k];
     }
 

@@ -110,6 +110,15 @@ __device__ __forceinline__ Real rdiv(Real a, Real b)
         return a / b;
 }
 
+template <typename Real>
+__device__ __forceinline__ Real rrsqrt(Real x)
+{
+    if constexpr (std::is_same<Real, float>::value)
+        return __frsqrt_rn(x);
+    else
+        return rsqrt(x);
+}
+
 template <typename Real, int NR>
 __device__ __forceinline__ void roots_real(const RysTable& tab, const void* s_tab, double T, Real* x, Real* w)
 {
@@ -147,22 +156,24 @@ __device__ __forceinline__ void herm_factors(Real pref_w, Real U, Real X, Real Y
 }
 
 // Calls f(HermRoot) for every Rys root of every operator term of one bra-ket pair.
+// c12, c34: sqrt(2 pi^(5/2)) / p of each pair, so that c12 c34 (p + q)^(-1/2) = 2 pi^(5/2) / (p q sqrt(p + q)).
 template <typename Real, int L, typename F>
 __device__ __forceinline__ void for_each_root(const RysTable& tab, const void* s_tab, const Ewald& ew, double p12,
-                                              double p34, double X, double Y, double Z, F&& f)
+                                              double c12, double p34, double c34, double X, double Y, double Z, F&& f)
 {
     constexpr int NR = L / 2 + 1;
     const Real p = (Real)p12, q = (Real)p34;
     const Real psum = p + q;
-    const Real rho = rdiv(p * q, psum);
-    const Real base = rdiv(Real(2.0 * 17.493418327624862), p * q * sqrt(psum));  // 2 pi^{5/2} / (p q sqrt(p+q))
+    const Real rs = rrsqrt(psum);
+    const Real rho = p * q * (rs * rs);
+    const Real base = (Real)c12 * (Real)c34 * rs;  // 2 pi^{5/2} / (p q sqrt(p+q))
     const double r2 = X * X + Y * Y + Z * Z;
     const Real Xr = (Real)X, Yr = (Real)Y, Zr = (Real)Z;
     for (int it = 0; it < ew.n; it++) {
         const double om = ew.omega[it];
         Real d2 = Real(1);
         Real pref = (Real)ew.alpha[it] * base;
-        if (om != -1.0) {
+        if (__double_as_longlong(om) != __double_as_longlong(-1.0)) {  // erf-attenuated term (bit test: scalar)
             const Real o = (Real)om;
             d2 = rdiv(o * o, rho + o * o);
             pref *= sqrt(d2);
@@ -183,12 +194,13 @@ __device__ __forceinline__ void for_each_root(const RysTable& tab, const void* s
 // R_{tuv} = pref * sum_i w_i Hx_t Hy_u Hz_v.
 template <typename Real, int L>
 __device__ __forceinline__ void hermite_R(const RysTable& tab, const void* s_tab, const Ewald& ew, double p12,
-                                          double p34, double X, double Y, double Z, Real (&R)[nherm(L)])
+                                          double c12, double p34, double c34, double X, double Y, double Z,
+                                          Real (&R)[nherm(L)])
 {
     constexpr int HL = nherm(L);
 #pragma unroll
     for (int t = 0; t < HL; t++) R[t] = Real(0);
-    for_each_root<Real, L>(tab, s_tab, ew, p12, p34, X, Y, Z, [&](const HermRoot<Real, L>& H) __attribute__((always_inline)) {
+    for_each_root<Real, L>(tab, s_tab, ew, p12, c12, p34, c34, X, Y, Z, [&](const HermRoot<Real, L>& H) __attribute__((always_inline)) {
         static_for<HL>([&](auto T_) {
             constexpr int t = decltype(T_)::value;
             constexpr int tx = herm_tx(t), ty = herm_ty(t), tz = herm_tz(t);
@@ -203,11 +215,11 @@ __device__ __forceinline__ void hermite_R(const RysTable& tab, const void* s_tab
 //   jp[tx,ty,tz] += Hx[tx+ux] B[ty][tz]   (for each ux).
 template <typename Real, int L12, int L34, typename Acc>
 __device__ __forceinline__ void hermite_contract_factorized(const RysTable& tab, const void* s_tab, const Ewald& ew,
-                                                            double p12, double p34, double X, double Y, double Z,
-                                                            const Real (&d)[nherm(L34)], Acc& jp)
+                                                            double p12, double c12, double p34, double c34, double X,
+                                                            double Y, double Z, const Real (&d)[nherm(L34)], Acc& jp)
 {
     constexpr int L = L12 + L34;
-    for_each_root<Real, L>(tab, s_tab, ew, p12, p34, X, Y, Z, [&](const HermRoot<Real, L>& H) __attribute__((always_inline)) {
+    for_each_root<Real, L>(tab, s_tab, ew, p12, c12, p34, c34, X, Y, Z, [&](const HermRoot<Real, L>& H) __attribute__((always_inline)) {
         static_for<L34 + 1>([&](auto UX_) {
             constexpr int ux = decltype(UX_)::value;
             Real A[L34 + 1 - ux][L12 + 1];
@@ -306,13 +318,14 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
         const int bra = task.bra0 + lane;
         const bool active = lane < task.nbra;
         float B12 = 0.0f;
-        double p12 = 1.0, Px = 0.0, Py = 0.0, Pz = 0.0;
+        double p12 = 1.0, c12 = 0.0, Px = 0.0, Py = 0.0, Pz = 0.0;
         if (active) {
             B12 = a.bra_b[bra];
             p12 = a.bra_geom[bra];
             Px = a.bra_geom[a.bra_n + bra];
             Py = a.bra_geom[2 * a.bra_n + bra];
             Pz = a.bra_geom[3 * a.bra_n + bra];
+            c12 = a.bra_geom[4 * a.bra_n + bra];
         }
         const bool bra_ok = active && (B12 >= thre_bra);
         double jp[H12];
@@ -332,6 +345,7 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
             nq++;
             if (step32) nq32++;
             const double q = ld_const(a.ket_geom + k);
+            const double c34 = ld_const(a.ket_geom + 4 * a.ket_n + k);
             const double X = Px - ld_const(a.ket_geom + a.ket_n + k);
             const double Y = Py - ld_const(a.ket_geom + 2 * a.ket_n + k);
             const double Z = Pz - ld_const(a.ket_geom + 3 * a.ket_n + k);
@@ -344,10 +358,10 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
 #pragma unroll
                 for (int u = 0; u < H34; u++) dreg[u] = ld_const(dsrc + dofs + u);
                 if constexpr (L >= 5) {
-                    hermite_contract_factorized<Real, L12, L34>(a.rys, tab, a.ew, p12, q, X, Y, Z, dreg, acc);
+                    hermite_contract_factorized<Real, L12, L34>(a.rys, tab, a.ew, p12, c12, q, c34, X, Y, Z, dreg, acc);
                 } else {
                     Real R[HL];
-                    hermite_R<Real, L>(a.rys, tab, a.ew, p12, q, X, Y, Z, R);
+                    hermite_R<Real, L>(a.rys, tab, a.ew, p12, c12, q, c34, X, Y, Z, R);
                     static_for<H34>([&](auto U_) {
                         constexpr int u = decltype(U_)::value;
                         constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);

@@ -139,7 +139,7 @@ __device__ __forceinline__ void rys_roots(const RysTable& tab, double T, double*
         }
         return;
     }
-    const double sT = fmax(T, 0.0) * tab.h_inv[NRYS];
+    const double sT = T * tab.h_inv[NRYS];  // 0 <= T < Tc here
     const int b = min((int)sT, tab.nbin[NRYS] - 1);
     const double u = 2.0 * (sT - b) - 1.0;
     const double* c = tab.coef + tab.offset[NRYS] + (size_t)b * NRYS * 2 * NC;
@@ -202,7 +202,7 @@ __device__ __forceinline__ void rys_roots_lds(const RysTable& tab, const double*
         }
         return;
     }
-    const double sT = fmax(T, 0.0) * tab.h_inv[NRYS];
+    const double sT = T * tab.h_inv[NRYS];  // 0 <= T < Tc here
     const int b = min((int)sT, tab.nbin[NRYS] - 1);
     const double u = 2.0 * (sT - b) - 1.0;
     const double* c = s_coef + (size_t)b * NRYS * 2 * NC;
@@ -249,7 +249,7 @@ __device__ __forceinline__ void rys_roots_f32(const RysTable& tab, const float* 
         }
         return;
     }
-    const double sT = fmax(T, 0.0) * tab.h_inv[NRYS];
+    const double sT = T * tab.h_inv[NRYS];  // 0 <= T < Tc here
     const int b = min((int)sT, tab.nbin[NRYS] - 1);
     const float u = (float)(2.0 * (sT - b) - 1.0);
     const float* c = c32 + (size_t)b * NRYS * 2 * NC;

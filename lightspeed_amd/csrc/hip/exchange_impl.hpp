@@ -217,7 +217,7 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab
         const double om = ew.omega[it];
         Real d2 = Real(1);
         Real pref = (Real)ew.alpha[it] * base;
-        if (om != -1.0) {
+        if (__double_as_longlong(om) != __double_as_longlong(-1.0)) {  // erf-attenuated term (bit test: scalar)
             const Real o = (Real)om;
             d2 = kdiv(o * o, rho + o * o);
             pref *= sqrt(d2);
@@ -300,7 +300,7 @@ __device__ __forceinline__ void k_quartet_grad(const RysTable& tab, const Ewald&
         const double om = ew.omega[it];
         double d2 = 1.0;
         double pref = ew.alpha[it] * base;
-        if (om != -1.0) {
+        if (__double_as_longlong(om) != __double_as_longlong(-1.0)) {  // erf-attenuated term (bit test: scalar)
             d2 = om * om / (rho + om * om);
             pref *= sqrt(d2);
         }
@@ -678,13 +678,24 @@ __global__ __launch_bounds__(64 * kKWaves) void k_ket(KArgs a, KGroups kg, const
 #pragma unroll
             for (int j = 0; j < N3; j++) acc[i][j] = 0.0;
         bool hit = false;
-        // the D block maximum of the next partner is loaded one step ahead
+        // Bra partners below bstop cannot pass with any slot of the chunk: float(b12 b34max) dmax
+        // < thr for b12 < thr / (b34max dmax) (1 - 2^-22), and below thre12 the partner fails the
+        // truncation.  The partners are sorted, so the walk stops at the first one below bstop
+        // (compared as bit patterns: the bounds are non-negative floats).
+        const float bstop = fmaxf(thre12, (float)(thr / ((double)b34max * dmax)) * (1.0f - 4.0f * 1.1920929e-7f));
+        const unsigned int bstop_bits = __builtin_amdgcn_readfirstlane(__float_as_uint(bstop));
+        // the next partner's bound, block and D block maximum are loaded one step ahead
+        float b12n = ld_const(a.s12.eb + a0);
         double dnext = valid ? dsh[(size_t)ld_const(a.s12.eblk + a0) * a.ldDsh] : 0.0;
         for (int k = a0; k < a1; k++) {
-            const float b12 = ld_const(a.s12.eb + k);
-            if (!(b12 >= thre12) || (double)(b12 * b34max) * dmax < thr) break;
+            const float b12 = b12n;
+            if (__float_as_uint(b12) < bstop_bits) break;
             const double dblk = dnext;
-            if (k + 1 < a1 && valid) dnext = dsh[(size_t)ld_const(a.s12.eblk + k + 1) * a.ldDsh];
+            if (k + 1 < a1) {
+                b12n = ld_const(a.s12.eb + k + 1);
+                const int blk2n = ld_const(a.s12.eblk + k + 1);
+                if (valid) dnext = dsh[(size_t)blk2n * a.ldDsh];
+            }
             bool pass = false, low = LO;
             if (valid) {
                 const double est = (double)(b12 * b34) * dblk;

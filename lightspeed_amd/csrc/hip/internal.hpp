@@ -50,7 +50,7 @@ struct PairList {
     int hstart[2 * kMaxL + 2] = {0};  // Hermite-order start of total-L group
     size_t hoff[2 * kMaxL + 2] = {0}; // offset (in doubles) of group in the H-strided buffers
     float* d_hb = nullptr;             // static bound, Hermite order
-    double* d_hgeom = nullptr;         // p, Px, Py, Pz (SoA, 4 * npair), Hermite order
+    double* d_hgeom = nullptr;         // p, Px, Py, Pz, sqrt(2 pi^(5/2))/p (SoA, 5 * npair), Hermite order
     std::vector<float> h_hb;
     // K view
     int nrow = 0;                      // = b1->nprim

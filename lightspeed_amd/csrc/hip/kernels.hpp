@@ -47,7 +47,7 @@ struct JQuartetArgs {
     int count_cls;
     // bra (Hermite order of the bra list)
     const float* bra_b;
-    const double* bra_geom;  // SoA p, Px, Py, Pz, stride bra_n
+    const double* bra_geom;  // SoA p, Px, Py, Pz, sqrt(2 pi^(5/2))/p, stride bra_n
     int bra_n;
     int bra_hstart;
     double* jp;  // accumulators of the bra group
