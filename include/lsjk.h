@@ -84,7 +84,9 @@ int lsjk_pairlist_destroy(lsjk_pairlist* pl);
  *   O(r) = sum_i scales[i] erf(omegas[i] r)/r  (omegas[i] == -1: 1/r), nterm <= 8.
  * Precision (intbox.hpp:736-742): with thredp <= thresp every quartet with est >= thresp is FP64;
  * with thredp > thresp quartets with est >= thredp are FP64 and those with thresp <= est < thredp
- * are evaluated in FP32; accumulation is FP64 throughout. */
+ * are evaluated in FP32 -- except that quartets evaluated together with an FP64 one in the same
+ * wave step are evaluated in FP64 as well (never less precision than asked for); accumulation is
+ * FP64 throughout.  lsjk_last_counts_fp32 reports how many ran in FP32. */
 int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
                  const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev);
 

@@ -639,9 +639,10 @@ constexpr long long kKUnitsTarget = 12288;
 // in the triangle mode, groups lying wholly before the bra row in that order are dropped.  The
 // cost estimate of a unit is the number of (bra partner, ket chunk) steps that pass the bound
 // test with |D| <= 1 (an ordering heuristic only; the kernel screens with the actual D).  Units
-// are dispatched in coarse cost buckets, heaviest first; inside a bucket in row order, split into
-// 8 contiguous ranges that are interleaved, so that with the round-robin workgroup-to-XCD
-// dispatch each XCD (one L2) works through neighbouring rows of D and K.
+// are dispatched in coarse cost buckets, heaviest first; inside a bucket in (spatial row, group)
+// order, split into 8 contiguous ranges that are interleaved, so that with the round-robin
+// workgroup-to-XCD dispatch each XCD (one L2) works through neighbouring bra rows.  (Group-major
+// and 2-D tiled orders were measured as well: no less L2 fill traffic, no faster.)
 std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34c, int L1,
                                                  int L2, int L3, int L4, bool tri, float thresp, double* cost_out)
 {
