@@ -640,9 +640,9 @@ constexpr long long kKUnitsTarget = 12288;
 // cost estimate of a unit is the number of (bra partner, ket chunk) steps that pass the bound
 // test with |D| <= 1 (an ordering heuristic only; the kernel screens with the actual D).  Units
 // are dispatched in coarse cost buckets, heaviest first; inside a bucket in (spatial row, group)
-// order, split into 8 contiguous ranges that are interleaved, so that with the round-robin
-// workgroup-to-XCD dispatch each XCD (one L2) works through neighbouring bra rows.  (Group-major
-// and 2-D tiled orders were measured as well: no less L2 fill traffic, no faster.)
+// order, split into 8 contiguous ranges that are interleaved by workgroup, so that with the
+// round-robin workgroup-to-XCD dispatch each XCD (one L2) works through neighbouring bra rows.
+// (Group-major and 2-D tiled orders were measured as well: no less L2 fill traffic, no faster.)
 std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34c, int L1,
                                                  int L2, int L3, int L4, bool tri, float thresp, double* cost_out)
 {
@@ -734,15 +734,33 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
                                                  const std::pair<long long, KUnitG>& b) {
             return bucket(a.first) > bucket(b.first);
         });
+        // Workgroup w (units kKWaves w .. kKWaves w + kKWaves - 1) runs on XCD w % 8.  Each bucket is
+        // cut into 8 contiguous ranges; workgroups take kKWaves consecutive units of range w % 8, so
+        // an XCD stays on neighbouring rows.  Inside a range the units are sorted by cost, so the
+        // waves of one workgroup (which holds its slots until its last wave ends) cost about the
+        // same.  Units that do not fill a whole workgroup of their range go at the bucket's end.
+        constexpr size_t W = lsjk::kKWaves;
         std::vector<KUnitG> u;
         u.reserve(v.size());
         for (size_t s0 = 0; s0 < v.size();) {
             size_t s1 = s0;
             while (s1 < v.size() && bucket(v[s1].first) == bucket(v[s0].first)) s1++;
-            const size_t n = s1 - s0, chunk = (n + 7) / 8;
-            for (size_t k = 0; k < chunk; k++)
-                for (size_t x = 0; x < 8; x++)
-                    if (x * chunk + k < n) u.push_back(v[s0 + x * chunk + k].second);
+            const size_t n = s1 - s0, per = ((n + 7) / 8 + W - 1) / W * W;
+            for (size_t x = 0; x < 8; x++) {
+                const size_t b = std::min(n, x * per), e = std::min(n, (x + 1) * per);
+                std::stable_sort(v.begin() + s0 + b, v.begin() + s0 + e,
+                                 [](const std::pair<long long, KUnitG>& a, const std::pair<long long, KUnitG>& c) {
+                                     return a.first > c.first;
+                                 });
+            }
+            std::vector<KUnitG> rest;
+            for (size_t k = 0; k < per; k += W)
+                for (size_t x = 0; x < 8; x++) {
+                    const size_t b = x * per + k, e = std::min(std::min(n, (x + 1) * per), b + W);
+                    if (b >= e) continue;
+                    for (size_t q = b; q < e; q++) (e - b == W ? u : rest).push_back(v[s0 + q].second);
+                }
+            u.insert(u.end(), rest.begin(), rest.end());
             s0 = s1;
         }
         if (u.size() > (size_t)std::numeric_limits<int>::max() / lsjk::kKWaves)
