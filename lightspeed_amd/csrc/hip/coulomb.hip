@@ -123,7 +123,7 @@ template <typename Real, int NR>
 __device__ __forceinline__ void roots_real(const RysTable& tab, const void* s_tab, double T, Real* x, Real* w)
 {
     if constexpr (std::is_same<Real, float>::value)
-        rys_roots_f32<NR>(tab, static_cast<const float*>(s_tab), T, x, w);
+        rys_roots_f32<NR, true>(tab, static_cast<const float*>(s_tab), T, x, w);
     else
         rys_roots_any<NR>(tab, static_cast<const double*>(s_tab), T, x, w);
 }

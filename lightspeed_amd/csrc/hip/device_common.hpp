@@ -159,8 +159,11 @@ __device__ __forceinline__ void rys_roots(const RysTable& tab, double T, double*
     }
 }
 
-// Doubles of LDS that hold the table block of one nrys ([bin][root][x|w][12] doubles).
-__host__ __device__ constexpr size_t rys_lds_doubles(int nbin, int nrys) { return (size_t)nbin * nrys * 2 * 12; }
+// LDS copy of the table block of one nrys: [bin][root][x|w][12] doubles with one pad double per
+// bin, so that lanes reading the same coefficient of different bins hit different LDS banks (an
+// unpadded bin of 24 nrys doubles puts every bin on the same few banks).
+__host__ __device__ constexpr int rys_lds_stride(int nrys) { return nrys * 2 * 12 + 1; }
+__host__ __device__ constexpr size_t rys_lds_doubles(int nbin, int nrys) { return (size_t)nbin * rys_lds_stride(nrys); }
 
 // Kernels stage the table block in LDS for nrys <= 3 (at most 27.6 KB, so several workgroups
 // share a CU); for more roots they read it from global memory (rys_roots).
@@ -174,16 +177,18 @@ template <int NRYS>
 __device__ __forceinline__ void rys_roots_any(const RysTable& tab, const double* s_coef, double T, double* x,
                                               double* w);
 
-// Cooperative copy of the nrys block of the table into LDS (all threads of the workgroup; the
-// caller synchronises the workgroup afterwards).  The block is a multiple of 24 doubles and
-// starts at a multiple of 24 doubles, so 16-byte copies are aligned.
+// Cooperative copy of the nrys block of the table into LDS, bins padded to rys_lds_stride (all
+// threads of the workgroup; the caller synchronises the workgroup afterwards).
 template <int NRYS>
 __device__ __forceinline__ void rys_stage_lds(const RysTable& tab, double* s_coef)
 {
-    const int n2 = (int)(rys_lds_doubles(tab.nbin[NRYS], NRYS) / 2);
-    const double2* src = reinterpret_cast<const double2*>(tab.coef + tab.offset[NRYS]);
-    double2* dst = reinterpret_cast<double2*>(s_coef);
-    for (int i = threadIdx.x; i < n2; i += blockDim.x) dst[i] = src[i];
+    constexpr int B = NRYS * 2 * 12, S = rys_lds_stride(NRYS);
+    const int n = tab.nbin[NRYS] * B;
+    const double* src = tab.coef + tab.offset[NRYS];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int b = i / B;
+        s_coef[b * S + (i - b * B)] = src[i];
+    }
 }
 
 // rys_roots with the coefficients of this nrys read from LDS (staged by rys_stage_lds).
@@ -205,7 +210,7 @@ __device__ __forceinline__ void rys_roots_lds(const RysTable& tab, const double*
     const double sT = T * tab.h_inv[NRYS];  // 0 <= T < Tc here
     const int b = min((int)sT, tab.nbin[NRYS] - 1);
     const double u = 2.0 * (sT - b) - 1.0;
-    const double* c = s_coef + (size_t)b * NRYS * 2 * NC;
+    const double* c = s_coef + (size_t)b * rys_lds_stride(NRYS);
 #pragma unroll
     for (int i = 0; i < NRYS; i++) {
         const double* cx = c + i * 2 * NC;
@@ -232,10 +237,14 @@ __device__ __forceinline__ void rys_roots_any(const RysTable& tab, const double*
         rys_roots<NRYS>(tab, T, x, w);
 }
 
+// LDS copy of the FP32 table block of one nrys: bins of nrys 2 kRysNcoef32 floats plus one pad.
+__host__ __device__ constexpr int rys32_lds_stride(int nrys) { return nrys * 2 * kRysNcoef32 + 1; }
+
 // FP32 Rys rule for the single-precision route.  T is formed in FP64 by the caller; for T >= Tc
 // the asymptotic roots are formed in FP64 and rounded; otherwise the FP32 bin polynomial is read
-// from `c32` (the global table or an LDS copy of this nrys block).
-template <int NRYS>
+// from `c32`: the global table (LDS = false) or an LDS copy of this nrys block with padded bins
+// (LDS = true).
+template <int NRYS, bool LDS>
 __device__ __forceinline__ void rys_roots_f32(const RysTable& tab, const float* c32, double T, float* x, float* w)
 {
     constexpr int NC = kRysNcoef32;
@@ -252,7 +261,7 @@ __device__ __forceinline__ void rys_roots_f32(const RysTable& tab, const float* 
     const double sT = T * tab.h_inv[NRYS];  // 0 <= T < Tc here
     const int b = min((int)sT, tab.nbin[NRYS] - 1);
     const float u = (float)(2.0 * (sT - b) - 1.0);
-    const float* c = c32 + (size_t)b * NRYS * 2 * NC;
+    const float* c = c32 + (size_t)b * (LDS ? rys32_lds_stride(NRYS) : NRYS * 2 * NC);
 #pragma unroll
     for (int i = 0; i < NRYS; i++) {
         const float* cx = c + i * 2 * NC;
@@ -269,21 +278,22 @@ __device__ __forceinline__ void rys_roots_f32(const RysTable& tab, const float* 
     }
 }
 
-// Floats of the FP32 table block of one nrys.
-__host__ __device__ constexpr size_t rys32_lds_floats(int nbin, int nrys) { return (size_t)nbin * nrys * 2 * kRysNcoef32; }
+__host__ __device__ constexpr size_t rys32_lds_floats(int nbin, int nrys) { return (size_t)nbin * rys32_lds_stride(nrys); }
 
 // Kernels of the FP32 route stage the FP32 block in LDS for nrys <= 5 (at most 14.2 KB).
 __host__ __device__ constexpr bool rys32_staged(int nrys) { return nrys <= 5; }
 
-// Cooperative copy of the FP32 nrys block into LDS (the block is a multiple of 12 floats and
-// starts at a multiple of 12 floats: 16-byte copies stay aligned when the base is).
+// Cooperative copy of the FP32 nrys block into LDS, bins padded to rys32_lds_stride.
 template <int NRYS>
 __device__ __forceinline__ void rys32_stage_lds(const RysTable& tab, float* s_coef)
 {
-    const int n4 = (int)(rys32_lds_floats(tab.nbin[NRYS], NRYS) / 4);
-    const float4* src = reinterpret_cast<const float4*>(tab.coef32 + tab.offset32[NRYS]);
-    float4* dst = reinterpret_cast<float4*>(s_coef);
-    for (int i = threadIdx.x; i < n4; i += blockDim.x) dst[i] = src[i];
+    constexpr int B = NRYS * 2 * kRysNcoef32, S = rys32_lds_stride(NRYS);
+    const int n = tab.nbin[NRYS] * B;
+    const float* src = tab.coef32 + tab.offset32[NRYS];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int b = i / B;
+        s_coef[b * S + (i - b * B)] = src[i];
+    }
 }
 
 // Load through the constant address space: a wave-uniform address becomes a scalar (SMEM) load,

@@ -224,8 +224,8 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab
         }
         Real x[NR], w[NR];
         if constexpr (std::is_same<Real, float>::value)
-            rys_roots_f32<NR>(tab, STAGED ? static_cast<const float*>(s_tab) : tab.coef32 + tab.offset32[NR],
-                              (double)(rho * d2) * r2, x, w);
+            rys_roots_f32<NR, STAGED>(tab, STAGED ? static_cast<const float*>(s_tab) : tab.coef32 + tab.offset32[NR],
+                                      (double)(rho * d2) * r2, x, w);
         else if constexpr (STAGED)
             rys_roots_lds<NR>(tab, static_cast<const double*>(s_tab), rho * d2 * r2, x, w);
         else
