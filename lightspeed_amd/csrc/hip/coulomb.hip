@@ -332,11 +332,21 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
 #pragma unroll
         for (int u = 0; u < H12; u++) jp[u] = 0.0;
 
+        // Early exit: B34' <= float(dmax * B34) and the static ket bounds are sorted, so no ket with
+        // static bound below kstop = thre_lo / (B12max dmax) (1 - 2^-21) can pass with a bra of the
+        // tile; the test compares bit patterns (non-negative floats) on the scalar unit.
+        const float kstop = (float)((double)thre_lo / ((double)B12max * dmax)) * (1.0f - 4.0f * 1.1920929e-7f);
+        const unsigned int kstop_bits = __builtin_amdgcn_readfirstlane(__float_as_uint(kstop));
+        // the next ket's bounds are loaded one step ahead
+        float bsn = ld_const(a.ket_static_b + task.ket0), kbn = ld_const(a.ket_b + task.ket0);
         for (int k = task.ket0; k < task.ket1; k++) {
-            // Exact early exit: B34' <= float(dmax * B34) and the static ket bounds are sorted.
-            const float bk = (float)(dmax * (double)ld_const(a.ket_static_b + k));
-            if (B12max * bk < thre_lo) break;
-            const float est = B12 * ld_const(a.ket_b + k);
+            const float bs = bsn, kb = kbn;
+            if (__float_as_uint(bs) < kstop_bits) break;
+            if (k + 1 < task.ket1) {
+                bsn = ld_const(a.ket_static_b + k + 1);
+                kbn = ld_const(a.ket_b + k + 1);
+            }
+            const float est = B12 * kb;
             bool take = bra_ok && est >= thre_lo;
             if constexpr (LO) take = take && !(est >= a.thredp);
             bool step32 = LO;
