@@ -83,3 +83,35 @@ def test_whole_build_properties(product_backend, name):
     assert sJ < 1.0e-11 and sK < 1.0e-11
     assert dKn < 1.0e-10
     assert dJm <= 1.0e-7 and dKm <= 1.0e-7 and abs(E2m - E2) <= 1.0e-6
+
+
+# The batched workload's molecules: (H2O)_n, n = 1..16, 6-31G*, seeded clusters (O atoms in a cube
+# of edge 10 A (n / 10)^(1/3), at least 4 A), D from a seeded random orthonormal C with nocc = 5n.
+# Each is checked against the oracle: the whole build up to 120 functions, a bra-pair sample
+# (every 8th pair of each class, whole ket list) above.
+@pytest.mark.parametrize("n", list(range(1, 17)))
+def test_water_clusters_1_to_16(product_backend, oracle_backend, n):
+    ls = product_backend.ls
+    O = oracle_backend.O
+    from lightspeed_amd import workloads as W
+    _say("(H2O)%d 6-31G*" % n)
+    mol = W.water_cluster(n, max(4.0, 10.0 * (n / 10.0) ** (1.0 / 3.0)), 1000 + n)
+    bas = ls.Basis.from_gbs_file(mol, "6-31gs")
+    D = W.random_density(bas.nao, 5 * n, 2000 + n)
+    pl = ls.PairList.build_schwarz(bas, bas, True, THRE)
+    ob = O.OBasis(bas)
+    opl = O.OPairList(ob, ob, True, THRE)
+    stride = 1 if bas.nao <= 120 else 8
+    sub, osub = (pl, opl) if stride == 1 else (pl.subsample(stride), opl.subsample(stride))
+    ew = ([1.0], [-1.0])
+    Jo = O.coulomb(ew, osub, opl, D, THRE)
+    J = product_backend.coulomb(ew, sub, pl, D, THRE)
+    if stride == 1:
+        Ko = O.exchange(ew, opl, opl, D, True, THRE)
+        K = product_backend.exchange(ew, pl, pl, D, True, THRE)
+    else:
+        Ko = O.exchange(ew, osub, opl, D, False, THRE)
+        K = product_backend.exchange(ew, sub, pl, D, False, THRE)
+    dJ, dK = np.abs(J - Jo).max(), np.abs(K - Ko).max()
+    _say("  nao %d stride %d: max|dJ| = %.2e  max|dK| = %.2e" % (bas.nao, stride, dJ, dK))
+    assert dJ < 1.0e-10 and dK < 1.0e-10
