@@ -1288,9 +1288,17 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
         hipStream_t st = S.stream;
         const Basis *b1 = pl12->b1, *b2 = pl12->b2, *b3 = pl34->b1, *b4 = pl34->b2;
         const bool symmode = D24symm && (pl12 == pl34);
+        // mirror-class choices of the symmetric mode (see below); a scratch K while they are timed
+        unsigned tb;
+        std::memcpy(&tb, &thresp, sizeof(tb));
+        std::map<int, int>* orient =
+            symmode ? &const_cast<lsjk_pairlist*>(pl12)->korient[std::make_pair(pl34->uid, tb)] : nullptr;
+        const bool tune = symmode && orient->empty() && ew.n > 0;
         double* Dsh = nullptr;
+        double* Ks = nullptr;
         Workspace wsp;
         wsp.add(&Dsh, (size_t)b2->nblk * b4->nblk);
+        if (tune) wsp.add(&Ks, (size_t)b1->ncart * b3->ncart);
         wsp.commit();
 
         HIPCHK(hipEventRecord(S.ev[0], st));
@@ -1303,7 +1311,6 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
         LaunchTimer lt;
         if (ew.n > 0) {
             Fanout fo;
-            fo.begin();
             struct KL {
                 int L1, L2, L3, L4;
                 const KUnitG* units;
@@ -1312,71 +1319,127 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                 KArgs ka;
                 KGroups kg;
             };
+            // The launch of one class: tri = the triangle rule (row1 rank <= row3 rank, equal row
+            // momenta of one list); K = the output.  False when the class has no work.
+            auto make_kl = [&](int L1, int L2, int L3, int L4, bool tri, double* K, KL& out) {
+                const auto& rows1 = pl12->h_rows[L1][L2];
+                const auto& rows3 = pl34->h_rows[L3][L4];
+                if (rows1.empty() || rows3.empty()) return false;
+                double cost = 0.0;
+                const auto units = get_kunits_g(pl12, pl34, L1, L2, L3, L4, tri, thresp, &cost);
+                if (units.second == 0) return false;
+                const KGroups kg = kgroups_args(get_kgroups(const_cast<lsjk_pairlist*>(pl34), L3, L4));
+                KArgs ka{};
+                ka.rys = S.rys;
+                ka.ew = ew;
+                ka.thresp = thresp;
+                ka.thredp = thredp;
+                ka.count = S.counting ? 1 : 0;
+                ka.symmetric = symmode ? 1 : 0;
+                ka.tri = tri ? 1 : 0;
+                const lsjk_pairlist* sides[2] = {pl12, pl34};
+                KSide* ks[2] = {&ka.s12, &ka.s34};
+                const int Ls[2] = {L1, L3}, Lps[2] = {L2, L4};
+                for (int q = 0; q < 2; q++) {
+                    const lsjk_pairlist* pl = sides[q];
+                    KSide& sd = *ks[q];
+                    sd.row_start = pl->d_row_start;
+                    sd.maxLp = pl->kmaxL2;
+                    sd.eb = pl->d_ebound;
+                    sd.ecart = pl->d_ecart;
+                    sd.eblk = pl->d_eblk;
+                    sd.eg = pl->d_egeom;
+                    sd.eg32 = pl->d_egeom32;
+                    sd.nent = pl->nent;
+                    sd.rows = pl->d_rows[Ls[q]][Lps[q]];
+                    sd.nrows = (int)pl->h_rows[Ls[q]][Lps[q]].size();
+                    sd.x = pl->b1->d_x;
+                    sd.y = pl->b1->d_y;
+                    sd.z = pl->b1->d_z;
+                    sd.cart = pl->b1->d_cart;
+                }
+                ka.D = D24_dev;
+                ka.ldD = b4->ncart;
+                ka.Dsh = Dsh;
+                ka.ldDsh = b4->nblk;
+                ka.K = K;
+                ka.ldK = b3->ncart;
+                ka.maxb12 = pl12->maxbound;
+                ka.maxb34 = pl34->maxbound;
+                ka.sc = S.d_sc;
+                // class cost: candidate pairs x FLOPs of one quartet of the class
+                const int Lt = L1 + L2 + L3 + L4;
+                const double Pi = (double)ncart(L1) * ncart(L2) * ncart(L3) * ncart(L4);
+                const double G = (double)(L1 + L2 + 1) * (L3 + L4 + 1);
+                const double F = 40.0 + (Lt / 2 + 1) * (30.0 + 9.0 * G + 3.0 * Pi) + 2.0 * Pi;
+                out = KL{L1, L2, L3, L4, units.first, units.second, cost * F, ka, kg};
+                return true;
+            };
+            // Symmetric mode (K = K^T): a class and its mirror (L3 L4 | L1 L2) hold the same quartets
+            // with bra and ket swapped, so one of them, over all row pairs and with the transposed
+            // block, gives both.  Which one runs faster depends on the class shapes and the lists;
+            // at first use both are timed once (on the library stream, into a scratch K) and the
+            // faster one is kept for this pair of lists.  A class that is its own mirror takes the
+            // row triangle instead.
+            if (tune) {
+                {
+                    hipEvent_t e0, e1;
+                    HIPCHK(hipEventCreate(&e0));
+                    HIPCHK(hipEventCreate(&e1));
+                    const int saved_count = S.counting;
+                    S.counting = 0;
+                    auto time_class = [&](int L1, int L2, int L3, int L4) {
+                        KL k;
+                        if (!make_kl(L1, L2, L3, L4, false, Ks, k)) return 0.0f;
+                        k.ka.count = 0;
+                        HIPCHK(hipEventRecord(e0, st));
+                        launch_k_ket(L1, L2, L3, L4, mixed ? kRouteMixed : kRoutePlain, k.ka, k.kg, k.units, k.nunits, st);
+                        HIPCHK(hipEventRecord(e1, st));
+                        HIPCHK(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                        return ms;
+                    };
+                    for (int L1 = 0; L1 <= b1->maxL; L1++)
+                        for (int L2 = 0; L2 <= b2->maxL; L2++)
+                            for (int L3 = 0; L3 <= b3->maxL; L3++)
+                                for (int L4 = 0; L4 <= b4->maxL; L4++) {
+                                    const int c = kcount_index(L1, L2, L3, L4), m = kcount_index(L3, L4, L1, L2);
+                                    if (c >= m) continue;  // each pair once; self-mirrors need no choice
+                                    // twice each, alternating; the faster of each pair of runs counts
+                                    float tc = time_class(L1, L2, L3, L4), tm = time_class(L3, L4, L1, L2);
+                                    tc = std::min(tc, time_class(L1, L2, L3, L4));
+                                    tm = std::min(tm, time_class(L3, L4, L1, L2));
+                                    (*orient)[c] = (*orient)[m] = (tc <= tm) ? c : m;
+                                }
+                    S.counting = saved_count;
+                    (*orient)[-1] = -1;  // tuned (also when no pair has work)
+                    HIPCHK(hipEventDestroy(e0));
+                    HIPCHK(hipEventDestroy(e1));
+                }
+            }
             std::vector<KL> kls;
             for (int L1 = 0; L1 <= b1->maxL; L1++)
                 for (int L3 = 0; L3 <= b3->maxL; L3++)
                     for (int L2 = 0; L2 <= b2->maxL; L2++)
                         for (int L4 = 0; L4 <= b4->maxL; L4++) {
-                            if (symmode && L3 < L1) continue;
-                            const auto& rows1 = pl12->h_rows[L1][L2];
-                            const auto& rows3 = pl34->h_rows[L3][L4];
-                            if (rows1.empty() || rows3.empty()) continue;
-                            // in the symmetric mode with equal row momenta only row1 <= row3 units
-                            // are evaluated
-                            const bool tri = symmode && (L1 == L3);
-                            double cost = 0.0;
-                            const auto units = get_kunits_g(pl12, pl34, L1, L2, L3, L4, tri, thresp, &cost);
-                            if (units.second == 0) continue;
-                            const KGroups kg = kgroups_args(get_kgroups(const_cast<lsjk_pairlist*>(pl34), L3, L4));
-                            KArgs ka{};
-                            ka.rys = S.rys;
-                            ka.ew = ew;
-                            ka.thresp = thresp;
-                            ka.thredp = thredp;
-                            ka.count = S.counting ? 1 : 0;
-                            ka.symmetric = symmode ? 1 : 0;
-                            ka.tri = tri ? 1 : 0;
-                            ka.tri_pre = tri ? pl12->d_tri_pre[L1][L2][L4] : nullptr;
-                            ka.tri_start = tri ? pl12->d_tri_start[L1][L2][L4] : nullptr;
-                            const lsjk_pairlist* sides[2] = {pl12, pl34};
-                            KSide* ks[2] = {&ka.s12, &ka.s34};
-                            const int Ls[2] = {L1, L3}, Lps[2] = {L2, L4};
-                            for (int q = 0; q < 2; q++) {
-                                const lsjk_pairlist* pl = sides[q];
-                                KSide& s = *ks[q];
-                                s.row_start = pl->d_row_start;
-                                s.maxLp = pl->kmaxL2;
-                                s.eb = pl->d_ebound;
-                                s.ecart = pl->d_ecart;
-                                s.eblk = pl->d_eblk;
-                                s.eg = pl->d_egeom;
-                                s.eg32 = pl->d_egeom32;
-                                s.nent = pl->nent;
-                                s.rows = pl->d_rows[Ls[q]][Lps[q]];
-                                s.nrows = (int)pl->h_rows[Ls[q]][Lps[q]].size();
-                                s.x = pl->b1->d_x;
-                                s.y = pl->b1->d_y;
-                                s.z = pl->b1->d_z;
-                                s.cart = pl->b1->d_cart;
+                            bool tri = false;
+                            if (symmode) {
+                                const int c = kcount_index(L1, L2, L3, L4), m = kcount_index(L3, L4, L1, L2);
+                                if (c == m)
+                                    tri = true;
+                                else if (orient->count(c) && orient->at(c) != c)
+                                    continue;
                             }
-                            ka.D = D24_dev;
-                            ka.ldD = b4->ncart;
-                            ka.Dsh = Dsh;
-                            ka.ldDsh = b4->nblk;
-                            ka.K = K13_dev;
-                            ka.ldK = b3->ncart;
-                            ka.maxb12 = pl12->maxbound;
-                            ka.maxb34 = pl34->maxbound;
-                            ka.sc = S.d_sc;
-                            // class cost: candidate pairs x FLOPs of one quartet of the class
-                            const int Lt = L1 + L2 + L3 + L4;
-                            const double Pi = (double)ncart(L1) * ncart(L2) * ncart(L3) * ncart(L4);
-                            const double G = (double)(L1 + L2 + 1) * (L3 + L4 + 1);
-                            const double F = 40.0 + (Lt / 2 + 1) * (30.0 + 9.0 * G + 3.0 * Pi) + 2.0 * Pi;
-                            kls.push_back({L1, L2, L3, L4, units.first, units.second, cost * F, ka, kg});
+                            KL k;
+                            if (!make_kl(L1, L2, L3, L4, tri, K13_dev, k)) continue;
+                            // evaluated for the mirror class too: its diagonal blocks are the transposes
+                            k.ka.mirror_diag = (symmode && !tri) ? 1 : 0;
+                            kls.push_back(k);
                         }
             // heaviest classes first (estimated cost)
             std::stable_sort(kls.begin(), kls.end(), [](const KL& a, const KL& b) { return a.cost > b.cost; });
+            fo.begin();
             for (auto& k : kls) {
                 const int idx = kcount_index(k.L1, k.L2, k.L3, k.L4);
                 hipStream_t ss = fo.pick();

@@ -770,7 +770,7 @@ __global__ __launch_bounds__(64 * kKWaves) void k_ket(KArgs a, KGroups kg, const
             const int r3 = kg.grp_rows[g * kKGroup + t];
             const int o3 = a.s34.cart[r3];
             atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
-            if (a.symmetric && r1 != r3) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
+            if (a.symmetric && (r1 != r3 || a.mirror_diag)) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
         }
     }
     if (a.count) {

@@ -111,6 +111,10 @@ struct PairList {
     // triangle flag, thresp bits), and their estimated cost
     std::map<std::pair<unsigned long long, long long>, std::pair<void*, int>> kunits_g;
     std::map<std::pair<unsigned long long, long long>, double> kcost_g;
+    // symmetric-mode K: for each pair of mirror classes (L1 L2 | L3 L4) / (L3 L4 | L1 L2), the one
+    // that evaluates both (with the transposed block), chosen by timing both at first use; keyed by
+    // (ket list uid, thresp bits), class counter index -> chosen class counter index
+    std::map<std::pair<unsigned long long, unsigned>, std::map<int, int>> korient;
 };
 
 // ---------------------------------------------------------------- per-call argument blocks

@@ -192,6 +192,8 @@ struct KArgs {
     float thredp;
     int count;
     int symmetric;  // compute only row1 <= row3 and add the transposed block
+    int mirror_diag;  // symmetric mode, a class evaluated for its mirror class as well: the
+                      // transposed block is added on the diagonal (row1 == row3) too
     int tri;        // units are the (row1 <= row3) pairs given by tri_pre / tri_start
     const long long* tri_pre;
     const int* tri_start;
