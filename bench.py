@@ -252,11 +252,19 @@ def main():
         idx = lid
         # one launch per class evaluates both parts of the mixed route
         if kind == "J":
-            L12, L34 = idx // 5, idx % 5
-            full = sum(NJ[c] * cls_flops(c) for c in range(625)
-                       if sum(cls_of(c)[:2]) == L12 and sum(cls_of(c)[2:]) == L34)
-            f32 = cj32[idx] / cj[idx] if cj[idx] else 0.0
-            return full * (1.0 - f32), full * f32
+            # a symmetric J launch (bit 24) evaluates two group pairs (the second id, + 1, in bits
+            # 12..23)
+            f64 = f32s = 0.0
+            for i in (idx & 0xfff, ((idx >> 12) & 0xfff) - 1):
+                if i < 0:
+                    continue
+                L12, L34 = i // 5, i % 5
+                full = sum(NJ[c] * cls_flops(c) for c in range(625)
+                           if sum(cls_of(c)[:2]) == L12 and sum(cls_of(c)[2:]) == L34)
+                f32 = cj32[i] / cj[i] if cj[i] else 0.0
+                f64 += full * (1.0 - f32)
+                f32s += full * f32
+            return f64, f32s
         c = idx - 256
         return (ck[idx] - ck32[idx]) * cls_flops(c), ck32[idx] * cls_flops(c)
 
@@ -265,6 +273,8 @@ def main():
         # template arguments as the profiler prints them: route 0 plain FP64, 4 mixed (FP64 and FP32
         # parts in one launch)
         if kind == "J":
+            if (idx >> 24) & 1:
+                return "j_quartets_sym<%d, %d>" % ((idx & 0xfff) // 5, (idx & 0xfff) % 5)
             return "j_quartets<%d, %d, %d>" % (idx // 5, idx % 5, 4 if head == "mixed" else 0)
         return "k_ket<%d, %d, %d, %d, %d>" % (cls_of(idx - 256) + (4 if head == "mixed" else 0,))
 

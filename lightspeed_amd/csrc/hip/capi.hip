@@ -451,6 +451,8 @@ void free_pairlist(lsjk_pairlist* pl)
     dev_free(pl->d_cls_L2);
     for (auto& kv : pl->jtasks) dev_free(kv.second.first);
     pl->jtasks.clear();
+    for (auto& kv : pl->jtasks_sym) dev_free(kv.second.first);
+    pl->jtasks_sym.clear();
     for (auto& kv : pl->kunits) dev_free(kv.second.first);
     pl->kunits.clear();
     pl->kcost.clear();
@@ -810,6 +812,43 @@ JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
         HIPCHK(hipMalloc((void**)&d, bytes));
         if (!tasks.empty()) HIPCHK(hipMemcpy(d, tasks.data(), tasks.size() * sizeof(lsjk::JTask), hipMemcpyHostToDevice));
         it = pl12->jtasks.insert({pl34->uid, {(int*)d, (int)tasks.size()}}).first;
+    }
+    seg.d_tasks = it->second.first;
+    return seg;
+}
+
+// Task list of the symmetric J kernel (one list on both sides): for the group pairs js_enabled
+// admits, tiles of 64 bra pairs x ket chunks as in get_jtasks; within one group a tile's kets stop
+// at its last bra pair (the triangle y <= x).
+JSegments get_jtasks_sym(const lsjk_pairlist* plc)
+{
+    lsjk_pairlist* pl = const_cast<lsjk_pairlist*>(plc);
+    JSegments seg;
+    std::vector<lsjk::JTask> tasks;
+    for (int L12 = 0; L12 <= 4; L12++)
+        for (int L34 = 0; L34 <= 4; L34++) {
+            seg.off[L12][L34] = (int)tasks.size();
+            if (!lsjk::js_enabled(L12, L34)) continue;
+            const int b0 = pl->hstart[L12], b1 = pl->hstart[L12 + 1];
+            const int k0 = pl->hstart[L34], k1 = pl->hstart[L34 + 1];
+            if (b1 <= b0 || k1 <= k0) continue;
+            const long long ntile = (b1 - b0 + 63) / 64;
+            long long chunk = ((long long)(k1 - k0) * ntile / (L12 == L34 ? 2 : 1) + kJTasksTarget - 1) / kJTasksTarget;
+            chunk = std::min<long long>(kJChunkMax, std::max<long long>(64, (chunk + 63) / 64 * 64));
+            for (int b = b0; b < b1; b += 64) {
+                const int kend = (L12 == L34) ? std::min(k1, b + 64) : k1;
+                for (int k = k0; k < kend; k += (int)chunk)
+                    tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + (int)chunk, kend)});
+            }
+            seg.cnt[L12][L34] = (int)tasks.size() - seg.off[L12][L34];
+        }
+    auto it = pl->jtasks_sym.find(0);
+    if (it == pl->jtasks_sym.end()) {
+        lsjk::JTask* d = nullptr;
+        size_t bytes = std::max<size_t>(1, tasks.size()) * sizeof(lsjk::JTask);
+        HIPCHK(hipMalloc((void**)&d, bytes));
+        if (!tasks.empty()) HIPCHK(hipMemcpy(d, tasks.data(), tasks.size() * sizeof(lsjk::JTask), hipMemcpyHostToDevice));
+        it = pl->jtasks_sym.insert({0, {(int*)d, (int)tasks.size()}}).first;
     }
     seg.d_tasks = it->second.first;
     return seg;
@@ -1208,10 +1247,17 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
             std::stable_sort(order.begin(), order.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
                 return a.first + a.second > b.first + b.second;
             });
+            // one list on both sides (FP64 route): the symmetric kernel takes the group pairs
+            // js_enabled admits, each integral once for both J blocks
+            const bool jsym = (pl12 == pl34) && !mixed;
+            JSegments sseg;
+            if (jsym) sseg = get_jtasks_sym(pl12);
             for (auto& o : order) {
                 {
                     const int L12 = o.first, L34 = o.second;
-                    if (seg.cnt[L12][L34] == 0) continue;
+                    const bool sym = jsym && js_enabled(L12, L34);
+                    if (jsym && !sym && js_enabled(L34, L12)) continue;  // served by (L34, L12)
+                    if ((sym ? sseg.cnt : seg.cnt)[L12][L34] == 0) continue;
                     JQuartetArgs qa{};
                     qa.rys = S.rys;
                     qa.ew = ew;
@@ -1232,13 +1278,27 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
                     qa.ket_d32 = kd32 ? kd32 + pl34->hoff[L34] : nullptr;
                     qa.thredp = thredp;
                     qa.sc = S.d_sc;
-                    const JTask* tk = (const JTask*)seg.d_tasks + seg.off[L12][L34];
-                    const int idx = jcount_index(L12, L34);
+                    int idx = jcount_index(L12, L34);
                     hipStream_t ss = fo.pick();
-                    lt.begin(idx, ss);
-                    launch_j_quartets(L12, L34, mixed ? kRouteMixed : kRoutePlain, qa, tk, seg.cnt[L12][L34],
-                                      &S.d_sc->queue[idx], ss);
-                    lt.end();
+                    if (sym) {
+                        // the bra group's densities and the ket group's accumulators, for the
+                        // second use; launch id: bit 24 set, the other class's index + 1 in bits
+                        // 12..23 when it differs
+                        qa.bra_d = kd + pl12->hoff[L12];
+                        qa.jq = jp + pl12->hoff[L34];
+                        qa.count_cls2 = jcount_index(L34, L12);
+                        const JTask* tk = (const JTask*)sseg.d_tasks + sseg.off[L12][L34];
+                        const int lid = idx | (1 << 24) | (L12 != L34 ? (jcount_index(L34, L12) + 1) << 12 : 0);
+                        lt.begin(lid, ss);
+                        launch_j_quartets_sym(L12, L34, qa, tk, sseg.cnt[L12][L34], &S.d_sc->queue[idx], ss);
+                        lt.end();
+                    } else {
+                        const JTask* tk = (const JTask*)seg.d_tasks + seg.off[L12][L34];
+                        lt.begin(idx, ss);
+                        launch_j_quartets(L12, L34, mixed ? kRouteMixed : kRoutePlain, qa, tk, seg.cnt[L12][L34],
+                                          &S.d_sc->queue[idx], ss);
+                        lt.end();
+                    }
                 }
             }
             fo.end();

@@ -423,6 +423,170 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
     }
 }
 
+// Symmetric J (one list on both sides, FP64): each integral (x|y) of bra group L12 and ket group
+// L34 (L12 >= L34; y < x by Hermite position within one group) is evaluated once and used twice,
+// as the reference's two quartets:
+//   A. J_x += (x|y) d_y  when x passes the bra truncation and float(B_x B'_y) >= thresp;
+//   B. J_y += (y|x) d_x  when y passes the bra truncation and float(B_y B'_x) >= thresp
+//      (not for y = x, where it is A again),
+// B' being the truncated (D-weighted) bounds.  Lanes own bra pairs x as in j_quartets; the B
+// contributions of a ket step are summed over the wave and added to J_y by one lane.
+template <int L12, int L34>
+__global__ __launch_bounds__(256) void j_quartets_sym(JQuartetArgs a, const JTask* __restrict__ tasks, int ntask,
+                                                      unsigned int* qctr)
+{
+    using Lds = JLds<L12, L34, kRoutePlain>;
+    constexpr int H12 = nherm(L12), H34 = nherm(L34), L = L12 + L34, HL = nherm(L), NR = L / 2 + 1;
+    constexpr bool SAME = L12 == L34;
+    extern __shared__ double s_dyn[];
+    double* s_tab64 = s_dyn;
+    if constexpr (Lds::STAGED64) {
+        rys_stage_lds<NR>(a.rys, s_tab64);
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63;
+    const float thre_lo = a.thresp;
+    const float max34 = __uint_as_float(a.sc->max34_bits);
+    const float thre_bra = a.thresp / max34;
+    const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
+    unsigned long long nqa = 0, nqb = 0;
+
+    for (;;) {
+        const int t = (int)wave_take(qctr);
+        if (t >= ntask) break;
+        JTask task;
+        task.bra0 = ld_const(&tasks[t].bra0);
+        task.nbra = ld_const(&tasks[t].nbra);
+        task.ket0 = ld_const(&tasks[t].ket0);
+        task.ket1 = ld_const(&tasks[t].ket1);
+        const float B12max = ld_const(a.bra_b + task.bra0);
+        const int x = task.bra0 + lane;
+        const bool active = lane < task.nbra;
+        float Bx = 0.0f, kbx = -1.0f;
+        double p12 = 1.0, c12 = 0.0, Px = 0.0, Py = 0.0, Pz = 0.0;
+        double dx[H12];
+#pragma unroll
+        for (int u = 0; u < H12; u++) dx[u] = 0.0;
+        if (active) {
+            Bx = a.bra_b[x];
+            kbx = a.ket_b[x];
+            p12 = a.bra_geom[x];
+            Px = a.bra_geom[a.bra_n + x];
+            Py = a.bra_geom[2 * a.bra_n + x];
+            Pz = a.bra_geom[3 * a.bra_n + x];
+            c12 = a.bra_geom[4 * a.bra_n + x];
+#pragma unroll
+            for (int u = 0; u < H12; u++) dx[u] = a.bra_d[(size_t)(x - a.bra_hstart) * H12 + u];
+        }
+        const bool bra_ok = active && (Bx >= thre_bra);
+        double jp[H12];
+#pragma unroll
+        for (int u = 0; u < H12; u++) jp[u] = 0.0;
+
+        // As in j_quartets, no ket with static bound below kstop passes with any bra of the tile,
+        // in either use: B'_x <= float(dmax B_x) too (margin 2^-20 for the two roundings).
+        const float kstop = (float)((double)thre_lo / ((double)B12max * dmax)) * (1.0f - 8.0f * 1.1920929e-7f);
+        const unsigned int kstop_bits = __builtin_amdgcn_readfirstlane(__float_as_uint(kstop));
+        float bsn = ld_const(a.ket_static_b + task.ket0), kbn = ld_const(a.ket_b + task.ket0);
+        for (int k = task.ket0; k < task.ket1; k++) {
+            const float By = bsn, kby = kbn;
+            if (__float_as_uint(By) < kstop_bits) break;
+            if (k + 1 < task.ket1) {
+                bsn = ld_const(a.ket_static_b + k + 1);
+                kbn = ld_const(a.ket_b + k + 1);
+            }
+            const bool takeA = bra_ok && (!SAME || k <= x) && Bx * kby >= thre_lo;
+            const bool takeB = active && (By >= thre_bra) && (!SAME || k < x) && By * kbx >= thre_lo;
+            const unsigned long long mB = __ballot(takeB);
+            if ((__ballot(takeA) | mB) == 0ull) continue;
+            const double q = ld_const(a.ket_geom + k);
+            const double c34 = ld_const(a.ket_geom + 4 * a.ket_n + k);
+            const double X = Px - ld_const(a.ket_geom + a.ket_n + k);
+            const double Y = Py - ld_const(a.ket_geom + 2 * a.ket_n + k);
+            const double Z = Pz - ld_const(a.ket_geom + 3 * a.ket_n + k);
+            const size_t dofs = (size_t)(k - a.ket_hstart) * H34;
+            double dy[H34];
+#pragma unroll
+            for (int u = 0; u < H34; u++) dy[u] = ld_const(a.ket_d + dofs + u);
+            double jy[H34];
+#pragma unroll
+            for (int u = 0; u < H34; u++) jy[u] = 0.0;
+            if (takeA || takeB) {
+                nqa += takeA ? 1 : 0;
+                nqb += takeB ? 1 : 0;
+                double R[HL];
+                hermite_R<double, L>(a.rys, s_tab64, a.ew, p12, c12, q, c34, X, Y, Z, R);
+                const double wa = takeA ? 1.0 : 0.0, wb = takeB ? 1.0 : 0.0;
+                static_for<H34>([&](auto U_) {
+                    constexpr int u = decltype(U_)::value;
+                    constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
+                    const double du = wa * dy[u];
+                    double s = 0.0;
+                    static_for<H12>([&](auto T_) {
+                        constexpr int tt = decltype(T_)::value;
+                        constexpr int idx = herm_index(herm_tx(tt) + ux, herm_ty(tt) + uy, herm_tz(tt) + uz);
+                        jp[tt] = fma(R[idx], du, jp[tt]);
+                        // (y|x) at Q - P = -(P - Q): R_{t+u}(-X) = (-1)^{|t|+|u|} R_{t+u}(X)
+                        if constexpr (((herm_tx(tt) + herm_ty(tt) + herm_tz(tt) + ux + uy + uz) & 1) != 0)
+                            s = fma(-R[idx], dx[tt], s);
+                        else
+                            s = fma(R[idx], dx[tt], s);
+                    });
+                    jy[u] = wb * s;
+                });
+            }
+            if (mB != 0ull) {
+                // sum of the B contributions over the wave; one lane adds them to J_y
+#pragma unroll
+                for (int u = 0; u < H34; u++) {
+                    double v = jy[u];
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                    jy[u] = v;
+                }
+                if (lane == 0) {
+                    double* dst = a.jq + dofs;
+#pragma unroll
+                    for (int u = 0; u < H34; u++)
+                        if (jy[u] != 0.0) atomic_add(dst + u, jy[u]);
+                }
+            }
+        }
+        if (bra_ok) {
+            double* dst = a.jp + (size_t)(x - a.bra_hstart) * H12;
+#pragma unroll
+            for (int u = 0; u < H12; u++)
+                if (jp[u] != 0.0) atomic_add(dst + u, jp[u]);
+        }
+    }
+    if (a.count) {
+        for (int o = 32; o > 0; o >>= 1) {
+            nqa += __shfl_xor(nqa, o);
+            nqb += __shfl_xor(nqb, o);
+        }
+        if (lane == 0) {
+            if (nqa) atomicAdd(&a.sc->counts[a.count_cls], nqa);
+            if (nqb) atomicAdd(&a.sc->counts[a.count_cls2], nqb);
+        }
+    }
+}
+
+void launch_j_quartets_sym(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask,
+                           unsigned int* qctr, hipStream_t s)
+{
+    if (ntask <= 0) return;
+#define CASE(X, Y)                                                                                       \
+    if (L12 == X && L34 == Y) {                                                                          \
+        const size_t lds = JLds<X, Y, kRoutePlain>::bytes(a.rys);                                        \
+        const int grid = persistent_grid((const void*)j_quartets_sym<X, Y>, 256, lds, ((long long)ntask + 3) / 4); \
+        hipLaunchKernelGGL((j_quartets_sym<X, Y>), dim3(grid), dim3(256), lds, s, a, tasks, ntask, qctr); \
+        return;                                                                                          \
+    }
+    CASE(0, 0) CASE(1, 0) CASE(2, 0) CASE(3, 0) CASE(4, 0)
+#undef CASE
+    throw std::runtime_error("lsjk: j_quartets_sym class not instantiated");
+}
+
 // ------------------------------------------------------------------ gradient (bra centres)
 // dE/dA_x = sum_ab Dsym_ab K_ab sum_t [2 alpha E^{a+1x,b}_tx - a_x E^{a-1x,b}_tx] E_ty E_tz V_t, the
 // derivative of the primitive, d phi_a / dA_x = 2 alpha phi_{a+1x} - a_x phi_{a-1x}, expanded in

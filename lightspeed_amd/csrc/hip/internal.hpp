@@ -84,6 +84,8 @@ struct PairList {
     unsigned long long uid = 0;
     // cached J task lists keyed by the ket list's uid
     std::map<unsigned long long, std::pair<int*, int>> jtasks;
+    // the symmetric J kernel's task list (this list on both sides)
+    std::map<int, std::pair<int*, int>> jtasks_sym;
     // cached K gradient unit lists (device pointer, count) keyed by (ket list uid, class, triangle flag)
     std::map<std::pair<unsigned long long, int>, std::pair<void*, int>> kunits;
     // cost of the class's units (sum over units of partner-count products), same key

@@ -61,6 +61,11 @@ struct JQuartetArgs {
     const float* ket_d32; // their FP32 copy (FP32 route)
     float thredp;
     CallScalars* sc;
+    // symmetric J (one list on both sides): the bra group's Hermite densities and the ket group's
+    // accumulators, for the second use of each integral (the ket pair as bra)
+    const double* bra_d;
+    double* jq;
+    int count_cls2;
 };
 
 struct JScatterArgs {
@@ -252,6 +257,13 @@ void launch_count_k(const CountKArgs& a, hipStream_t s);
 void launch_j_ket_prep(int LA, int LB, const JPrepArgs& a, int k0, int nk, int hstart, size_t hoff, hipStream_t s);
 void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk, int hstart, size_t hoff,
                           hipStream_t s);
+// Group pairs (L12 >= L34) of the symmetric J kernel: with one list on both sides it evaluates each
+// bra-ket pair of the two groups once for both J blocks (the triangle within one group).  Used
+// where measured faster than the two one-way launches: s-type kets (one Hermite component to
+// reduce over the wave per ket); with p or d kets the per-ket reduction costs what it saves.
+__host__ __device__ constexpr bool js_enabled(int L12, int L34) { return L12 >= L34 && L34 == 0; }
+void launch_j_quartets_sym(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask,
+                           unsigned int* qctr, hipStream_t s);
 void launch_j_quartets(int L12, int L34, int route, const JQuartetArgs& a, const JTask* tasks, int ntask,
                        unsigned int* qctr, hipStream_t s);
 void launch_j_quartets_grad(int L12p1, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask,
