@@ -196,6 +196,7 @@ void check_ewald(int nterm, const double* scales, const double* omegas, lsjk::Ew
         ew.omega[ew.n] = omegas[i];
         ew.n++;
     }
+    ew.plain = ew.n == 1 && ew.omega[0] == -1.0;
 }
 
 // ------------------------------------------------------------------ pair-list layouts
@@ -470,10 +471,23 @@ void free_pairlist(lsjk_pairlist* pl)
 }
 
 // J task list for (bra list, ket list): per (L12, L34) group pair, tiles of 64 bra pairs x chunks of
-// ket pairs (between 64 and 1024, sized so a group pair yields about 2048 tasks), heaviest (largest
-// bounds) first.
+// ket pairs (at most 1024, sized so a group pair yields about kJTasksTarget tasks), heaviest
+// (largest bounds) first.
 constexpr int kJChunkMax = 1024;
 constexpr int kJTasksTarget = 16384;
+
+// Ket chunk of a group pair with `nket` ket pairs and `ntile` bra tiles: normally between 64 and
+// kJChunkMax kets, sized for about kJTasksTarget tasks.  A group pair that would then have fewer
+// than 256 tasks (heavy classes with few kets and few bra tiles: a handful of long-running waves)
+// takes shorter chunks, down to nket / 16 kets: each task adds its tile's bra potentials with
+// atomics, so more tasks per tile would contend on the same addresses.
+long long j_chunk(long long nket, long long ntile)
+{
+    long long c = (nket * ntile + kJTasksTarget - 1) / kJTasksTarget;
+    c = std::min<long long>(kJChunkMax, std::max<long long>(64, (c + 63) / 64 * 64));
+    const long long few = std::max<long long>({(nket * ntile + 255) / 256, (nket + 15) / 16, 1});
+    return std::min(c, few);
+}
 
 struct JSegments {
     int* d_tasks = nullptr;
@@ -798,8 +812,7 @@ JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
             int k0 = pl34->hstart[L34], k1 = pl34->hstart[L34 + 1];
             if (b1 <= b0 || k1 <= k0) continue;
             const long long ntile = (b1 - b0 + 63) / 64;
-            long long chunk = ((long long)(k1 - k0) * ntile + kJTasksTarget - 1) / kJTasksTarget;
-            chunk = std::min<long long>(kJChunkMax, std::max<long long>(64, (chunk + 63) / 64 * 64));
+            const long long chunk = j_chunk(k1 - k0, ntile);
             for (int b = b0; b < b1; b += 64)
                 for (int k = k0; k < k1; k += (int)chunk)
                     tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + (int)chunk, k1)});
@@ -833,8 +846,7 @@ JSegments get_jtasks_sym(const lsjk_pairlist* plc)
             const int k0 = pl->hstart[L34], k1 = pl->hstart[L34 + 1];
             if (b1 <= b0 || k1 <= k0) continue;
             const long long ntile = (b1 - b0 + 63) / 64;
-            long long chunk = ((long long)(k1 - k0) * ntile / (L12 == L34 ? 2 : 1) + kJTasksTarget - 1) / kJTasksTarget;
-            chunk = std::min<long long>(kJChunkMax, std::max<long long>(64, (chunk + 63) / 64 * 64));
+            const long long chunk = j_chunk((k1 - k0) / (L12 == L34 ? 2 : 1), ntile);
             for (int b = b0; b < b1; b += 64) {
                 const int kend = (L12 == L34) ? std::min(k1, b + 64) : k1;
                 for (int k = k0; k < kend; k += (int)chunk)

@@ -169,15 +169,8 @@ __device__ __forceinline__ void for_each_root(const RysTable& tab, const void* s
     const Real base = (Real)c12 * (Real)c34 * rs;  // 2 pi^{5/2} / (p q sqrt(p+q))
     const double r2 = X * X + Y * Y + Z * Z;
     const Real Xr = (Real)X, Yr = (Real)Y, Zr = (Real)Z;
-    for (int it = 0; it < ew.n; it++) {
-        const double om = ew.omega[it];
-        Real d2 = Real(1);
-        Real pref = (Real)ew.alpha[it] * base;
-        if (__double_as_longlong(om) != __double_as_longlong(-1.0)) {  // erf-attenuated term (bit test: scalar)
-            const Real o = (Real)om;
-            d2 = rdiv(o * o, rho + o * o);
-            pref *= sqrt(d2);
-        }
+    // one operator term: Rys roots at T = rho d2 r2, Hermite factors with prefactor pref
+    auto term = [&](Real pref, Real d2) __attribute__((always_inline)) {
         const double T = (double)(rho * d2) * r2;
         Real x[NR], w[NR];
         roots_real<Real, NR>(tab, s_tab, T, x, w);
@@ -187,6 +180,22 @@ __device__ __forceinline__ void for_each_root(const RysTable& tab, const void* s
             herm_factors<Real, L>(pref * w[i], Real(-2) * rho * d2 * x[i], Xr, Yr, Zr, H);
             f(H);
         }
+    };
+    if (ew.plain) {
+        // one bare 1/r term: d2 = 1, no term loop (the common case)
+        term((Real)ew.alpha[0] * base, Real(1));
+        return;
+    }
+    for (int it = 0; it < ew.n; it++) {
+        const double om = ew.omega[it];
+        Real d2 = Real(1);
+        Real pref = (Real)ew.alpha[it] * base;
+        if (__double_as_longlong(om) != __double_as_longlong(-1.0)) {  // erf-attenuated term (bit test: scalar)
+            const Real o = (Real)om;
+            d2 = rdiv(o * o, rho + o * o);
+            pref *= sqrt(d2);
+        }
+        term(pref, d2);
     }
 }
 

@@ -213,15 +213,8 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab
 #pragma unroll
             for (int c = 0; c < N3; c++) qacc[a][c] = Real(0);
     }
-    for (int it = 0; it < ew.n; it++) {
-        const double om = ew.omega[it];
-        Real d2 = Real(1);
-        Real pref = (Real)ew.alpha[it] * base;
-        if (__double_as_longlong(om) != __double_as_longlong(-1.0)) {  // erf-attenuated term (bit test: scalar)
-            const Real o = (Real)om;
-            d2 = kdiv(o * o, rho + o * o);
-            pref *= sqrt(d2);
-        }
+    // one operator term: Rys roots at T = rho d2 |PQ|^2, assembly with prefactor pref
+    auto term = [&](Real pref, Real d2) __attribute__((always_inline)) {
         Real x[NR], w[NR];
         if constexpr (std::is_same<Real, float>::value)
             rys_roots_f32<NR, STAGED>(tab, STAGED ? static_cast<const float*>(s_tab) : tab.coef32 + tab.offset32[NR],
@@ -263,6 +256,25 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab
                         acc[a][c] += s;
                 });
             });
+        }
+    };
+    // one bare 1/r term (the common case): d2 = 1 and no term loop.  Taken for the light classes
+    // only, where the loop's overhead is a visible share of a quartet; for the others the second
+    // copy of the quartet code measured slower.
+    constexpr bool kPlainBranch = L1 + L2 + L3 + L4 <= 2;
+    if (kPlainBranch && ew.plain) {
+        term((Real)ew.alpha[0] * base, Real(1));
+    } else {
+        for (int it = 0; it < ew.n; it++) {
+            const double om = ew.omega[it];
+            Real d2 = Real(1);
+            Real pref = (Real)ew.alpha[it] * base;
+            if (__double_as_longlong(om) != __double_as_longlong(-1.0)) {  // erf-attenuated term (bit test: scalar)
+                const Real o = (Real)om;
+                d2 = kdiv(o * o, rho + o * o);
+                pref *= sqrt(d2);
+            }
+            term(pref, d2);
         }
     }
     if constexpr (FP32) {
@@ -600,6 +612,26 @@ __device__ __forceinline__ KEnt<float> ent_to_f32(const KEnt<double>& e)
     return f;
 }
 
+// Waves per SIMD the compiler is asked to keep room for (register budget 512 / waves), per plain
+// route class: classes whose register count sits just above an occupancy step are held below it
+// where that costs at most a few spilled dwords and measured faster (classes that already spill,
+// or spill more, ran 15-55 % slower when held).  1 = no constraint.
+constexpr int kket_class(int L1, int L2, int L3, int L4) { return ((L1 * 3 + L2) * 3 + L3) * 3 + L4; }
+constexpr int kket_waves(int L1, int L2, int L3, int L4, int route)
+{
+    if (route != kRoutePlain) return 1;
+    switch (kket_class(L1, L2, L3, L4)) {
+    case kket_class(0, 0, 0, 0):
+        return 8;  // 62 -> 57 VGPRs, no spill
+    case kket_class(0, 0, 0, 1):
+    case kket_class(0, 0, 2, 0):
+    case kket_class(0, 0, 1, 1):
+        return 5;  // 98-108 -> 96 VGPRs, 12-32 bytes spilled
+    default:
+        return 1;
+    }
+}
+
 // K build over ket groups.  Each wave of the workgroup takes one unit (bra row r1, chunks of ket
 // group g); for each 64-slot chunk, lane l holds ket pair (r3, 4) of slot l in registers and the
 // wave walks r1's partners 2 in bound order (wave-uniform, scalar loads), stopping where even the
@@ -613,7 +645,7 @@ __device__ __forceinline__ KEnt<float> ent_to_f32(const KEnt<double>& e)
 // the block is added to K with one atomic per element, plus the transposed block in the symmetric
 // mode (K = K^T) for r3 != r1.
 template <int L1, int L2, int L3, int L4, int ROUTE>
-__global__ __launch_bounds__(64 * kKWaves) void k_ket(KArgs a, KGroups kg, const KUnitG* __restrict__ units, int nunits)
+__global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kket_waves(L1, L2, L3, L4, ROUTE)))) void k_ket(KArgs a, KGroups kg, const KUnitG* __restrict__ units, int nunits)
 {
     using Lds = KKetLds<L1, L2, L3, L4, ROUTE>;
     constexpr bool LO = ROUTE == kRouteLo;        // FP32 only

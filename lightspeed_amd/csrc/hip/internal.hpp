@@ -123,6 +123,7 @@ struct PairList {
 
 struct Ewald {
     int n = 0;
+    int plain = 0;  // one bare 1/r term (omega = -1): the kernels take a branch without the term loop
     double alpha[kMaxEwald];
     double omega[kMaxEwald];
 };
