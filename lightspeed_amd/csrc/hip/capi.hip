@@ -604,11 +604,11 @@ const std::vector<int>& get_mrank(lsjk_pairlist* pl)
 }
 
 // Ket groups of the (row L, partner Lp) class of a list (see KGroups): the class's rows in spatial
-// order, kKGroup per group, each group's partner entries merged by bound (descending, ties by
+// order, kgroup_size(L, Lp) per group, each group's partner entries merged by bound (descending, ties by
 // entry index) and padded to a multiple of 64 slots.
 const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp)
 {
-    using lsjk::kKGroup;
+    const int gsize = lsjk::kgroup_size(L, Lp);
     const int key = L * 8 + Lp;
     auto it = pl->kgroups.find(key);
     if (it != pl->kgroups.end()) return it->second;
@@ -619,18 +619,18 @@ const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp)
     G.order.resize(nr);
     for (int i = 0; i < nr; i++) G.order[i] = i;
     std::sort(G.order.begin(), G.order.end(), [&](int a, int b) { return mrank[rows[a]] < mrank[rows[b]]; });
-    G.ngroup = (nr + kKGroup - 1) / kKGroup;
+    G.ngroup = (nr + gsize - 1) / gsize;
     const int m = pl->kmaxL2 + 1;
     std::vector<int2> slot;
-    std::vector<int> grp_rows((size_t)G.ngroup * kKGroup, -1);
-    G.grp_rank.assign((size_t)G.ngroup * kKGroup, std::numeric_limits<int>::max());
+    std::vector<int> grp_rows((size_t)G.ngroup * gsize, -1);
+    G.grp_rank.assign((size_t)G.ngroup * gsize, std::numeric_limits<int>::max());
     G.grp_slot.assign(G.ngroup + 1, 0);
     for (int g = 0; g < G.ngroup; g++) {
         std::vector<int2> v;
-        for (int t = 0; t < kKGroup && g * kKGroup + t < nr; t++) {
-            const int r = rows[G.order[g * kKGroup + t]];
-            grp_rows[(size_t)g * kKGroup + t] = r;
-            G.grp_rank[(size_t)g * kKGroup + t] = mrank[r];
+        for (int t = 0; t < gsize && g * gsize + t < nr; t++) {
+            const int r = rows[G.order[g * gsize + t]];
+            grp_rows[(size_t)g * gsize + t] = r;
+            G.grp_rank[(size_t)g * gsize + t] = mrank[r];
             for (int e = pl->h_row_start[(size_t)r * m + Lp]; e < pl->h_row_start[(size_t)r * m + Lp + 1]; e++)
                 v.push_back(make_int2(e, t));
         }
@@ -662,8 +662,8 @@ constexpr long long kKUnitsTarget = 12288;
 std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34c, int L1,
                                                  int L2, int L3, int L4, bool tri, float thresp, double* cost_out)
 {
-    using lsjk::kKGroup;
     using lsjk::KUnitG;
+    const int gsize = lsjk::kgroup_size(L3, L4);
     lsjk_pairlist* pl12 = const_cast<lsjk_pairlist*>(pl12c);
     lsjk_pairlist* pl34 = const_cast<lsjk_pairlist*>(pl34c);
     unsigned int tb;
@@ -696,8 +696,8 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
             for (int g = 0; g < G34.ngroup; g++) {
                 if (tri) {
                     int rmax = -1;
-                    for (int t = 0; t < kKGroup; t++) {
-                        const int rk = G34.grp_rank[(size_t)g * kKGroup + t];
+                    for (int t = 0; t < gsize; t++) {
+                        const int rk = G34.grp_rank[(size_t)g * gsize + t];
                         if (rk != std::numeric_limits<int>::max()) rmax = std::max(rmax, rk);
                     }
                     if (rmax < rank1) continue;

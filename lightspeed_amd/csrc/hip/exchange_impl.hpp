@@ -568,7 +568,7 @@ __device__ __forceinline__ KEnt<float> load_ent32_uniform(const double* __restri
     return e;
 }
 
-// LDS of one k_ket workgroup: kKWaves group accumulators (kKGroup x N1 x N3 doubles each), then
+// LDS of one k_ket workgroup: kKWaves group accumulators (G x N1 x N3 doubles each), then
 // the staged table blocks: FP64 (routes with FP64 quartets, nrys <= 3), then FP32 (routes with
 // FP32 quartets, nrys <= 5).
 template <int L1, int L2, int L3, int L4, int ROUTE>
@@ -578,7 +578,8 @@ struct KKetLds {
     static constexpr bool HAS64 = ROUTE != kRouteLo;
     static constexpr bool STAGED64 = HAS64 && rys_staged(NR);
     static constexpr bool STAGED32 = HAS32 && rys32_staged(NR);
-    static constexpr int ACC = kKGroup * ncart(L1) * ncart(L3);  // doubles per wave
+    static constexpr int G = kgroup_size(L3, L4);              // rows per ket group
+    static constexpr int ACC = G * ncart(L1) * ncart(L3);  // doubles per wave
     static __host__ __device__ size_t bytes64(const RysTable& tab)
     {
         return STAGED64 ? rys_lds_doubles(tab.nbin[NR], NR) * sizeof(double) : 0;
@@ -691,7 +692,7 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
         const int2 sl = kg.slot[c0 + lane];
         const int e4 = max(sl.x, 0);
         const float b34 = a.s34.eb[e4];
-        const bool valid = sl.x >= 0 && kg.grp_rank[g * kKGroup + sl.y] >= rank1 && b34 >= thre34;
+        const bool valid = sl.x >= 0 && kg.grp_rank[g * Lds::G + sl.y] >= rank1 && b34 >= thre34;
         const int blk4 = a.s34.eblk[e4];
         KEnt<double> E34;
         KEnt<float> E34f;
@@ -799,7 +800,7 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
             const double v = s_acc[q];
             if (v == 0.0) continue;
             const int t = q / N13, ij = q - t * N13, i = ij / N3, j = ij - i * N3;
-            const int r3 = kg.grp_rows[g * kKGroup + t];
+            const int r3 = kg.grp_rows[g * Lds::G + t];
             const int o3 = a.s34.cart[r3];
             atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
             if (a.symmetric && (r1 != r3 || a.mirror_diag)) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);

@@ -146,17 +146,21 @@ struct KUnit {
 };
 
 // K build: ket groups.  The rows of one (row L, partner L) class of the ket list are put in
-// spatial (Morton) order and cut into groups of kKGroup consecutive rows; the partner entries of a
-// group's rows are merged, sorted by bound (descending) and padded to a multiple of 64 slots.  A
-// wave holds 64 slots (one ket pair per lane) in registers and streams the bra row's partners
-// through scalar loads.
-constexpr int kKGroup = 16;
+// spatial (Morton) order and cut into groups of kgroup_size(L, Lp) consecutive rows; the partner
+// entries of a group's rows are merged, sorted by bound (descending) and padded to a multiple of
+// 64 slots.  A wave holds 64 slots (one ket pair per lane) in registers and streams the bra row's
+// partners through scalar loads.  Rows with few partners per row (p or d partners) leave the
+// bounds of a 16-row group's chunks spread wide, so that few lanes pass the sieve together;
+// larger groups narrow each chunk's bound range: 64 rows for s rows with p or d partners, 32 for
+// p rows with d partners (measured up to 42 % faster), 16 otherwise (larger groups measured
+// slower there: longer units, and fewer ket rows of the bra row's triangle per group).
+constexpr int kgroup_size(int L, int Lp) { return (L == 0 && Lp > 0) ? 64 : ((L == 1 && Lp == 2) ? 32 : 16); }
 constexpr int kKWaves = 4;  // waves (independent work units) per workgroup
 struct KGroups {
     const int2* slot;      // [nslot] {entry index or -1 (padding), position of the row in the group}
     const int* grp_slot;   // [ngroup + 1] first slot of each group
-    const int* grp_rows;   // [ngroup * kKGroup] row primitive at each position (-1: none)
-    const int* grp_rank;   // [ngroup * kKGroup] spatial rank of that row (INT_MAX: none)
+    const int* grp_rows;   // [ngroup * group size] row primitive at each position (-1: none)
+    const int* grp_rank;   // [ngroup * group size] spatial rank of that row (INT_MAX: none)
     const float* chunk_b;  // [nslot / 64] largest bound of each 64-slot chunk
 };
 // A K work unit: bra row rows1[i1] against the 64-slot chunks [c0, c0 + nc) of ket group g
