@@ -545,15 +545,11 @@ __global__ __launch_bounds__(256) void j_quartets_sym(JQuartetArgs a, const JTas
                 });
             }
             if (mB != 0ull) {
-                // sum of the B contributions over the wave; one lane adds them to J_y
+                // sum of the B contributions over the wave (all lanes active here: the ket loop
+                // and this test are wave-uniform); lane 63 adds them to J_y
 #pragma unroll
-                for (int u = 0; u < H34; u++) {
-                    double v = jy[u];
-#pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-                    jy[u] = v;
-                }
-                if (lane == 0) {
+                for (int u = 0; u < H34; u++) jy[u] = wave_sum_last(jy[u]);
+                if (lane == 63) {
                     double* dst = a.jq + dofs;
 #pragma unroll
                     for (int u = 0; u < H34; u++)

@@ -312,6 +312,32 @@ __device__ __forceinline__ unsigned int wave_take(unsigned int* ctr)
     return __builtin_amdgcn_readfirstlane(__shfl(t, 0));
 }
 
+// v + (v moved across lanes by the DPP control CTRL in the rows of ROW_MASK; 0 elsewhere), the
+// double moved as its two 32-bit halves
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return v + __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
+}
+
+// Sum of v over the 64 lanes of a wave with every lane active, complete in lane 63: DPP moves
+// within 16-lane rows (swap neighbours, swap pairs, half-row mirror, row mirror: each lane then
+// holds its row's sum), then row broadcasts (lane 15 into rows 1 and 3, lane 31 into rows 2 and
+// 3).  No LDS traffic.
+__device__ __forceinline__ double wave_sum_last(double v)
+{
+    v = dpp_add<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+    v = dpp_add<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+    v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
+    v = dpp_add<0x140, 0xf>(v);  // row_mirror
+    v = dpp_add<0x142, 0xa>(v);  // row_bcast:15
+    v = dpp_add<0x143, 0xc>(v);  // row_bcast:31
+    return v;
+}
+
 // McMurchie-Davidson Hermite expansion coefficients for one axis:
 // E[t][i][j] for i <= LA, j <= LB, t <= i + j (Helgaker, Jorgensen & Olsen eqs. 9.5.6-9.5.7):
 //   E^{i+1,j}_t = E^{ij}_{t-1}/(2p) + X_PA E^{ij}_t + (t+1) E^{ij}_{t+1}

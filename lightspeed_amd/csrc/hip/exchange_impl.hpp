@@ -568,11 +568,9 @@ __device__ __forceinline__ KEnt<float> load_ent32_uniform(const double* __restri
     return e;
 }
 
-// LDS of one k_ket workgroup: the staged table blocks first, FP64 (routes with FP64 quartets,
-// nrys <= 3) then FP32 (routes with FP32 quartets, nrys <= 5), then kKWaves group accumulators
-// (G x N1 x N3 doubles each).  With the FP64 table at LDS address 0 every coefficient read of a
-// bin is one base register plus an immediate offset (ds_read2_b64 offsets reach 2040 bytes; a
-// table placed after 2 KB of accumulators cost one address add per read pair).
+// LDS of one k_ket workgroup: kKWaves group accumulators (G x N1 x N3 doubles each), then
+// the staged table blocks: FP64 (routes with FP64 quartets, nrys <= 3), then FP32 (routes with
+// FP32 quartets, nrys <= 5).
 template <int L1, int L2, int L3, int L4, int ROUTE>
 struct KKetLds {
     static constexpr int NR = (L1 + L2 + L3 + L4) / 2 + 1;
@@ -586,14 +584,12 @@ struct KKetLds {
     {
         return STAGED64 ? rys_lds_doubles(tab.nbin[NR], NR) * sizeof(double) : 0;
     }
-    // both table blocks, rounded up to a multiple of 8 bytes (the accumulators follow)
-    static __host__ __device__ size_t tab_bytes(const RysTable& tab)
+    static size_t bytes(const RysTable& tab)
     {
-        size_t b = bytes64(tab);
+        size_t b = (size_t)kKWaves * ACC * sizeof(double) + bytes64(tab);
         if (STAGED32) b += rys32_lds_floats(tab.nbin[NR], NR) * sizeof(float);
-        return (b + 7) / 8 * 8;
+        return b;
     }
-    static size_t bytes(const RysTable& tab) { return tab_bytes(tab) + (size_t)kKWaves * ACC * sizeof(double); }
 };
 
 // An FP64 entry rounded to the FP32 route's arithmetic (the values of its FP32 record).
@@ -628,9 +624,10 @@ constexpr int kket_waves(int L1, int L2, int L3, int L4, int route)
     switch (kket_class(L1, L2, L3, L4)) {
     case kket_class(0, 0, 0, 0):
         return 8;  // 62 -> 57 VGPRs, no spill
+    case kket_class(0, 0, 0, 1):
     case kket_class(0, 0, 2, 0):
     case kket_class(0, 0, 1, 1):
-        return 5;  // 100-108 -> 96 VGPRs, 24-48 bytes spilled
+        return 5;  // 98-108 -> 96 VGPRs, 12-32 bytes spilled
     default:
         return 1;
     }
@@ -659,9 +656,9 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
     extern __shared__ double s_dyn[];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    double* s_tab64 = s_dyn;
-    float* s_tab32 = reinterpret_cast<float*>(reinterpret_cast<char*>(s_dyn) + Lds::bytes64(a.rys));
-    double* s_acc = reinterpret_cast<double*>(reinterpret_cast<char*>(s_dyn) + Lds::tab_bytes(a.rys)) + w * Lds::ACC;
+    double* s_acc = s_dyn + w * Lds::ACC;
+    double* s_tab64 = s_dyn + kKWaves * Lds::ACC;
+    float* s_tab32 = reinterpret_cast<float*>(reinterpret_cast<char*>(s_tab64) + Lds::bytes64(a.rys));
     if constexpr (Lds::STAGED64) rys_stage_lds<NR>(a.rys, s_tab64);
     if constexpr (Lds::STAGED32) rys32_stage_lds<NR>(a.rys, s_tab32);
     for (int q = lane; q < Lds::ACC; q += 64) s_acc[q] = 0.0;
