@@ -318,8 +318,14 @@ template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ double dpp_add(double v)
 {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    int lo, hi;
+    if constexpr (ROW_MASK == 0xf) {  // every lane written from a valid source: no initial value
+        lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, ROW_MASK, 0xf, false);
+        hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    } else {
+        lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+        hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    }
     return v + __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
 }
 
