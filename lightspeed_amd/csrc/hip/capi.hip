@@ -745,6 +745,41 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
             }
             v.swap(w);
         }
+        // still few units (heavy classes: few rows, costly quartets): split each unit's walk over
+        // its bra row's partners into pieces, so that the launch has enough waves; each piece adds
+        // its part of the same K block
+        if (v.size() < (size_t)kKUnitsTarget / 4) {
+            long long walk_total = 0;
+            std::vector<int> walk(v.size());
+            for (size_t q = 0; q < v.size(); q++) {
+                const KUnitG& x = v[q].second;
+                const int r1 = rows1[x.i1];
+                const int a0 = pl12->h_row_start[(size_t)r1 * m12 + L2], a1 = pl12->h_row_start[(size_t)r1 * m12 + L2 + 1];
+                // partners that can pass with the unit's first (largest-bound) chunk
+                const double x0 = (double)thresp / (double)G34.chunk_b[G34.grp_slot[x.g] / 64 + (x.crange & 0xffff)];
+                walk[q] = (a1 - a0 >= 0xffff) ? 0 : (int)(std::partition_point(pl12->h_eb.begin() + a0, pl12->h_eb.begin() + a1,
+                                                                                [&](float b) { return (double)b >= x0; }) -
+                                                           (pl12->h_eb.begin() + a0));
+                walk_total += walk[q];
+            }
+            const long long piece = std::max<long long>(1, (walk_total + kKUnitsTarget / 4 - 1) / (kKUnitsTarget / 4));
+            std::vector<std::pair<long long, KUnitG>> w;
+            for (size_t q = 0; q < v.size(); q++) {
+                const int n = walk[q];
+                if (n <= piece) {
+                    w.push_back(v[q]);
+                    continue;
+                }
+                const int np = (int)((n + piece - 1) / piece), step = (n + np - 1) / np;
+                for (int p0 = 0; p0 < n; p0 += step) {
+                    KUnitG uu = v[q].second;
+                    // the last piece runs to the end of the row's partners (the stop test ends it)
+                    uu.prange = p0 | ((p0 + step < n ? step : 0) << 16);
+                    w.push_back({std::max<long long>(1, v[q].first / np), uu});
+                }
+            }
+            v.swap(w);
+        }
         auto bucket = [](long long c) { return c > 0 ? 64 - __builtin_clzll((unsigned long long)c) : 0; };
         std::stable_sort(v.begin(), v.end(), [&](const std::pair<long long, KUnitG>& a,
                                                  const std::pair<long long, KUnitG>& b) {

@@ -669,8 +669,13 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
     const int i1 = ld_const(&units[u].i1), g = ld_const(&units[u].g), rank1 = ld_const(&units[u].rank1);
     const int crange = ld_const(&units[u].crange);
     const int r1 = ld_const(a.s12.rows + i1);
-    const int a0 = ld_const(a.s12.row_start + r1 * (a.s12.maxLp + 1) + L2);
-    const int a1 = ld_const(a.s12.row_start + r1 * (a.s12.maxLp + 1) + L2 + 1);
+    int a0 = ld_const(a.s12.row_start + r1 * (a.s12.maxLp + 1) + L2);
+    int a1 = ld_const(a.s12.row_start + r1 * (a.s12.maxLp + 1) + L2 + 1);
+    const int prange = ld_const(&units[u].prange);
+    if (prange != 0) {  // a piece of the row's partner walk
+        a0 = min(a1, a0 + (prange & 0xffff));
+        if (prange >> 16) a1 = min(a1, a0 + (prange >> 16));
+    }
     const int s0 = ld_const(kg.grp_slot + g) + 64 * (crange & 0xffff);
     const int s1 = min(ld_const(kg.grp_slot + g + 1), s0 + 64 * (crange >> 16));
 

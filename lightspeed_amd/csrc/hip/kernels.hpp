@@ -166,9 +166,10 @@ struct KGroups {
 // A K work unit: bra row rows1[i1] against the 64-slot chunks [c0, c0 + nc) of ket group g
 // (crange = c0 | nc << 16, chunks counted from the group's first slot).  rank1: in the triangle
 // mode (one list, symmetric D24) only ket rows of spatial rank >= rank1 (the bra row's) take
-// part; -1 otherwise.
+// part; -1 otherwise.  prange = p0 | np << 16: the unit walks the row's partners from p0 on, np
+// of them (np = 0: to the end); 0 = the whole walk.
 struct KUnitG {
-    int i1, g, rank1, crange;
+    int i1, g, rank1, crange, prange;
 };
 // A packed partner entry (row primitive A, partner primitive B), 128 bytes: p, Kp, P (3),
 // P - A (3), A - B (3), {the partner's Cartesian offset, its block id} (two ints), 1/(2p), 3 zero
