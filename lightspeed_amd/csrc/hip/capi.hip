@@ -480,12 +480,14 @@ constexpr int kJTasksTarget = 16384;
 // kJChunkMax kets, sized for about kJTasksTarget tasks.  A group pair that would then have fewer
 // than 256 tasks (heavy classes with few kets and few bra tiles: a handful of long-running waves)
 // takes shorter chunks, down to nket / 16 kets: each task adds its tile's bra potentials with
-// atomics, so more tasks per tile would contend on the same addresses.
-long long j_chunk(long long nket, long long ntile)
+// atomics, so more tasks per tile would contend on the same addresses.  For total angular
+// momentum L >= 6 a quartet costs so much more than the atomics that the floor is nket / 64.
+long long j_chunk(long long nket, long long ntile, int L)
 {
     long long c = (nket * ntile + kJTasksTarget - 1) / kJTasksTarget;
     c = std::min<long long>(kJChunkMax, std::max<long long>(64, (c + 63) / 64 * 64));
-    const long long few = std::max<long long>({(nket * ntile + 255) / 256, (nket + 15) / 16, 1});
+    const long long per_tile = L >= 6 ? 64 : 16;
+    const long long few = std::max<long long>({(nket * ntile + 255) / 256, (nket + per_tile - 1) / per_tile, 1});
     return std::min(c, few);
 }
 
@@ -847,7 +849,7 @@ JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
             int k0 = pl34->hstart[L34], k1 = pl34->hstart[L34 + 1];
             if (b1 <= b0 || k1 <= k0) continue;
             const long long ntile = (b1 - b0 + 63) / 64;
-            const long long chunk = j_chunk(k1 - k0, ntile);
+            const long long chunk = j_chunk(k1 - k0, ntile, L12 + L34);
             for (int b = b0; b < b1; b += 64)
                 for (int k = k0; k < k1; k += (int)chunk)
                     tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + (int)chunk, k1)});
@@ -881,7 +883,7 @@ JSegments get_jtasks_sym(const lsjk_pairlist* plc)
             const int k0 = pl->hstart[L34], k1 = pl->hstart[L34 + 1];
             if (b1 <= b0 || k1 <= k0) continue;
             const long long ntile = (b1 - b0 + 63) / 64;
-            const long long chunk = j_chunk((k1 - k0) / (L12 == L34 ? 2 : 1), ntile);
+            const long long chunk = j_chunk((k1 - k0) / (L12 == L34 ? 2 : 1), ntile, L12 + L34);
             for (int b = b0; b < b1; b += 64) {
                 const int kend = (L12 == L34) ? std::min(k1, b + 64) : k1;
                 for (int k = k0; k < kend; k += (int)chunk)
