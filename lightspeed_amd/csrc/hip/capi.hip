@@ -9,6 +9,8 @@
 #include <mutex>
 #include <tuple>
 #include <numeric>
+#include <set>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -70,6 +72,7 @@ struct Session {
     std::vector<double> kms;          // their durations (ms)
 };
 Session S;
+
 constexpr int kSide = 4;
 
 template <typename T>
@@ -450,10 +453,10 @@ void free_pairlist(lsjk_pairlist* pl)
     dev_free(pl->d_cls_start);
     dev_free(pl->d_cls_L1);
     dev_free(pl->d_cls_L2);
-    for (auto& kv : pl->jtasks) dev_free(kv.second.first);
+    for (auto& kv : pl->jtasks) dev_free(kv.second.d_tasks);
     pl->jtasks.clear();
-    for (auto& kv : pl->jtasks_sym) dev_free(kv.second.first);
-    pl->jtasks_sym.clear();
+    dev_free(pl->jtasks_sym.d_tasks);
+    pl->jtasks_sym = lsjk::JTaskList();
     for (auto& kv : pl->kunits) dev_free(kv.second.first);
     pl->kunits.clear();
     pl->kcost.clear();
@@ -468,6 +471,40 @@ void free_pairlist(lsjk_pairlist* pl)
         dev_free(kv.second.d_chunk_b);
     }
     pl->kgroups.clear();
+}
+
+// Live pair lists.  The caches on a list keyed by another list's uid (J task lists, K units and
+// their costs, mirror-class choices) are dropped when that other list is destroyed, so a
+// long-lived list paired with many short-lived ones does not keep their device buffers.
+std::set<lsjk_pairlist*> g_lists;
+
+template <typename M, typename F>
+void erase_if_key(M& m, F&& pred, bool free_dev)
+{
+    for (auto it = m.begin(); it != m.end();) {
+        if (pred(it->first)) {
+            if constexpr (std::is_same<typename M::mapped_type, std::pair<void*, int>>::value)
+                if (free_dev) dev_free(it->second.first);
+            it = m.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+void drop_caches_for(lsjk_pairlist* pl, unsigned long long uid)
+{
+    auto it = pl->jtasks.find(uid);
+    if (it != pl->jtasks.end()) {
+        dev_free(it->second.d_tasks);
+        pl->jtasks.erase(it);
+    }
+    auto by_uid = [uid](const auto& k) { return k.first == uid; };
+    erase_if_key(pl->kunits, by_uid, true);
+    erase_if_key(pl->kcost, by_uid, false);
+    erase_if_key(pl->kunits_g, by_uid, true);
+    erase_if_key(pl->kcost_g, by_uid, false);
+    erase_if_key(pl->korient, by_uid, false);
 }
 
 // J task list for (bra list, ket list): per (L12, L34) group pair, tiles of 64 bra pairs x chunks of
@@ -490,12 +527,6 @@ long long j_chunk(long long nket, long long ntile, int L)
     const long long few = std::max<long long>({(nket * ntile + 255) / 256, (nket + per_tile - 1) / per_tile, 1});
     return std::min(c, few);
 }
-
-struct JSegments {
-    int* d_tasks = nullptr;
-    int off[5][5] = {{0}};
-    int cnt[5][5] = {{0}};
-};
 
 // K work units of one class.  A unit is (row1, block of kKTile row3s, chunk of at most kKChunk2 of
 // row1's partners); its cost estimate is (chunk partners) x (partners of the block's row3s) from
@@ -652,6 +683,8 @@ const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp)
 
 // Work units per K class below which the groups' chunk ranges are split.
 constexpr long long kKUnitsTarget = 12288;
+// thresp values whose K unit lists are cached per pair of lists (older ones are dropped)
+constexpr int kKThrespCached = 4;
 
 // K build units of one class: (bra row, ket group) pairs.  Rows are taken in spatial order and,
 // in the triangle mode, groups lying wholly before the bra row in that order are dropped.  The
@@ -674,6 +707,17 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
     const auto key = std::make_pair(pl34->uid, cls);
     auto it = pl12->kunits_g.find(key);
     if (it == pl12->kunits_g.end()) {
+        // the unit lists depend on thresp: keep those of at most kKThrespCached values per ket list
+        {
+            std::set<long long> seen;
+            for (const auto& kv : pl12->kunits_g)
+                if (kv.first.first == pl34->uid) seen.insert(kv.first.second >> 8);
+            if (!seen.count(cls >> 8) && (int)seen.size() >= kKThrespCached) {
+                auto same = [&](const std::pair<unsigned long long, long long>& k) { return k.first == pl34->uid; };
+                erase_if_key(pl12->kunits_g, same, true);
+                erase_if_key(pl12->kcost_g, same, false);
+            }
+        }
         const auto& rows1 = pl12->h_rows[L1][L2];
         const auto& G34 = get_kgroups(pl34, L3, L4);
         const auto& G12 = get_kgroups(pl12, L1, L2);
@@ -837,10 +881,23 @@ lsjk::KGroups kgroups_args(const lsjk_pairlist::KGroupData& G)
     return k;
 }
 
-JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
+lsjk::JTaskList upload_jtasks(const std::vector<lsjk::JTask>& tasks, const lsjk::JTaskList& seg)
+{
+    lsjk::JTaskList out = seg;
+    out.ntask = (int)tasks.size();
+    out.d_tasks = reinterpret_cast<int*>(dev_upload(tasks));
+    return out;
+}
+
+// The J task list of (bra list, ket list), built and uploaded at first use and cached on the bra
+// list under the ket list's uid.  `build` false: only look it up (nullptr if absent).
+const lsjk::JTaskList* get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34, bool build = true)
 {
     lsjk_pairlist* pl12 = const_cast<lsjk_pairlist*>(pl12c);
-    JSegments seg;
+    auto it = pl12->jtasks.find(pl34->uid);
+    if (it != pl12->jtasks.end()) return &it->second;
+    if (!build) return nullptr;
+    lsjk::JTaskList seg;
     std::vector<lsjk::JTask> tasks;
     for (int L12 = 0; L12 <= 4; L12++)
         for (int L34 = 0; L34 <= 4; L34++) {
@@ -855,25 +912,15 @@ JSegments get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34)
                     tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + (int)chunk, k1)});
             seg.cnt[L12][L34] = (int)tasks.size() - seg.off[L12][L34];
         }
-    auto it = pl12->jtasks.find(pl34->uid);
-    if (it == pl12->jtasks.end()) {
-        lsjk::JTask* d = nullptr;
-        size_t bytes = std::max<size_t>(1, tasks.size()) * sizeof(lsjk::JTask);
-        HIPCHK(hipMalloc((void**)&d, bytes));
-        if (!tasks.empty()) HIPCHK(hipMemcpy(d, tasks.data(), tasks.size() * sizeof(lsjk::JTask), hipMemcpyHostToDevice));
-        it = pl12->jtasks.insert({pl34->uid, {(int*)d, (int)tasks.size()}}).first;
-    }
-    seg.d_tasks = it->second.first;
-    return seg;
+    return &pl12->jtasks.emplace(pl34->uid, upload_jtasks(tasks, seg)).first->second;
 }
 
 // Task list of the symmetric J kernel (one list on both sides): for the group pairs js_enabled
 // admits, tiles of 64 bra pairs x ket chunks as in get_jtasks; within one group a tile's kets stop
-// at its last bra pair (the triangle y <= x).
-JSegments get_jtasks_sym(const lsjk_pairlist* plc)
+// at its last bra pair (the triangle y <= x).  Built with the list.
+void build_jtasks_sym(lsjk_pairlist* pl)
 {
-    lsjk_pairlist* pl = const_cast<lsjk_pairlist*>(plc);
-    JSegments seg;
+    lsjk::JTaskList seg;
     std::vector<lsjk::JTask> tasks;
     for (int L12 = 0; L12 <= 4; L12++)
         for (int L34 = 0; L34 <= 4; L34++) {
@@ -891,16 +938,7 @@ JSegments get_jtasks_sym(const lsjk_pairlist* plc)
             }
             seg.cnt[L12][L34] = (int)tasks.size() - seg.off[L12][L34];
         }
-    auto it = pl->jtasks_sym.find(0);
-    if (it == pl->jtasks_sym.end()) {
-        lsjk::JTask* d = nullptr;
-        size_t bytes = std::max<size_t>(1, tasks.size()) * sizeof(lsjk::JTask);
-        HIPCHK(hipMalloc((void**)&d, bytes));
-        if (!tasks.empty()) HIPCHK(hipMemcpy(d, tasks.data(), tasks.size() * sizeof(lsjk::JTask), hipMemcpyHostToDevice));
-        it = pl->jtasks_sym.insert({0, {(int*)d, (int)tasks.size()}}).first;
-    }
-    seg.d_tasks = it->second.first;
-    return seg;
+    pl->jtasks_sym = upload_jtasks(tasks, seg);
 }
 
 // Fork/join of the class launches over the side streams: launches are dispatched heaviest first,
@@ -925,11 +963,12 @@ struct Fanout {
 // Per-launch timing: bracket a launch with an event pair from the pool when profiling.
 struct LaunchTimer {
     int n = 0;
+    bool on = S.profiling;
     hipStream_t cur = nullptr;
     void begin(int id, hipStream_t st)
     {
         cur = st;
-        if (!S.profiling) return;
+        if (!on) return;
         if ((size_t)(2 * n + 2) > S.kev.size()) {
             size_t old = S.kev.size();
             S.kev.resize(old + 64);
@@ -940,13 +979,13 @@ struct LaunchTimer {
     }
     void end()
     {
-        if (!S.profiling) return;
+        if (!on) return;
         HIPCHK(hipEventRecord(S.kev[2 * n + 1], cur));
         n++;
     }
 };
 
-void finish_call(bool count)
+void finish_call(bool count, const lsjk::CallScalars* sc)
 {
     HIPCHK(hipStreamSynchronize(S.stream));
     S.kms.assign(S.kid.size(), 0.0);
@@ -962,7 +1001,7 @@ void finish_call(bool count)
     S.ms[1] = m1;
     if (count) {
         lsjk::CallScalars h;
-        HIPCHK(hipMemcpy(&h, S.d_sc, sizeof(h), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&h, sc, sizeof(h), hipMemcpyDeviceToHost));
         std::memcpy(S.counts, h.counts, sizeof(S.counts));
         std::memcpy(S.counts32, h.counts32, sizeof(S.counts32));
     } else {
@@ -994,6 +1033,162 @@ int lsjk::persistent_grid(const void* kernel, int block, size_t lds, long long n
     const long long g = (long long)per_cu * S.num_cu;
     return (int)std::max<long long>(1, std::min<long long>(g, nwork_groups));
 }
+
+// lsjk_coulomb: workspace, task lists, the launches on the library stream.
+static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
+                         const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev)
+{
+        using namespace lsjk;
+        const char* who = "lsjk_coulomb";
+        require_init();
+        if (!pl12 || !pl34 || !D34_dev || !J12_dev) throw LsjkError(LSJK_E_INVALID, std::string(who) + ": null argument");
+        // mixed precision (intbox.hpp:736-742) when thredp > thresp: FP64 at or above thredp,
+        // FP32 between thresp and thredp
+        const bool mixed = thredp > thresp;
+        Ewald ew;
+        check_ewald(nterm, scales, omegas, ew);
+        hipStream_t st = S.stream;
+        const Basis *b1 = pl12->b1, *b2 = pl12->b2, *b3 = pl34->b1, *b4 = pl34->b2;
+        // workspace
+        float* kb = nullptr;
+        double* kd = nullptr;
+        double* jp = nullptr;
+        Workspace wsp;
+        wsp.add(&kb, (size_t)pl34->npair);
+        wsp.add(&kd, pl34->hoff[2 * kMaxL + 1]);
+        wsp.add(&jp, pl12->hoff[2 * kMaxL + 1]);
+        float* kd32 = nullptr;
+        if (mixed) wsp.add(&kd32, pl34->hoff[2 * kMaxL + 1]);
+        CallScalars* sc = S.d_sc;
+        wsp.commit();
+        // one list on both sides (FP64 route): the symmetric kernel takes the group pairs
+        // js_enabled admits, each integral once for both J blocks
+        const bool jsym = (pl12 == pl34) && !mixed;
+        const JTaskList& seg = *get_jtasks(pl12, pl34);
+        const JTaskList& sseg = pl12->jtasks_sym;
+
+        HIPCHK(hipEventRecord(S.ev[0], st));
+        launch_fill(sc, sizeof(CallScalars), st);
+        launch_fill(jp, std::max<size_t>(1, pl12->hoff[2 * kMaxL + 1]) * sizeof(double), st);
+        const float max12 = pl12->maxbound;
+        const float thre_ket = thresp / max12;
+
+        JPrepArgs pa{};
+        pa.p1 = pl34->d_p1;
+        pa.p2 = pl34->d_p2;
+        pa.bound = pl34->d_bound;
+        pa.hpos = pl34->d_hpos;
+        pa.K = pl34->d_K;
+        pa.eA = b3->d_e; pa.xA = b3->d_x; pa.yA = b3->d_y; pa.zA = b3->d_z; pa.cartA = b3->d_cart;
+        pa.eB = b4->d_e; pa.xB = b4->d_x; pa.yB = b4->d_y; pa.zB = b4->d_z; pa.cartB = b4->d_cart;
+        pa.D = D34_dev;
+        pa.ldD = b4->ncart;
+        pa.symmetric = pl34->symmetric;
+        pa.thre_ket = thre_ket;
+        pa.kb = kb;
+        pa.kd = kd;
+        pa.kd32 = kd32;
+        pa.sc = sc;
+        for (size_t c = 0; c + 1 < pl34->cls_start.size(); c++) {
+            int L = pl34->cls_L1[c] + pl34->cls_L2[c];
+            launch_j_ket_prep(pl34->cls_L1[c], pl34->cls_L2[c], pa, pl34->cls_start[c],
+                              pl34->cls_start[c + 1] - pl34->cls_start[c], pl34->hstart[L], pl34->hoff[L], st);
+        }
+        HIPCHK(hipGetLastError());
+
+        HIPCHK(hipEventRecord(S.ev[1], st));
+        S.kid.clear();
+        LaunchTimer lt;
+        if (ew.n > 0) {
+            Fanout fo;
+            fo.begin();
+            std::vector<std::pair<int, int>> order;
+            for (int L12 = 4; L12 >= 0; L12--)
+                for (int L34 = 4; L34 >= 0; L34--) order.push_back({L12, L34});
+            std::stable_sort(order.begin(), order.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                return a.first + a.second > b.first + b.second;
+            });
+            for (auto& o : order) {
+                {
+                    const int L12 = o.first, L34 = o.second;
+                    const bool sym = jsym && js_enabled(L12, L34);
+                    if (jsym && !sym && js_enabled(L34, L12)) continue;  // served by (L34, L12)
+                    if ((sym ? sseg.cnt : seg.cnt)[L12][L34] == 0) continue;
+                    JQuartetArgs qa{};
+                    qa.rys = S.rys;
+                    qa.ew = ew;
+                    qa.thresp = thresp;
+                    qa.count = S.counting ? 1 : 0;
+                    qa.count_cls = jcount_index(L12, L34);
+                    qa.bra_b = pl12->d_hb;
+                    qa.bra_geom = pl12->d_hgeom;
+                    qa.bra_n = pl12->npair;
+                    qa.bra_hstart = pl12->hstart[L12];
+                    qa.jp = jp + pl12->hoff[L12];
+                    qa.ket_b = kb;
+                    qa.ket_static_b = pl34->d_hb;
+                    qa.ket_geom = pl34->d_hgeom;
+                    qa.ket_n = pl34->npair;
+                    qa.ket_hstart = pl34->hstart[L34];
+                    qa.ket_d = kd + pl34->hoff[L34];
+                    qa.ket_d32 = kd32 ? kd32 + pl34->hoff[L34] : nullptr;
+                    qa.thredp = thredp;
+                    qa.sc = sc;
+                    int idx = jcount_index(L12, L34);
+                    hipStream_t ss = fo.pick();
+                    if (sym) {
+                        // the bra group's densities and the ket group's accumulators, for the
+                        // second use; launch id: bit 24 set, the other class's index + 1 in bits
+                        // 12..23 when it differs
+                        qa.bra_d = kd + pl12->hoff[L12];
+                        qa.jq = jp + pl12->hoff[L34];
+                        qa.count_cls2 = jcount_index(L34, L12);
+                        const JTask* tk = (const JTask*)sseg.d_tasks + sseg.off[L12][L34];
+                        const int lid = idx | (1 << 24) | (L12 != L34 ? (jcount_index(L34, L12) + 1) << 12 : 0);
+                        lt.begin(lid, ss);
+                        launch_j_quartets_sym(L12, L34, qa, tk, sseg.cnt[L12][L34], &sc->queue[idx], ss);
+                        lt.end();
+                    } else {
+                        const JTask* tk = (const JTask*)seg.d_tasks + seg.off[L12][L34];
+                        lt.begin(idx, ss);
+                        launch_j_quartets(L12, L34, mixed ? kRouteMixed : kRoutePlain, qa, tk, seg.cnt[L12][L34],
+                                          &sc->queue[idx], ss);
+                        lt.end();
+                    }
+                }
+            }
+            fo.end();
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(S.ev[2], st));
+
+        JScatterArgs sa{};
+        sa.p1 = pl12->d_p1;
+        sa.p2 = pl12->d_p2;
+        sa.bound = pl12->d_bound;
+        sa.hpos = pl12->d_hpos;
+        sa.K = pl12->d_K;
+        sa.eA = b1->d_e; sa.xA = b1->d_x; sa.yA = b1->d_y; sa.zA = b1->d_z; sa.cartA = b1->d_cart;
+        sa.eB = b2->d_e; sa.xB = b2->d_x; sa.yB = b2->d_y; sa.zB = b2->d_z; sa.cartB = b2->d_cart;
+        sa.jp = jp;
+        sa.J = J12_dev;
+        sa.ldJ = b2->ncart;
+        sa.symmetric = pl12->symmetric;
+        sa.screen = 1;
+        sa.thresp = thresp;
+        sa.sc = sc;
+        if (ew.n > 0) {
+            for (size_t c = 0; c + 1 < pl12->cls_start.size(); c++) {
+                int L = pl12->cls_L1[c] + pl12->cls_L2[c];
+                launch_j_bra_scatter(pl12->cls_L1[c], pl12->cls_L2[c], sa, pl12->cls_start[c],
+                                     pl12->cls_start[c + 1] - pl12->cls_start[c], pl12->hstart[L], pl12->hoff[L], st);
+            }
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(S.ev[3], st));
+        finish_call(S.counting, sc);
+}
+
 
 // ====================================================================== C-ABI
 
@@ -1059,6 +1254,7 @@ int lsjk_init(int device)
         }
         HIPCHK(hipMalloc((void**)&S.d_sc, sizeof(lsjk::CallScalars)));
         for (auto& e : S.ev) HIPCHK(hipEventCreate(&e));
+
         S.device = device;
         S.init = true;
     });
@@ -1210,11 +1406,16 @@ int lsjk_pairlist_create(const lsjk_basis* b1, const lsjk_basis* b2, int symmetr
             pl->h_p2.assign(p2, p2 + npair);
             pl->h_bound.assign(bound, bound + npair);
             build_pairlist(pl);
+            // the J task lists of the list on both sides (the common use), so that a call with it
+            // on both sides allocates nothing
+            get_jtasks(pl, pl);
+            build_jtasks_sym(pl);
         } catch (...) {
             free_pairlist(pl);
             delete pl;
             throw;
         }
+        g_lists.insert(pl);
         *out = pl;
     });
 }
@@ -1223,6 +1424,8 @@ int lsjk_pairlist_destroy(lsjk_pairlist* pl)
 {
     return guarded([&] {
         if (!pl) return;
+        g_lists.erase(pl);
+        for (lsjk_pairlist* o : g_lists) drop_caches_for(o, pl->uid);
         free_pairlist(pl);
         delete pl;
     });
@@ -1231,156 +1434,7 @@ int lsjk_pairlist_destroy(lsjk_pairlist* pl)
 int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
                  const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev)
 {
-    return guarded([&] {
-        using namespace lsjk;
-        require_init();
-        if (!pl12 || !pl34 || !D34_dev || !J12_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_coulomb: null argument");
-        // mixed precision (intbox.hpp:736-742) when thredp > thresp: FP64 at or above thredp,
-        // FP32 between thresp and thredp
-        const bool mixed = thredp > thresp;
-        Ewald ew;
-        check_ewald(nterm, scales, omegas, ew);
-        hipStream_t st = S.stream;
-        const Basis *b1 = pl12->b1, *b2 = pl12->b2, *b3 = pl34->b1, *b4 = pl34->b2;
-        // workspace
-        float* kb = nullptr;
-        double* kd = nullptr;
-        double* jp = nullptr;
-        Workspace wsp;
-        wsp.add(&kb, (size_t)pl34->npair);
-        wsp.add(&kd, pl34->hoff[2 * kMaxL + 1]);
-        wsp.add(&jp, pl12->hoff[2 * kMaxL + 1]);
-        float* kd32 = nullptr;
-        if (mixed) wsp.add(&kd32, pl34->hoff[2 * kMaxL + 1]);
-        wsp.commit();
-        JSegments seg = get_jtasks(pl12, pl34);
-
-        HIPCHK(hipEventRecord(S.ev[0], st));
-        launch_fill(S.d_sc, sizeof(CallScalars), st);
-        launch_fill(jp, std::max<size_t>(1, pl12->hoff[2 * kMaxL + 1]) * sizeof(double), st);
-        const float max12 = pl12->maxbound;
-        const float thre_ket = thresp / max12;
-
-        JPrepArgs pa{};
-        pa.p1 = pl34->d_p1;
-        pa.p2 = pl34->d_p2;
-        pa.bound = pl34->d_bound;
-        pa.hpos = pl34->d_hpos;
-        pa.K = pl34->d_K;
-        pa.eA = b3->d_e; pa.xA = b3->d_x; pa.yA = b3->d_y; pa.zA = b3->d_z; pa.cartA = b3->d_cart;
-        pa.eB = b4->d_e; pa.xB = b4->d_x; pa.yB = b4->d_y; pa.zB = b4->d_z; pa.cartB = b4->d_cart;
-        pa.D = D34_dev;
-        pa.ldD = b4->ncart;
-        pa.symmetric = pl34->symmetric;
-        pa.thre_ket = thre_ket;
-        pa.kb = kb;
-        pa.kd = kd;
-        pa.kd32 = kd32;
-        pa.sc = S.d_sc;
-        for (size_t c = 0; c + 1 < pl34->cls_start.size(); c++) {
-            int L = pl34->cls_L1[c] + pl34->cls_L2[c];
-            launch_j_ket_prep(pl34->cls_L1[c], pl34->cls_L2[c], pa, pl34->cls_start[c],
-                              pl34->cls_start[c + 1] - pl34->cls_start[c], pl34->hstart[L], pl34->hoff[L], st);
-        }
-        HIPCHK(hipGetLastError());
-
-        HIPCHK(hipEventRecord(S.ev[1], st));
-        S.kid.clear();
-        LaunchTimer lt;
-        if (ew.n > 0) {
-            Fanout fo;
-            fo.begin();
-            std::vector<std::pair<int, int>> order;
-            for (int L12 = 4; L12 >= 0; L12--)
-                for (int L34 = 4; L34 >= 0; L34--) order.push_back({L12, L34});
-            std::stable_sort(order.begin(), order.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
-                return a.first + a.second > b.first + b.second;
-            });
-            // one list on both sides (FP64 route): the symmetric kernel takes the group pairs
-            // js_enabled admits, each integral once for both J blocks
-            const bool jsym = (pl12 == pl34) && !mixed;
-            JSegments sseg;
-            if (jsym) sseg = get_jtasks_sym(pl12);
-            for (auto& o : order) {
-                {
-                    const int L12 = o.first, L34 = o.second;
-                    const bool sym = jsym && js_enabled(L12, L34);
-                    if (jsym && !sym && js_enabled(L34, L12)) continue;  // served by (L34, L12)
-                    if ((sym ? sseg.cnt : seg.cnt)[L12][L34] == 0) continue;
-                    JQuartetArgs qa{};
-                    qa.rys = S.rys;
-                    qa.ew = ew;
-                    qa.thresp = thresp;
-                    qa.count = S.counting ? 1 : 0;
-                    qa.count_cls = jcount_index(L12, L34);
-                    qa.bra_b = pl12->d_hb;
-                    qa.bra_geom = pl12->d_hgeom;
-                    qa.bra_n = pl12->npair;
-                    qa.bra_hstart = pl12->hstart[L12];
-                    qa.jp = jp + pl12->hoff[L12];
-                    qa.ket_b = kb;
-                    qa.ket_static_b = pl34->d_hb;
-                    qa.ket_geom = pl34->d_hgeom;
-                    qa.ket_n = pl34->npair;
-                    qa.ket_hstart = pl34->hstart[L34];
-                    qa.ket_d = kd + pl34->hoff[L34];
-                    qa.ket_d32 = kd32 ? kd32 + pl34->hoff[L34] : nullptr;
-                    qa.thredp = thredp;
-                    qa.sc = S.d_sc;
-                    int idx = jcount_index(L12, L34);
-                    hipStream_t ss = fo.pick();
-                    if (sym) {
-                        // the bra group's densities and the ket group's accumulators, for the
-                        // second use; launch id: bit 24 set, the other class's index + 1 in bits
-                        // 12..23 when it differs
-                        qa.bra_d = kd + pl12->hoff[L12];
-                        qa.jq = jp + pl12->hoff[L34];
-                        qa.count_cls2 = jcount_index(L34, L12);
-                        const JTask* tk = (const JTask*)sseg.d_tasks + sseg.off[L12][L34];
-                        const int lid = idx | (1 << 24) | (L12 != L34 ? (jcount_index(L34, L12) + 1) << 12 : 0);
-                        lt.begin(lid, ss);
-                        launch_j_quartets_sym(L12, L34, qa, tk, sseg.cnt[L12][L34], &S.d_sc->queue[idx], ss);
-                        lt.end();
-                    } else {
-                        const JTask* tk = (const JTask*)seg.d_tasks + seg.off[L12][L34];
-                        lt.begin(idx, ss);
-                        launch_j_quartets(L12, L34, mixed ? kRouteMixed : kRoutePlain, qa, tk, seg.cnt[L12][L34],
-                                          &S.d_sc->queue[idx], ss);
-                        lt.end();
-                    }
-                }
-            }
-            fo.end();
-            HIPCHK(hipGetLastError());
-        }
-        HIPCHK(hipEventRecord(S.ev[2], st));
-
-        JScatterArgs sa{};
-        sa.p1 = pl12->d_p1;
-        sa.p2 = pl12->d_p2;
-        sa.bound = pl12->d_bound;
-        sa.hpos = pl12->d_hpos;
-        sa.K = pl12->d_K;
-        sa.eA = b1->d_e; sa.xA = b1->d_x; sa.yA = b1->d_y; sa.zA = b1->d_z; sa.cartA = b1->d_cart;
-        sa.eB = b2->d_e; sa.xB = b2->d_x; sa.yB = b2->d_y; sa.zB = b2->d_z; sa.cartB = b2->d_cart;
-        sa.jp = jp;
-        sa.J = J12_dev;
-        sa.ldJ = b2->ncart;
-        sa.symmetric = pl12->symmetric;
-        sa.screen = 1;
-        sa.thresp = thresp;
-        sa.sc = S.d_sc;
-        if (ew.n > 0) {
-            for (size_t c = 0; c + 1 < pl12->cls_start.size(); c++) {
-                int L = pl12->cls_L1[c] + pl12->cls_L2[c];
-                launch_j_bra_scatter(pl12->cls_L1[c], pl12->cls_L2[c], sa, pl12->cls_start[c],
-                                     pl12->cls_start[c + 1] - pl12->cls_start[c], pl12->hstart[L], pl12->hoff[L], st);
-            }
-            HIPCHK(hipGetLastError());
-        }
-        HIPCHK(hipEventRecord(S.ev[3], st));
-        finish_call(S.counting);
-    });
+    return guarded([&] { coulomb_call(pl12, pl34, D34_dev, nterm, scales, omegas, thresp, thredp, J12_dev); });
 }
 
 int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
@@ -1400,9 +1454,16 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
         // mirror-class choices of the symmetric mode (see below); a scratch K while they are timed
         unsigned tb;
         std::memcpy(&tb, &thresp, sizeof(tb));
-        std::map<int, int>* orient =
-            symmode ? &const_cast<lsjk_pairlist*>(pl12)->korient[std::make_pair(pl34->uid, tb)] : nullptr;
-        const bool tune = symmode && orient->empty() && ew.n > 0;
+        // keyed by route as well: the two routes' kernels differ in speed
+        const auto okey = std::make_pair(pl34->uid, ((unsigned long long)tb << 1) | (mixed ? 1ull : 0ull));
+        auto& ocache = const_cast<lsjk_pairlist*>(pl12)->korient;
+        const bool tune = symmode && ew.n > 0 && !ocache.count(okey);
+        std::map<int, int> tuned;  // filled by the tuning below, stored only once it is complete
+        const std::map<int, int>* orient = nullptr;
+        if (symmode && !tune) {
+            auto it = ocache.find(okey);
+            if (it != ocache.end()) orient = &it->second;
+        }
         double* Dsh = nullptr;
         double* Ks = nullptr;
         Workspace wsp;
@@ -1491,41 +1552,45 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
             // faster one is kept for this pair of lists.  A class that is its own mirror takes the
             // row triangle instead.
             if (tune) {
-                {
-                    hipEvent_t e0, e1;
-                    HIPCHK(hipEventCreate(&e0));
-                    HIPCHK(hipEventCreate(&e1));
-                    const int saved_count = S.counting;
-                    S.counting = 0;
-                    auto time_class = [&](int L1, int L2, int L3, int L4) {
-                        KL k;
-                        if (!make_kl(L1, L2, L3, L4, false, Ks, k)) return 0.0f;
-                        k.ka.count = 0;
-                        HIPCHK(hipEventRecord(e0, st));
-                        launch_k_ket(L1, L2, L3, L4, mixed ? kRouteMixed : kRoutePlain, k.ka, k.kg, k.units, k.nunits, st);
-                        HIPCHK(hipEventRecord(e1, st));
-                        HIPCHK(hipEventSynchronize(e1));
-                        float ms = 0.0f;
-                        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                        return ms;
-                    };
-                    for (int L1 = 0; L1 <= b1->maxL; L1++)
-                        for (int L2 = 0; L2 <= b2->maxL; L2++)
-                            for (int L3 = 0; L3 <= b3->maxL; L3++)
-                                for (int L4 = 0; L4 <= b4->maxL; L4++) {
-                                    const int c = kcount_index(L1, L2, L3, L4), m = kcount_index(L3, L4, L1, L2);
-                                    if (c >= m) continue;  // each pair once; self-mirrors need no choice
-                                    // twice each, alternating; the faster of each pair of runs counts
-                                    float tc = time_class(L1, L2, L3, L4), tm = time_class(L3, L4, L1, L2);
-                                    tc = std::min(tc, time_class(L1, L2, L3, L4));
-                                    tm = std::min(tm, time_class(L3, L4, L1, L2));
-                                    (*orient)[c] = (*orient)[m] = (tc <= tm) ? c : m;
-                                }
-                    S.counting = saved_count;
-                    (*orient)[-1] = -1;  // tuned (also when no pair has work)
-                    HIPCHK(hipEventDestroy(e0));
-                    HIPCHK(hipEventDestroy(e1));
-                }
+                // events and the counting flag are restored whatever happens in between
+                struct Guard {
+                    hipEvent_t e0 = nullptr, e1 = nullptr;
+                    bool counting = S.counting;
+                    ~Guard()
+                    {
+                        if (e0) (void)hipEventDestroy(e0);
+                        if (e1) (void)hipEventDestroy(e1);
+                        S.counting = counting;
+                    }
+                } g;
+                HIPCHK(hipEventCreate(&g.e0));
+                HIPCHK(hipEventCreate(&g.e1));
+                S.counting = false;
+                auto time_class = [&](int L1, int L2, int L3, int L4) {
+                    KL k;
+                    if (!make_kl(L1, L2, L3, L4, false, Ks, k)) return 0.0f;
+                    k.ka.count = 0;
+                    HIPCHK(hipEventRecord(g.e0, st));
+                    launch_k_ket(L1, L2, L3, L4, mixed ? kRouteMixed : kRoutePlain, k.ka, k.kg, k.units, k.nunits, st);
+                    HIPCHK(hipEventRecord(g.e1, st));
+                    HIPCHK(hipEventSynchronize(g.e1));
+                    float ms = 0.0f;
+                    HIPCHK(hipEventElapsedTime(&ms, g.e0, g.e1));
+                    return ms;
+                };
+                for (int L1 = 0; L1 <= b1->maxL; L1++)
+                    for (int L2 = 0; L2 <= b2->maxL; L2++)
+                        for (int L3 = 0; L3 <= b3->maxL; L3++)
+                            for (int L4 = 0; L4 <= b4->maxL; L4++) {
+                                const int c = kcount_index(L1, L2, L3, L4), m = kcount_index(L3, L4, L1, L2);
+                                if (c >= m) continue;  // each pair once; self-mirrors need no choice
+                                // twice each, alternating; the faster of each pair of runs counts
+                                float tc = time_class(L1, L2, L3, L4), tm = time_class(L3, L4, L1, L2);
+                                tc = std::min(tc, time_class(L1, L2, L3, L4));
+                                tm = std::min(tm, time_class(L3, L4, L1, L2));
+                                tuned[c] = tuned[m] = (tc <= tm) ? c : m;
+                            }
+                orient = &(ocache[okey] = tuned);
             }
             std::vector<KL> kls;
             for (int L1 = 0; L1 <= b1->maxL; L1++)
@@ -1535,10 +1600,15 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                             bool tri = false;
                             if (symmode) {
                                 const int c = kcount_index(L1, L2, L3, L4), m = kcount_index(L3, L4, L1, L2);
-                                if (c == m)
+                                if (c == m) {
                                     tri = true;
-                                else if (orient->count(c) && orient->at(c) != c)
-                                    continue;
+                                } else {
+                                    // the stored choice, else the class of the lower index (both
+                                    // classes of a pair never run together)
+                                    const auto it = orient ? orient->find(c) : std::map<int, int>::const_iterator();
+                                    const int pick = (orient && it != orient->end()) ? it->second : std::min(c, m);
+                                    if (pick != c) continue;
+                                }
                             }
                             KL k;
                             if (!make_kl(L1, L2, L3, L4, tri, K13_dev, k)) continue;
@@ -1561,7 +1631,7 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
         }
         HIPCHK(hipEventRecord(S.ev[2], st));
         HIPCHK(hipEventRecord(S.ev[3], st));
-        finish_call(S.counting);
+        finish_call(S.counting, S.d_sc);
     });
 }
 
@@ -1592,7 +1662,7 @@ int lsjk_coulomb_grad(const lsjk_pairlist* pl, const double* Dbra_dev, const dou
         wsp.add(&kd, pl->hoff[2 * kMaxL + 1]);
         wsp.add(&vp, hoff2[2 * kMaxL + 1]);
         wsp.commit();
-        JSegments seg = get_jtasks(pl, pl);
+        const JTaskList& seg = *get_jtasks(pl, pl);
         launch_fill(S.d_sc, sizeof(CallScalars), st);
         launch_fill(vp, std::max<size_t>(1, hoff2[2 * kMaxL + 1]) * sizeof(double), st);
         JPrepArgs pa{};

@@ -27,6 +27,14 @@ struct Basis {
 };
 
 // ---------------------------------------------------------------- pair list
+// A J task list on the device: tasks of group pair (L12, L34) at [off, off + cnt).
+struct JTaskList {
+    int* d_tasks = nullptr;
+    int ntask = 0;
+    int off[5][5] = {{0}};
+    int cnt[5][5] = {{0}};
+};
+
 // J view ("Hermite order"): pairs grouped by total angular momentum L = L1 + L2 and sorted by
 // Schwarz bound (descending) inside each group.  The per-call ket Hermite densities and bra
 // accumulators live in this order.
@@ -83,9 +91,9 @@ struct PairList {
     // process-unique id (never reused, unlike the object's address)
     unsigned long long uid = 0;
     // cached J task lists keyed by the ket list's uid
-    std::map<unsigned long long, std::pair<int*, int>> jtasks;
-    // the symmetric J kernel's task list (this list on both sides)
-    std::map<int, std::pair<int*, int>> jtasks_sym;
+    std::map<unsigned long long, JTaskList> jtasks;
+    // the symmetric J kernel's task list (this list on both sides), built with the list
+    JTaskList jtasks_sym;
     // cached K gradient unit lists (device pointer, count) keyed by (ket list uid, class, triangle flag)
     std::map<std::pair<unsigned long long, int>, std::pair<void*, int>> kunits;
     // cost of the class's units (sum over units of partner-count products), same key
@@ -115,8 +123,9 @@ struct PairList {
     std::map<std::pair<unsigned long long, long long>, double> kcost_g;
     // symmetric-mode K: for each pair of mirror classes (L1 L2 | L3 L4) / (L3 L4 | L1 L2), the one
     // that evaluates both (with the transposed block), chosen by timing both at first use; keyed by
-    // (ket list uid, thresp bits), class counter index -> chosen class counter index
-    std::map<std::pair<unsigned long long, unsigned>, std::map<int, int>> korient;
+    // (ket list uid, thresp bits << 1 | mixed route), class counter index -> chosen class counter
+    // index.  Every class pair with work has an entry once the map is stored.
+    std::map<std::pair<unsigned long long, unsigned long long>, std::map<int, int>> korient;
 };
 
 // ---------------------------------------------------------------- per-call argument blocks

@@ -74,6 +74,13 @@ def test_whole_build_properties(product_backend, name):
     Kn = np.array(ls.IntBox.exchange(res, ew, pl, pl, Dt, False, THRE, THRE).np)
     Jm = np.array(ls.IntBox.coulomb(res, ew, pl, pl, Dt, THRE, 1.0e-6).np)
     Km = np.array(ls.IntBox.exchange(res, ew, pl, pl, Dt, True, THRE, 1.0e-6).np)
+    # the symmetric J kernel (one list object on both sides) against the one-way kernels (a second,
+    # equal list as the ket list)
+    pl2 = ls.PairList.build_schwarz(bas, bas, True, THRE)
+    J2 = np.array(ls.IntBox.coulomb(res, ew, pl, pl2, Dt, THRE, THRE).np)
+    dJ2 = np.abs(J - J2).max()
+    _say("  |J(pl, pl) - J(pl, pl2)| %.1e" % dJ2)
+    assert dJ2 < 1.0e-11
     sJ, sK, dKn = np.abs(J - J.T).max(), np.abs(K - K.T).max(), np.abs(K - Kn).max()
     dJm, dKm = np.abs(Jm - J).max(), np.abs(Km - K).max()
     E2 = 2.0 * np.sum(D * J) - np.sum(D * K)

@@ -50,7 +50,7 @@ def test_rhf_h2o_sto3g_oracle(oracle_backend):
     hf = scf.RHF(mol, OracleIntegrals(oracle_backend.O, mol, "sto-3g"))
     E = hf.run()
     assert hf.converged
-    assert abs(E - E_REF) < 1.0e-9, E
+    assert abs(E - E_REF) < 1.0e-11, E  # the reference's tolerance (tests/coverage/rhf.py:609)
 
 
 def test_nuclear_repulsion_closed_form():
@@ -68,7 +68,7 @@ def test_rhf_h2o_sto3g_gpu(product_backend):
     E = hf.run()
     print("RHF H2O/STO-3G: E = %.12f after %d iterations" % (E, hf.iterations), flush=True)
     assert hf.converged
-    assert abs(E - E_REF) < 1.0e-9, E
+    assert abs(E - E_REF) < 1.0e-11, E  # the reference's tolerance (tests/coverage/rhf.py:609)
 
 
 @pytest.mark.gpu
