@@ -100,6 +100,33 @@ int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const do
                   int nterm, const double* scales, const double* omegas, float thresp, float thredp,
                   double* K13_dev);
 
+/* Batches of independent molecules.  The reference has no batched path: its callers run one
+ * IntBox::coulomb + IntBox::exchange per molecule (an SCF per molecule, reference
+ * est/rhf.py:616-643; MO integrals per pair density, src/cpp/intbox/mo_eri.cpp:10-250).  A batch
+ * runs the J and K builds of many molecules together, one launch per angular-momentum class over
+ * all of them, replayed from a captured hipGraph.
+ *
+ * lsjk_batch_create: nmol molecules, each given by a symmetric pair list over its own basis
+ * (lsjk_pairlist_create with symmetric = 1 and b1 == b2).  The data are copied; the lists and
+ * bases may be destroyed afterwards.
+ * lsjk_batch_layout: the device matrix layout of the batch.  A batch matrix holds every molecule's
+ * own ncart_m x ncart_m block: element (i, j) of molecule m at offsets[m] + i * ld + j in a buffer
+ * of nelem doubles (ld = the largest ncart_m; the blocks do not overlap).  offsets: nmol entries,
+ * or NULL.
+ * lsjk_batch_jk: J_m += J[D_m] and K_m += K[D_m] (D24symm: each D_m must be symmetric) for every
+ * molecule, with the semantics of lsjk_coulomb / lsjk_exchange on that molecule's list alone
+ * (Ewald operator, thresp, thredp and the mixed route as there); J_dev or K_dev may be NULL to
+ * skip that build.  The first call with a given (D_dev, J_dev, K_dev, thresp, thredp, operator)
+ * runs the launches directly and captures them into a hipGraph; later calls with the same
+ * arguments replay the graph (the graph keeps the device pointers: the buffers must stay
+ * allocated while the batch lives or until other arguments have been used 4 times). */
+typedef struct lsjk_batch lsjk_batch;
+int lsjk_batch_create(int nmol, const lsjk_pairlist* const* lists, lsjk_batch** out);
+int lsjk_batch_destroy(lsjk_batch* b);
+int lsjk_batch_layout(const lsjk_batch* b, int* ld, size_t* nelem, size_t* offsets);
+int lsjk_batch_jk(lsjk_batch* b, const double* D_dev, int nterm, const double* scales, const double* omegas,
+                  float thresp, float thredp, double* J_dev, double* K_dev);
+
 /* Coulomb gradient, bra-centre part (the reference's coulombGrad, intbox.hpp:954-1008 and
  * cpu/coulomb_grad.cpp:19-563, is the sum of this with the densities in both roles):
  *   G_dev (natom x 3) += sum_{ab,cd} Dbra_ab Dket_cd d/dR [bra centres a, b] (ab|O|cd),

@@ -38,6 +38,7 @@ ResourceList = _native.ResourceList
 PureTransform = _native.PureTransform
 IntBox = _native.IntBox
 DeviceArray = _native.DeviceArray
+JKBatch = _native.JKBatch
 
 hip_init = _native.hip_init
 hip_finalize = _native.hip_finalize
@@ -95,5 +96,6 @@ Basis.basisdir = _loaders.BASISDIR
 
 __all__ = [
     "Molecule", "Atom", "Tensor", "Primitive", "Basis", "Ewald", "PairList", "PairListL",
-    "ResourceList", "PureTransform", "IntBox", "DeviceArray", "hip_init", "hip_finalize", "device_count",
+    "ResourceList", "PureTransform", "IntBox", "DeviceArray", "JKBatch", "hip_init", "hip_finalize",
+    "device_count",
 ]

@@ -26,6 +26,7 @@ namespace {
 struct lsjk_basis : public lsjk::Basis {};
 struct lsjk_pairlist : public lsjk::PairList {};
 
+
 namespace {
 
 struct LsjkError : public std::runtime_error {
@@ -141,15 +142,42 @@ void require_init()
     if (!S.init) throw LsjkError(LSJK_E_NOINIT, "lsjk: lsjk_init has not been called");
 }
 
+// Elements of a Cartesian matrix over (basis a, basis b) and of the block-maximum matrix: dense
+// rows x columns, or for a batch basis the span of its molecules' diagonal blocks at stride ld.
+size_t mat_elems(const lsjk::Basis* a, const lsjk::Basis* b)
+{
+    if (a->ld) return a->ncart == 0 ? 1 : (size_t)(a->ncart - 1) * (size_t)(a->ld + 1) + 1;
+    return (size_t)a->ncart * b->ncart;
+}
+size_t blk_elems(const lsjk::Basis* a, const lsjk::Basis* b)
+{
+    if (a->ldblk) return a->nblk == 0 ? 1 : (size_t)(a->nblk - 1) * (size_t)(a->ldblk + 1) + 1;
+    return (size_t)a->nblk * b->nblk;
+}
+
 // Bump allocator over one grown workspace buffer (256-byte aligned pieces).
 struct Workspace {
     std::vector<std::pair<void**, size_t>> req;
     template <typename T>
     void add(T** p, size_t count) { req.push_back({(void**)p, std::max<size_t>(1, count) * sizeof(T)}); }
-    void commit()
+    // ext: carve the pieces from this fixed buffer of ext_bytes (nothing is allocated; a request
+    // that does not fit raises); nullptr: from the session workspace, grown as needed.
+    void commit(void* ext = nullptr, size_t ext_bytes = 0, const char* who = nullptr)
     {
         size_t total = 0;
         for (auto& r : req) total += (r.second + 255) & ~size_t(255);
+        if (ext) {
+            if (total > ext_bytes)
+                throw LsjkError(LSJK_E_UNSUPPORTED, std::string(who ? who : "lsjk") + ": the call needs " +
+                                                        std::to_string(total) + " bytes of scratch, more than the " +
+                                                        std::to_string(ext_bytes) + " bytes of the fixed scratch array");
+            size_t off = 0;
+            for (auto& r : req) {
+                *r.first = (char*)ext + off;
+                off += (r.second + 255) & ~size_t(255);
+            }
+            return;
+        }
         if (total > S.ws_bytes) {
             if (S.ws) HIPCHK(hipFree(S.ws));
             S.ws = nullptr;
@@ -223,14 +251,14 @@ void build_pairlist(lsjk_pairlist* pl)
             pl->cls_L1.push_back(L1);
             pl->cls_L2.push_back(L2);
             pl->cls_start.push_back(k);
-        } else if (pl->h_bound[k] > pl->h_bound[k - 1]) {
+        } else if (pl->h_bound[k] > pl->h_bound[k - 1] &&
+                   (pl->nmol == 0 || pl->h_mol[pl->h_p1[k]] == pl->h_mol[pl->h_p1[k - 1]])) {
             throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: bounds not sorted descending within a class");
         }
     }
     pl->cls_start.push_back(n);
     pl->maxbound = 0.0f;
-    for (size_t c = 0; c + 1 < pl->cls_start.size(); c++)
-        if (pl->cls_start[c + 1] > pl->cls_start[c]) pl->maxbound = std::max(pl->maxbound, pl->h_bound[pl->cls_start[c]]);
+    for (int k = 0; k < n; k++) pl->maxbound = std::max(pl->maxbound, pl->h_bound[k]);
 
     // pair prefactors K = c1 c2 exp(-e1 e2 / p R^2)
     std::vector<double> K(n);
@@ -254,7 +282,20 @@ void build_pairlist(lsjk_pairlist* pl)
         std::vector<int> grp;
         for (int k = 0; k < n; k++)
             if (b1->h_L[pl->h_p1[k]] + b2->h_L[pl->h_p2[k]] == L) grp.push_back(k);
-        std::stable_sort(grp.begin(), grp.end(), [&](int a, int b) { return pl->h_bound[a] > pl->h_bound[b]; });
+        if (pl->nmol > 0) {
+            // batch list: molecule-major, bound-sorted inside each molecule
+            std::stable_sort(grp.begin(), grp.end(), [&](int a, int b) {
+                const int ma = pl->h_mol[pl->h_p1[a]], mb = pl->h_mol[pl->h_p1[b]];
+                return ma != mb ? ma < mb : pl->h_bound[a] > pl->h_bound[b];
+            });
+            auto& hm = pl->h_hmol[L];
+            hm.assign(pl->nmol + 1, 0);
+            for (int k : grp) hm[pl->h_mol[pl->h_p1[k]] + 1]++;
+            hm[0] = (int)order.size();
+            for (int m = 0; m < pl->nmol; m++) hm[m + 1] += hm[m];
+        } else {
+            std::stable_sort(grp.begin(), grp.end(), [&](int a, int b) { return pl->h_bound[a] > pl->h_bound[b]; });
+        }
         for (int k : grp) {
             hpos[k] = (int)order.size();
             order.push_back(k);
@@ -628,6 +669,8 @@ const std::vector<int>& get_mrank(lsjk_pairlist* pl)
             const unsigned int q = ext > 0.0 ? (unsigned int)std::min(1023.0, (b->h_xyz[3 * i + d] - lo[d]) / ext * 1023.0) : 0u;
             m |= spread(q) << d;
         }
+        // batch list: molecule-major (the molecule in the bits above the 30-bit Morton code)
+        if (pl->nmol > 0) m |= (unsigned long long)pl->h_mol[i] << 30;
         key[i] = {m, i};
     }
     std::sort(key.begin(), key.end());
@@ -652,7 +695,28 @@ const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp)
     G.order.resize(nr);
     for (int i = 0; i < nr; i++) G.order[i] = i;
     std::sort(G.order.begin(), G.order.end(), [&](int a, int b) { return mrank[rows[a]] < mrank[rows[b]]; });
-    G.ngroup = (nr + gsize - 1) / gsize;
+    // groups: gsize consecutive rows of the spatial order; a batch list starts a new group at
+    // every molecule (a group's rows are of one molecule)
+    std::vector<int> gstart;  // first position (in G.order) of each group
+    {
+        int cur = -1, fill = 0;
+        for (int i = 0; i < nr; i++) {
+            const int mol = pl->nmol > 0 ? pl->h_mol[rows[G.order[i]]] : 0;
+            if (gstart.empty() || fill == gsize || mol != cur) {
+                gstart.push_back(i);
+                fill = 0;
+                cur = mol;
+            }
+            fill++;
+        }
+        gstart.push_back(nr);
+    }
+    G.ngroup = (int)gstart.size() - 1;
+    if (pl->nmol > 0) {
+        G.mol_grp.assign(pl->nmol + 1, 0);
+        for (int g = 0; g < G.ngroup; g++) G.mol_grp[pl->h_mol[rows[G.order[gstart[g]]]] + 1]++;
+        for (int q = 0; q < pl->nmol; q++) G.mol_grp[q + 1] += G.mol_grp[q];
+    }
     const int m = pl->kmaxL2 + 1;
     std::vector<int2> slot;
     std::vector<int> grp_rows((size_t)G.ngroup * gsize, -1);
@@ -660,8 +724,8 @@ const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp)
     G.grp_slot.assign(G.ngroup + 1, 0);
     for (int g = 0; g < G.ngroup; g++) {
         std::vector<int2> v;
-        for (int t = 0; t < gsize && g * gsize + t < nr; t++) {
-            const int r = rows[G.order[g * gsize + t]];
+        for (int t = 0; t < gsize && gstart[g] + t < gstart[g + 1]; t++) {
+            const int r = rows[G.order[gstart[g] + t]];
             grp_rows[(size_t)g * gsize + t] = r;
             G.grp_rank[(size_t)g * gsize + t] = mrank[r];
             for (int e = pl->h_row_start[(size_t)r * m + Lp]; e < pl->h_row_start[(size_t)r * m + Lp + 1]; e++)
@@ -739,7 +803,10 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
                 return (long long)(lo - a0);
             };
             const int rank1 = tri ? mrank[r1] : -1;
-            for (int g = 0; g < G34.ngroup; g++) {
+            // a batch list pairs a row only with the groups of its own molecule
+            const int g0 = pl34->nmol > 0 ? G34.mol_grp[pl12->h_mol[r1]] : 0;
+            const int g1 = pl34->nmol > 0 ? G34.mol_grp[pl12->h_mol[r1] + 1] : G34.ngroup;
+            for (int g = g0; g < g1; g++) {
                 if (tri) {
                     int rmax = -1;
                     for (int t = 0; t < gsize; t++) {
@@ -889,6 +956,31 @@ lsjk::JTaskList upload_jtasks(const std::vector<lsjk::JTask>& tasks, const lsjk:
     return out;
 }
 
+// J tasks of one group pair of a batch list (the list on both sides): per molecule, tiles of 64 of
+// its bra pairs x chunks of its own ket pairs, never another molecule's.  The ket chunk is sized for
+// the whole group pair as for one list (about kJTasksTarget tasks over all molecules, 64..1024 kets)
+// and clipped to each molecule's kets.  sym: the symmetric kernel's triangle (within one group a
+// tile's kets stop at its last bra pair).
+void append_batch_jtasks(const lsjk_pairlist* pl, int L12, int L34, bool sym, std::vector<lsjk::JTask>& tasks)
+{
+    const auto& hb = pl->h_hmol[L12];
+    const auto& hk = pl->h_hmol[L34];
+    double work = 0.0;
+    for (int m = 0; m < pl->nmol; m++)
+        work += (double)((hb[m + 1] - hb[m] + 63) / 64) * (double)(hk[m + 1] - hk[m]);
+    long long chunk = (long long)std::ceil(work / kJTasksTarget);
+    chunk = std::min<long long>(kJChunkMax, std::max<long long>(64, (chunk + 63) / 64 * 64));
+    for (int m = 0; m < pl->nmol; m++) {
+        const int b0 = hb[m], b1 = hb[m + 1], k0 = hk[m], k1 = hk[m + 1];
+        if (b1 <= b0 || k1 <= k0) continue;
+        for (int b = b0; b < b1; b += 64) {
+            const int kend = (sym && L12 == L34) ? std::min(k1, b + 64) : k1;
+            for (int k = k0; k < kend; k += (int)chunk)
+                tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + (int)chunk, kend)});
+        }
+    }
+}
+
 // The J task list of (bra list, ket list), built and uploaded at first use and cached on the bra
 // list under the ket list's uid.  `build` false: only look it up (nullptr if absent).
 const lsjk::JTaskList* get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34, bool build = true)
@@ -905,11 +997,15 @@ const lsjk::JTaskList* get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlis
             int b0 = pl12->hstart[L12], b1 = pl12->hstart[L12 + 1];
             int k0 = pl34->hstart[L34], k1 = pl34->hstart[L34 + 1];
             if (b1 <= b0 || k1 <= k0) continue;
-            const long long ntile = (b1 - b0 + 63) / 64;
-            const long long chunk = j_chunk(k1 - k0, ntile, L12 + L34);
-            for (int b = b0; b < b1; b += 64)
-                for (int k = k0; k < k1; k += (int)chunk)
-                    tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + (int)chunk, k1)});
+            if (pl12->nmol > 0) {
+                append_batch_jtasks(pl12, L12, L34, false, tasks);
+            } else {
+                const long long ntile = (b1 - b0 + 63) / 64;
+                const long long chunk = j_chunk(k1 - k0, ntile, L12 + L34);
+                for (int b = b0; b < b1; b += 64)
+                    for (int k = k0; k < k1; k += (int)chunk)
+                        tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + (int)chunk, k1)});
+            }
             seg.cnt[L12][L34] = (int)tasks.size() - seg.off[L12][L34];
         }
     return &pl12->jtasks.emplace(pl34->uid, upload_jtasks(tasks, seg)).first->second;
@@ -929,6 +1025,11 @@ void build_jtasks_sym(lsjk_pairlist* pl)
             const int b0 = pl->hstart[L12], b1 = pl->hstart[L12 + 1];
             const int k0 = pl->hstart[L34], k1 = pl->hstart[L34 + 1];
             if (b1 <= b0 || k1 <= k0) continue;
+            if (pl->nmol > 0) {
+                append_batch_jtasks(pl, L12, L34, true, tasks);
+                seg.cnt[L12][L34] = (int)tasks.size() - seg.off[L12][L34];
+                continue;
+            }
             const long long ntile = (b1 - b0 + 63) / 64;
             const long long chunk = j_chunk((k1 - k0) / (L12 == L34 ? 2 : 1), ntile, L12 + L34);
             for (int b = b0; b < b1; b += 64) {
@@ -1034,12 +1135,37 @@ int lsjk::persistent_grid(const void* kernel, int block, size_t lds, long long n
     return (int)std::max<long long>(1, std::min<long long>(g, nwork_groups));
 }
 
+// How a build call runs.  ws: a fixed workspace of ws_bytes for the per-call buffers (nullptr:
+// the session workspace, grown as needed).  noalloc: allocate no device memory (the J task list
+// must exist already; lsjk_coulomb_noalloc).  graph: only enqueue the launches on the library
+// stream (it is being captured into a hipGraph): no timing events, no synchronisation, no
+// mirror-class timing, no per-launch timers.
+struct CallOpts {
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    bool noalloc = false;
+    bool graph = false;
+    const char* who = nullptr;
+};
+
+void commit_ws(Workspace& wsp, const CallOpts& o, const char* who)
+{
+    if (o.ws) {
+        wsp.commit(o.ws, o.ws_bytes, who);
+    } else {
+        if (o.noalloc || o.graph) throw LsjkError(LSJK_E_INVALID, std::string(who) + ": no fixed workspace given");
+        wsp.commit();
+    }
+}
+
 // lsjk_coulomb: workspace, task lists, the launches on the library stream.
 static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
-                         const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev)
+                         const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev,
+                         const CallOpts& o = CallOpts())
 {
         using namespace lsjk;
-        const char* who = "lsjk_coulomb";
+        const bool noalloc = o.noalloc;
+        const char* who = o.who ? o.who : "lsjk_coulomb";
         require_init();
         if (!pl12 || !pl34 || !D34_dev || !J12_dev) throw LsjkError(LSJK_E_INVALID, std::string(who) + ": null argument");
         // mixed precision (intbox.hpp:736-742) when thredp > thresp: FP64 at or above thredp,
@@ -1060,14 +1186,20 @@ static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
         float* kd32 = nullptr;
         if (mixed) wsp.add(&kd32, pl34->hoff[2 * kMaxL + 1]);
         CallScalars* sc = S.d_sc;
-        wsp.commit();
+        commit_ws(wsp, o, who);
         // one list on both sides (FP64 route): the symmetric kernel takes the group pairs
         // js_enabled admits, each integral once for both J blocks
         const bool jsym = (pl12 == pl34) && !mixed;
-        const JTaskList& seg = *get_jtasks(pl12, pl34);
+        const JTaskList* segp = get_jtasks(pl12, pl34, !noalloc && !o.graph);
+        if (!segp)
+            throw LsjkError(LSJK_E_INVALID, std::string(who) + ": no J task list for this pair of lists (one is built "
+                                            "when a list is created, for that list on both sides, and by the first "
+                                            "lsjk_coulomb call of any other pair)");
+        const JTaskList& seg = *segp;
         const JTaskList& sseg = pl12->jtasks_sym;
 
-        HIPCHK(hipEventRecord(S.ev[0], st));
+        const bool timed = !o.graph;
+        if (timed) HIPCHK(hipEventRecord(S.ev[0], st));
         launch_fill(sc, sizeof(CallScalars), st);
         launch_fill(jp, std::max<size_t>(1, pl12->hoff[2 * kMaxL + 1]) * sizeof(double), st);
         const float max12 = pl12->maxbound;
@@ -1082,7 +1214,7 @@ static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
         pa.eA = b3->d_e; pa.xA = b3->d_x; pa.yA = b3->d_y; pa.zA = b3->d_z; pa.cartA = b3->d_cart;
         pa.eB = b4->d_e; pa.xB = b4->d_x; pa.yB = b4->d_y; pa.zB = b4->d_z; pa.cartB = b4->d_cart;
         pa.D = D34_dev;
-        pa.ldD = b4->ncart;
+        pa.ldD = mat_ld(b4);
         pa.symmetric = pl34->symmetric;
         pa.thre_ket = thre_ket;
         pa.kb = kb;
@@ -1096,9 +1228,10 @@ static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
         }
         HIPCHK(hipGetLastError());
 
-        HIPCHK(hipEventRecord(S.ev[1], st));
+        if (timed) HIPCHK(hipEventRecord(S.ev[1], st));
         S.kid.clear();
         LaunchTimer lt;
+        if (noalloc || o.graph) lt.on = false;  // the per-launch event pool would grow
         if (ew.n > 0) {
             Fanout fo;
             fo.begin();
@@ -1160,7 +1293,7 @@ static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
             fo.end();
             HIPCHK(hipGetLastError());
         }
-        HIPCHK(hipEventRecord(S.ev[2], st));
+        if (timed) HIPCHK(hipEventRecord(S.ev[2], st));
 
         JScatterArgs sa{};
         sa.p1 = pl12->d_p1;
@@ -1172,7 +1305,7 @@ static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
         sa.eB = b2->d_e; sa.xB = b2->d_x; sa.yB = b2->d_y; sa.zB = b2->d_z; sa.cartB = b2->d_cart;
         sa.jp = jp;
         sa.J = J12_dev;
-        sa.ldJ = b2->ncart;
+        sa.ldJ = mat_ld(b2);
         sa.symmetric = pl12->symmetric;
         sa.screen = 1;
         sa.thresp = thresp;
@@ -1185,10 +1318,599 @@ static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
             }
             HIPCHK(hipGetLastError());
         }
-        HIPCHK(hipEventRecord(S.ev[3], st));
-        finish_call(S.counting, sc);
+        if (timed) {
+            HIPCHK(hipEventRecord(S.ev[3], st));
+            finish_call(S.counting, sc);
+        }
 }
 
+
+namespace {
+void basis_free(lsjk_basis* b)
+{
+    if (!b) return;
+    dev_free(b->d_atom);
+    dev_free(b->d_c);
+    dev_free(b->d_e);
+    dev_free(b->d_x);
+    dev_free(b->d_y);
+    dev_free(b->d_z);
+    dev_free(b->d_L);
+    dev_free(b->d_cart);
+    dev_free(b->d_blk);
+    dev_free(b->d_blk_cart);
+    dev_free(b->d_blk_L);
+    dev_free(b->d_blk_lo);
+    dev_free(b->d_blk_hi);
+    delete b;
+}
+
+// A device basis from primitive arrays (lsjk_basis_create; also the combined basis of a batch).
+lsjk_basis* basis_new(int nprim, const double* c, const double* e, const double* xyz, const int* L, const int* cart,
+                      int ncart)
+{
+    lsjk_basis* b = new lsjk_basis();
+    try {
+        b->nprim = nprim;
+        b->ncart = ncart;
+        b->h_c.assign(c, c + nprim);
+        b->h_e.assign(e, e + nprim);
+        b->h_xyz.assign(xyz, xyz + 3 * (size_t)nprim);
+        b->h_L.assign(L, L + nprim);
+        b->h_cart.assign(cart, cart + nprim);
+        b->maxL = 0;
+        for (int i = 0; i < nprim; i++) {
+            if (L[i] < 0 || L[i] > lsjk::kMaxL)
+                throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk_basis_create: angular momentum above d is not supported");
+            if (cart[i] < 0 || cart[i] + lsjk::ncart(L[i]) > ncart)
+                throw LsjkError(LSJK_E_INVALID, "lsjk_basis_create: Cartesian offset out of range");
+            b->maxL = std::max(b->maxL, L[i]);
+        }
+        // blocks: distinct Cartesian offsets (one per shell)
+        b->h_blk.resize(nprim);
+        for (int i = 0; i < nprim; i++) {
+            if (i == 0 || cart[i] != cart[i - 1]) {
+                b->h_blk_cart.push_back(cart[i]);
+                b->h_blk_L.push_back(L[i]);
+            } else if (L[i] != L[i - 1]) {
+                throw LsjkError(LSJK_E_INVALID, "lsjk_basis_create: primitives sharing a Cartesian offset differ in L");
+            }
+            b->h_blk[i] = (int)b->h_blk_cart.size() - 1;
+        }
+        b->nblk = (int)b->h_blk_cart.size();
+        std::vector<double> x(nprim), y(nprim), z(nprim);
+        for (int i = 0; i < nprim; i++) {
+            x[i] = xyz[3 * i];
+            y[i] = xyz[3 * i + 1];
+            z[i] = xyz[3 * i + 2];
+        }
+        b->d_c = dev_upload(b->h_c);
+        b->d_e = dev_upload(b->h_e);
+        b->d_x = dev_upload(x);
+        b->d_y = dev_upload(y);
+        b->d_z = dev_upload(z);
+        b->d_L = dev_upload(b->h_L);
+        b->d_cart = dev_upload(b->h_cart);
+        b->d_blk = dev_upload(b->h_blk);
+        b->d_blk_cart = dev_upload(b->h_blk_cart);
+        b->d_blk_L = dev_upload(b->h_blk_L);
+    } catch (...) {
+        basis_free(b);
+        throw;
+    }
+    return b;
+}
+
+}  // namespace
+
+// lsjk_exchange: workspace, unit lists, mirror-class choices, the launches on the library stream.
+static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
+                          int nterm, const double* scales, const double* omegas, float thresp, float thredp,
+                          double* K13_dev, const CallOpts& o = CallOpts())
+{
+        using namespace lsjk;
+        const char* who = o.who ? o.who : "lsjk_exchange";
+        require_init();
+        if (!pl12 || !pl34 || !D24_dev || !K13_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_exchange: null argument");
+        // mixed precision when thredp > thresp (intbox.hpp:787-793)
+        const bool mixed = thredp > thresp;
+        Ewald ew;
+        check_ewald(nterm, scales, omegas, ew);
+        hipStream_t st = S.stream;
+        const Basis *b1 = pl12->b1, *b2 = pl12->b2, *b3 = pl34->b1, *b4 = pl34->b2;
+        const bool symmode = D24symm && (pl12 == pl34);
+        // mirror-class choices of the symmetric mode (see below); a scratch K while they are timed
+        unsigned tb;
+        std::memcpy(&tb, &thresp, sizeof(tb));
+        // keyed by route as well: the two routes' kernels differ in speed
+        const auto okey = std::make_pair(pl34->uid, ((unsigned long long)tb << 1) | (mixed ? 1ull : 0ull));
+        auto& ocache = const_cast<lsjk_pairlist*>(pl12)->korient;
+        const bool tune = symmode && ew.n > 0 && !ocache.count(okey) && !o.graph && !o.noalloc;
+        std::map<int, int> tuned;  // filled by the tuning below, stored only once it is complete
+        const std::map<int, int>* orient = nullptr;
+        if (symmode && !tune) {
+            auto it = ocache.find(okey);
+            if (it != ocache.end()) orient = &it->second;
+        }
+        double* Dsh = nullptr;
+        double* Ks = nullptr;
+        Workspace wsp;
+        wsp.add(&Dsh, blk_elems(b2, b4));
+        if (tune) wsp.add(&Ks, mat_elems(b1, b3));
+        commit_ws(wsp, o, who);
+        const bool timed = !o.graph;
+
+        if (timed) HIPCHK(hipEventRecord(S.ev[0], st));
+        launch_fill(S.d_sc, sizeof(CallScalars), st);
+        launch_k_blockmax(D24_dev, mat_ld(b4), b2->d_blk_cart, b2->d_blk_L, b2->nblk, b4->d_blk_cart, b4->d_blk_L,
+                          b4->nblk, Dsh, blk_ld(b4), b2->d_blk_lo, b2->d_blk_hi, b2->ldblk, S.d_sc, st);
+        HIPCHK(hipGetLastError());
+        if (timed) HIPCHK(hipEventRecord(S.ev[1], st));
+        S.kid.clear();
+        LaunchTimer lt;
+        if (o.noalloc || o.graph) lt.on = false;
+        if (ew.n > 0) {
+            Fanout fo;
+            struct KL {
+                int L1, L2, L3, L4;
+                const KUnitG* units;
+                int nunits;
+                double cost;
+                KArgs ka;
+                KGroups kg;
+            };
+            // The launch of one class: tri = the triangle rule (row1 rank <= row3 rank, equal row
+            // momenta of one list); K = the output.  False when the class has no work.
+            auto make_kl = [&](int L1, int L2, int L3, int L4, bool tri, double* K, KL& out) {
+                const auto& rows1 = pl12->h_rows[L1][L2];
+                const auto& rows3 = pl34->h_rows[L3][L4];
+                if (rows1.empty() || rows3.empty()) return false;
+                double cost = 0.0;
+                const auto units = get_kunits_g(pl12, pl34, L1, L2, L3, L4, tri, thresp, &cost);
+                if (units.second == 0) return false;
+                const KGroups kg = kgroups_args(get_kgroups(const_cast<lsjk_pairlist*>(pl34), L3, L4));
+                KArgs ka{};
+                ka.rys = S.rys;
+                ka.ew = ew;
+                ka.thresp = thresp;
+                ka.thredp = thredp;
+                ka.count = S.counting ? 1 : 0;
+                ka.symmetric = symmode ? 1 : 0;
+                ka.tri = tri ? 1 : 0;
+                const lsjk_pairlist* sides[2] = {pl12, pl34};
+                KSide* ks[2] = {&ka.s12, &ka.s34};
+                const int Ls[2] = {L1, L3}, Lps[2] = {L2, L4};
+                for (int q = 0; q < 2; q++) {
+                    const lsjk_pairlist* pl = sides[q];
+                    KSide& sd = *ks[q];
+                    sd.row_start = pl->d_row_start;
+                    sd.maxLp = pl->kmaxL2;
+                    sd.eb = pl->d_ebound;
+                    sd.ecart = pl->d_ecart;
+                    sd.eblk = pl->d_eblk;
+                    sd.eg = pl->d_egeom;
+                    sd.eg32 = pl->d_egeom32;
+                    sd.nent = pl->nent;
+                    sd.rows = pl->d_rows[Ls[q]][Lps[q]];
+                    sd.nrows = (int)pl->h_rows[Ls[q]][Lps[q]].size();
+                    sd.x = pl->b1->d_x;
+                    sd.y = pl->b1->d_y;
+                    sd.z = pl->b1->d_z;
+                    sd.cart = pl->b1->d_cart;
+                }
+                ka.D = D24_dev;
+                ka.ldD = mat_ld(b4);
+                ka.Dsh = Dsh;
+                ka.ldDsh = blk_ld(b4);
+                ka.K = K;
+                ka.ldK = mat_ld(b3);
+                ka.maxb12 = pl12->maxbound;
+                ka.maxb34 = pl34->maxbound;
+                ka.sc = S.d_sc;
+                // class cost: candidate pairs x FLOPs of one quartet of the class
+                const int Lt = L1 + L2 + L3 + L4;
+                const double Pi = (double)ncart(L1) * ncart(L2) * ncart(L3) * ncart(L4);
+                const double G = (double)(L1 + L2 + 1) * (L3 + L4 + 1);
+                const double F = 40.0 + (Lt / 2 + 1) * (30.0 + 9.0 * G + 3.0 * Pi) + 2.0 * Pi;
+                out = KL{L1, L2, L3, L4, units.first, units.second, cost * F, ka, kg};
+                return true;
+            };
+            // Symmetric mode (K = K^T): a class and its mirror (L3 L4 | L1 L2) hold the same quartets
+            // with bra and ket swapped, so one of them, over all row pairs and with the transposed
+            // block, gives both.  Which one runs faster depends on the class shapes and the lists;
+            // at first use both are timed once (on the library stream, into a scratch K) and the
+            // faster one is kept for this pair of lists.  A class that is its own mirror takes the
+            // row triangle instead.
+            if (tune) {
+                // events and the counting flag are restored whatever happens in between
+                struct Guard {
+                    hipEvent_t e0 = nullptr, e1 = nullptr;
+                    bool counting = S.counting;
+                    ~Guard()
+                    {
+                        if (e0) (void)hipEventDestroy(e0);
+                        if (e1) (void)hipEventDestroy(e1);
+                        S.counting = counting;
+                    }
+                } g;
+                HIPCHK(hipEventCreate(&g.e0));
+                HIPCHK(hipEventCreate(&g.e1));
+                S.counting = false;
+                auto time_class = [&](int L1, int L2, int L3, int L4) {
+                    KL k;
+                    if (!make_kl(L1, L2, L3, L4, false, Ks, k)) return 0.0f;
+                    k.ka.count = 0;
+                    HIPCHK(hipEventRecord(g.e0, st));
+                    launch_k_ket(L1, L2, L3, L4, mixed ? kRouteMixed : kRoutePlain, k.ka, k.kg, k.units, k.nunits, st);
+                    HIPCHK(hipEventRecord(g.e1, st));
+                    HIPCHK(hipEventSynchronize(g.e1));
+                    float ms = 0.0f;
+                    HIPCHK(hipEventElapsedTime(&ms, g.e0, g.e1));
+                    return ms;
+                };
+                for (int L1 = 0; L1 <= b1->maxL; L1++)
+                    for (int L2 = 0; L2 <= b2->maxL; L2++)
+                        for (int L3 = 0; L3 <= b3->maxL; L3++)
+                            for (int L4 = 0; L4 <= b4->maxL; L4++) {
+                                const int c = kcount_index(L1, L2, L3, L4), m = kcount_index(L3, L4, L1, L2);
+                                if (c >= m) continue;  // each pair once; self-mirrors need no choice
+                                // twice each, alternating; the faster of each pair of runs counts
+                                float tc = time_class(L1, L2, L3, L4), tm = time_class(L3, L4, L1, L2);
+                                tc = std::min(tc, time_class(L1, L2, L3, L4));
+                                tm = std::min(tm, time_class(L3, L4, L1, L2));
+                                tuned[c] = tuned[m] = (tc <= tm) ? c : m;
+                            }
+                orient = &(ocache[okey] = tuned);
+            }
+            std::vector<KL> kls;
+            for (int L1 = 0; L1 <= b1->maxL; L1++)
+                for (int L3 = 0; L3 <= b3->maxL; L3++)
+                    for (int L2 = 0; L2 <= b2->maxL; L2++)
+                        for (int L4 = 0; L4 <= b4->maxL; L4++) {
+                            bool tri = false;
+                            if (symmode) {
+                                const int c = kcount_index(L1, L2, L3, L4), m = kcount_index(L3, L4, L1, L2);
+                                if (c == m) {
+                                    tri = true;
+                                } else {
+                                    // the stored choice, else the class of the lower index (both
+                                    // classes of a pair never run together)
+                                    const auto it = orient ? orient->find(c) : std::map<int, int>::const_iterator();
+                                    const int pick = (orient && it != orient->end()) ? it->second : std::min(c, m);
+                                    if (pick != c) continue;
+                                }
+                            }
+                            KL k;
+                            if (!make_kl(L1, L2, L3, L4, tri, K13_dev, k)) continue;
+                            // evaluated for the mirror class too: its diagonal blocks are the transposes
+                            k.ka.mirror_diag = (symmode && !tri) ? 1 : 0;
+                            kls.push_back(k);
+                        }
+            // heaviest classes first (estimated cost)
+            std::stable_sort(kls.begin(), kls.end(), [](const KL& a, const KL& b) { return a.cost > b.cost; });
+            fo.begin();
+            for (auto& k : kls) {
+                const int idx = kcount_index(k.L1, k.L2, k.L3, k.L4);
+                hipStream_t ss = fo.pick();
+                lt.begin(idx, ss);
+                launch_k_ket(k.L1, k.L2, k.L3, k.L4, mixed ? kRouteMixed : kRoutePlain, k.ka, k.kg, k.units, k.nunits, ss);
+                lt.end();
+            }
+            fo.end();
+            HIPCHK(hipGetLastError());
+        }
+        if (timed) {
+            HIPCHK(hipEventRecord(S.ev[2], st));
+            HIPCHK(hipEventRecord(S.ev[3], st));
+            finish_call(S.counting, S.d_sc);
+        }
+}
+
+namespace {
+// A device pair list (lsjk_pairlist_create; also the combined list of a batch, nmol > 0 with the
+// molecule of every basis-1 primitive in mol).
+lsjk_pairlist* pairlist_new(const lsjk_basis* b1, const lsjk_basis* b2, int symmetric, int npair, const int* p1,
+                            const int* p2, const float* bound, int nmol, const int* mol)
+{
+    if (symmetric && (b1->nprim != b2->nprim || b1->ncart != b2->ncart))
+        throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: symmetric list over bases of different size");
+    for (int k = 0; k < npair; k++) {
+        if (p1[k] < 0 || p1[k] >= b1->nprim || p2[k] < 0 || p2[k] >= b2->nprim)
+            throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: primitive index out of range");
+        if (symmetric && p2[k] > p1[k])
+            throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: symmetric list must have p2 <= p1");
+        if (nmol > 0 && mol[p1[k]] != mol[p2[k]])
+            throw LsjkError(LSJK_E_INVALID, "lsjk_batch_create: a pair joins two molecules");
+    }
+    lsjk_pairlist* pl = new lsjk_pairlist();
+    static unsigned long long next_uid = 0;
+    pl->uid = ++next_uid;
+    try {
+        pl->b1 = b1;
+        pl->b2 = b2;
+        pl->symmetric = symmetric ? 1 : 0;
+        pl->npair = npair;
+        pl->h_p1.assign(p1, p1 + npair);
+        pl->h_p2.assign(p2, p2 + npair);
+        pl->h_bound.assign(bound, bound + npair);
+        if (nmol > 0) {
+            pl->nmol = nmol;
+            pl->h_mol.assign(mol, mol + b1->nprim);
+        }
+        build_pairlist(pl);
+        // the J task lists of the list on both sides (the common use), so that a call with it
+        // on both sides allocates nothing
+        get_jtasks(pl, pl);
+        build_jtasks_sym(pl);
+    } catch (...) {
+        free_pairlist(pl);
+        delete pl;
+        throw;
+    }
+    g_lists.insert(pl);
+    return pl;
+}
+
+void pairlist_delete(lsjk_pairlist* pl)
+{
+    if (!pl) return;
+    g_lists.erase(pl);
+    for (lsjk_pairlist* o : g_lists) drop_caches_for(o, pl->uid);
+    free_pairlist(pl);
+    delete pl;
+}
+
+// ------------------------------------------------------------------ batches of molecules
+//
+// A batch holds many independent molecules as one basis (the molecules' primitives one after
+// another, Cartesian numbering continued) and one symmetric pair list whose pairs never join two
+// molecules.  J tasks and K units only pair a molecule with itself, so every class is one launch
+// over all molecules.  Matrices hold only the molecules' diagonal blocks: element (i, j) of the
+// global Cartesian numbering at i * ld + j with ld = the largest molecule (blocks do not overlap).
+// The first lsjk_batch_jk of a (D, J, K, thresholds, operator) combination runs eagerly (building
+// the caches and timing the mirror classes) and then captures the same launches into a hipGraph;
+// later calls replay the graph.
+struct BatchGraph {
+    const double* D = nullptr;
+    double *J = nullptr, *K = nullptr;
+    float thresp = 0.0f, thredp = 0.0f;
+    lsjk::Ewald ew{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool same(const double* d, const double* j, const double* k, float tp, float td, const lsjk::Ewald& e) const
+    {
+        if (d != D || j != J || k != K || tp != thresp || td != thredp || e.n != ew.n) return false;
+        for (int i = 0; i < e.n; i++)
+            if (e.alpha[i] != ew.alpha[i] || e.omega[i] != ew.omega[i]) return false;
+        return true;
+    }
+};
+}  // namespace
+
+struct lsjk_batch {
+    int nmol = 0;
+    lsjk_basis* basis = nullptr;
+    lsjk_pairlist* list = nullptr;
+    std::vector<int> cart0, ncart;  // per molecule: first global Cartesian index, count
+    int ld = 0;
+    size_t nelem = 0;
+    void* ws = nullptr;  // fixed workspace of the captured graphs
+    size_t ws_bytes = 0;
+    std::vector<BatchGraph> graphs;  // most recent last
+};
+
+namespace {
+
+constexpr size_t kBatchGraphsKept = 4;
+
+void batch_drop_graph(BatchGraph& g)
+{
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+}
+
+void batch_delete(lsjk_batch* b)
+{
+    if (!b) return;
+    for (auto& g : b->graphs) batch_drop_graph(g);
+    b->graphs.clear();
+    if (b->ws) (void)hipFree(b->ws);
+    pairlist_delete(b->list);
+    basis_free(b->basis);
+    delete b;
+}
+
+lsjk_batch* batch_new(int nmol, const lsjk_pairlist* const* lists)
+{
+    using namespace lsjk;
+    for (int m = 0; m < nmol; m++) {
+        const lsjk_pairlist* pl = lists[m];
+        if (!pl) throw LsjkError(LSJK_E_INVALID, "lsjk_batch_create: null pair list");
+        if (!pl->symmetric || pl->b1 != pl->b2)
+            throw LsjkError(LSJK_E_INVALID, "lsjk_batch_create: each molecule needs a symmetric list over one basis");
+        if (pl->b1->ld) throw LsjkError(LSJK_E_INVALID, "lsjk_batch_create: a list of a batch basis");
+    }
+    std::vector<double> c, e, xyz;
+    std::vector<int> L, cart, mol;
+    std::vector<int> prim0(nmol + 1, 0);
+    lsjk_batch* b = new lsjk_batch();
+    try {
+        b->nmol = nmol;
+        b->cart0.resize(nmol);
+        b->ncart.resize(nmol);
+        int nc = 0;
+        for (int m = 0; m < nmol; m++) {
+            const Basis* bm = lists[m]->b1;
+            b->cart0[m] = nc;
+            b->ncart[m] = bm->ncart;
+            prim0[m] = (int)L.size();
+            c.insert(c.end(), bm->h_c.begin(), bm->h_c.end());
+            e.insert(e.end(), bm->h_e.begin(), bm->h_e.end());
+            xyz.insert(xyz.end(), bm->h_xyz.begin(), bm->h_xyz.end());
+            L.insert(L.end(), bm->h_L.begin(), bm->h_L.end());
+            for (int i = 0; i < bm->nprim; i++) {
+                cart.push_back(bm->h_cart[i] + nc);
+                mol.push_back(m);
+            }
+            nc += bm->ncart;
+            b->ld = std::max(b->ld, bm->ncart);
+        }
+        prim0[nmol] = (int)L.size();
+        if ((double)std::max(nc, 1) * (double)(b->ld + 1) > 4.0e18)
+            throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk_batch_create: batch too large");
+        b->basis = basis_new((int)L.size(), c.data(), e.data(), xyz.data(), L.data(), cart.data(), nc);
+        Basis* bb = b->basis;
+        bb->ld = std::max(b->ld, 1);
+        bb->h_mol = mol;
+        // per block: its molecule's block range; ldblk = the most blocks of one molecule
+        std::vector<int> mb0(nmol + 1, bb->nblk);
+        for (int i = (int)mol.size() - 1; i >= 0; i--) mb0[mol[i]] = bb->h_blk[i];
+        for (int m = nmol - 1; m >= 0; m--)
+            if (prim0[m] == prim0[m + 1]) mb0[m] = mb0[m + 1];
+        bb->h_blk_lo.resize(bb->nblk);
+        bb->h_blk_hi.resize(bb->nblk);
+        for (int i = 0; i < (int)mol.size(); i++) {
+            const int k = bb->h_blk[i], m = mol[i];
+            bb->h_blk_lo[k] = mb0[m];
+            bb->h_blk_hi[k] = mb0[m + 1];
+        }
+        for (int m = 0; m < nmol; m++) bb->ldblk = std::max(bb->ldblk, mb0[m + 1] - mb0[m]);
+        bb->ldblk = std::max(bb->ldblk, 1);
+        bb->d_blk_lo = dev_upload(bb->h_blk_lo);
+        bb->d_blk_hi = dev_upload(bb->h_blk_hi);
+        b->nelem = mat_elems(bb, bb);
+        // pairs: class by class ((L1, L2) in L1-major order), molecule by molecule, each molecule's
+        // pairs of the class in its own (bound-sorted) order
+        std::vector<int> p1, p2;
+        std::vector<float> bd;
+        for (int L1 = 0; L1 <= kMaxL; L1++)
+            for (int L2 = 0; L2 <= kMaxL; L2++)
+                for (int m = 0; m < nmol; m++) {
+                    const lsjk_pairlist* pl = lists[m];
+                    const Basis* bm = pl->b1;
+                    for (int k = 0; k < pl->npair; k++) {
+                        if (bm->h_L[pl->h_p1[k]] != L1 || bm->h_L[pl->h_p2[k]] != L2) continue;
+                        p1.push_back(pl->h_p1[k] + prim0[m]);
+                        p2.push_back(pl->h_p2[k] + prim0[m]);
+                        bd.push_back(pl->h_bound[k]);
+                    }
+                }
+        b->list = pairlist_new(b->basis, b->basis, 1, (int)p1.size(), p1.data(), p2.data(), bd.data(), nmol, mol.data());
+    } catch (...) {
+        batch_delete(b);
+        throw;
+    }
+    return b;
+}
+
+// Bytes of the per-call buffers of one J + K build of a batch (as coulomb_call / exchange_call
+// carve them from a workspace): kb, kd, jp (and kd32 on the mixed route); Dsh.
+size_t batch_ws_bytes(const lsjk_batch* b, bool mixed, size_t* jbytes)
+{
+    auto r = [](size_t n) { return (std::max<size_t>(1, n) + 255) & ~size_t(255); };
+    const lsjk_pairlist* pl = b->list;
+    const size_t h = pl->hoff[2 * lsjk::kMaxL + 1];
+    size_t j = r((size_t)pl->npair * sizeof(float)) + r(h * sizeof(double)) + r(h * sizeof(double));
+    if (mixed) j += r(h * sizeof(float));
+    *jbytes = j;
+    return j + r(blk_elems(b->basis, b->basis) * sizeof(double));
+}
+
+void batch_jk(lsjk_batch* b, const double* D_dev, int nterm, const double* scales, const double* omegas, float thresp,
+              float thredp, double* J_dev, double* K_dev)
+{
+    using namespace lsjk;
+    require_init();
+    if (!b || !D_dev || (!J_dev && !K_dev)) throw LsjkError(LSJK_E_INVALID, "lsjk_batch_jk: null argument");
+    Ewald ew;
+    check_ewald(nterm, scales, omegas, ew);
+    const lsjk_pairlist* pl = b->list;
+    for (size_t i = 0; i < b->graphs.size(); i++) {
+        if (!b->graphs[i].same(D_dev, J_dev, K_dev, thresp, thredp, ew)) continue;
+        // replay
+        BatchGraph g = b->graphs[i];
+        b->graphs.erase(b->graphs.begin() + i);
+        b->graphs.push_back(g);
+        HIPCHK(hipEventRecord(S.ev[0], S.stream));
+        HIPCHK(hipGraphLaunch(g.exec, S.stream));
+        HIPCHK(hipEventRecord(S.ev[3], S.stream));
+        HIPCHK(hipStreamSynchronize(S.stream));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, S.ev[0], S.ev[3]));
+        S.ms[0] = S.ms[1] = ms;
+        S.kid.clear();
+        S.kms.clear();
+        std::memset(S.counts, 0, sizeof(S.counts));
+        std::memset(S.counts32, 0, sizeof(S.counts32));
+        return;
+    }
+    // eager run: the results of this call, and every cache the launches need (task and unit lists,
+    // ket groups, mirror-class choices, occupancy)
+    double ms = 0.0;
+    if (J_dev) {
+        coulomb_call(pl, pl, D_dev, nterm, scales, omegas, thresp, thredp, J_dev);
+        ms += S.ms[0];
+    }
+    if (K_dev) {
+        exchange_call(pl, pl, D_dev, 1, nterm, scales, omegas, thresp, thredp, K_dev);
+        ms += S.ms[0];
+    }
+    // then capture the same launches into a graph, for the next calls with these arguments
+    size_t jbytes = 0;
+    const size_t need = batch_ws_bytes(b, thredp > thresp, &jbytes);
+    if (need > b->ws_bytes) {
+        for (auto& g : b->graphs) batch_drop_graph(g);  // they use the old workspace
+        b->graphs.clear();
+        if (b->ws) HIPCHK(hipFree(b->ws));
+        b->ws = nullptr;
+        b->ws_bytes = 0;
+        HIPCHK(hipMalloc(&b->ws, need));
+        b->ws_bytes = need;
+    }
+    if (b->graphs.size() >= kBatchGraphsKept) {
+        batch_drop_graph(b->graphs.front());
+        b->graphs.erase(b->graphs.begin());
+    }
+    BatchGraph g;
+    g.D = D_dev;
+    g.J = J_dev;
+    g.K = K_dev;
+    g.thresp = thresp;
+    g.thredp = thredp;
+    g.ew = ew;
+    HIPCHK(hipStreamBeginCapture(S.stream, hipStreamCaptureModeRelaxed));
+    try {
+        CallOpts o;
+        o.graph = true;
+        o.who = "lsjk_batch_jk";
+        if (J_dev) {
+            o.ws = b->ws;
+            o.ws_bytes = jbytes;
+            coulomb_call(pl, pl, D_dev, nterm, scales, omegas, thresp, thredp, J_dev, o);
+        }
+        if (K_dev) {
+            o.ws = (char*)b->ws + jbytes;
+            o.ws_bytes = b->ws_bytes - jbytes;
+            exchange_call(pl, pl, D_dev, 1, nterm, scales, omegas, thresp, thredp, K_dev, o);
+        }
+    } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(S.stream, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(S.stream, &g.graph));
+    if (hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0) != hipSuccess) {
+        batch_drop_graph(g);
+        throw LsjkError(LSJK_E_HIP, "lsjk_batch_jk: graph instantiation failed");
+    }
+    b->graphs.push_back(g);
+    S.ms[0] = S.ms[1] = ms;
+}
+
+}  // namespace
 
 // ====================================================================== C-ABI
 
@@ -1293,56 +2015,7 @@ int lsjk_basis_create(int nprim, const double* c, const double* e, const double*
         require_init();
         if (!out || nprim < 0 || (nprim > 0 && (!c || !e || !xyz || !L || !cart)))
             throw LsjkError(LSJK_E_INVALID, "lsjk_basis_create: null argument");
-        lsjk_basis* b = new lsjk_basis();
-        try {
-            b->nprim = nprim;
-            b->ncart = ncart;
-            b->h_c.assign(c, c + nprim);
-            b->h_e.assign(e, e + nprim);
-            b->h_xyz.assign(xyz, xyz + 3 * (size_t)nprim);
-            b->h_L.assign(L, L + nprim);
-            b->h_cart.assign(cart, cart + nprim);
-            b->maxL = 0;
-            for (int i = 0; i < nprim; i++) {
-                if (L[i] < 0 || L[i] > lsjk::kMaxL)
-                    throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk_basis_create: angular momentum above d is not supported");
-                if (cart[i] < 0 || cart[i] + lsjk::ncart(L[i]) > ncart)
-                    throw LsjkError(LSJK_E_INVALID, "lsjk_basis_create: Cartesian offset out of range");
-                b->maxL = std::max(b->maxL, L[i]);
-            }
-            // blocks: distinct Cartesian offsets (one per shell)
-            b->h_blk.resize(nprim);
-            for (int i = 0; i < nprim; i++) {
-                if (i == 0 || cart[i] != cart[i - 1]) {
-                    b->h_blk_cart.push_back(cart[i]);
-                    b->h_blk_L.push_back(L[i]);
-                } else if (L[i] != L[i - 1]) {
-                    throw LsjkError(LSJK_E_INVALID, "lsjk_basis_create: primitives sharing a Cartesian offset differ in L");
-                }
-                b->h_blk[i] = (int)b->h_blk_cart.size() - 1;
-            }
-            b->nblk = (int)b->h_blk_cart.size();
-            std::vector<double> x(nprim), y(nprim), z(nprim);
-            for (int i = 0; i < nprim; i++) {
-                x[i] = xyz[3 * i];
-                y[i] = xyz[3 * i + 1];
-                z[i] = xyz[3 * i + 2];
-            }
-            b->d_c = dev_upload(b->h_c);
-            b->d_e = dev_upload(b->h_e);
-            b->d_x = dev_upload(x);
-            b->d_y = dev_upload(y);
-            b->d_z = dev_upload(z);
-            b->d_L = dev_upload(b->h_L);
-            b->d_cart = dev_upload(b->h_cart);
-            b->d_blk = dev_upload(b->h_blk);
-            b->d_blk_cart = dev_upload(b->h_blk_cart);
-            b->d_blk_L = dev_upload(b->h_blk_L);
-        } catch (...) {
-            lsjk_basis_destroy(b);
-            throw;
-        }
-        *out = b;
+        *out = basis_new(nprim, c, e, xyz, L, cart, ncart);
     });
 }
 
@@ -1362,21 +2035,7 @@ int lsjk_basis_set_atoms(lsjk_basis* b, int natom, const int* atom)
 
 int lsjk_basis_destroy(lsjk_basis* b)
 {
-    return guarded([&] {
-        if (!b) return;
-        dev_free(b->d_atom);
-        dev_free(b->d_c);
-        dev_free(b->d_e);
-        dev_free(b->d_x);
-        dev_free(b->d_y);
-        dev_free(b->d_z);
-        dev_free(b->d_L);
-        dev_free(b->d_cart);
-        dev_free(b->d_blk);
-        dev_free(b->d_blk_cart);
-        dev_free(b->d_blk_L);
-        delete b;
-    });
+    return guarded([&] { basis_free(b); });
 }
 
 int lsjk_pairlist_create(const lsjk_basis* b1, const lsjk_basis* b2, int symmetric, int npair, const int* p1,
@@ -1386,49 +2045,44 @@ int lsjk_pairlist_create(const lsjk_basis* b1, const lsjk_basis* b2, int symmetr
         require_init();
         if (!b1 || !b2 || !out || npair < 0 || (npair > 0 && (!p1 || !p2 || !bound)))
             throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: null argument");
-        if (symmetric && (b1->nprim != b2->nprim || b1->ncart != b2->ncart))
-            throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: symmetric list over bases of different size");
-        for (int k = 0; k < npair; k++) {
-            if (p1[k] < 0 || p1[k] >= b1->nprim || p2[k] < 0 || p2[k] >= b2->nprim)
-                throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: primitive index out of range");
-            if (symmetric && p2[k] > p1[k])
-                throw LsjkError(LSJK_E_INVALID, "lsjk_pairlist_create: symmetric list must have p2 <= p1");
-        }
-        lsjk_pairlist* pl = new lsjk_pairlist();
-        static unsigned long long next_uid = 0;
-        pl->uid = ++next_uid;
-        try {
-            pl->b1 = b1;
-            pl->b2 = b2;
-            pl->symmetric = symmetric ? 1 : 0;
-            pl->npair = npair;
-            pl->h_p1.assign(p1, p1 + npair);
-            pl->h_p2.assign(p2, p2 + npair);
-            pl->h_bound.assign(bound, bound + npair);
-            build_pairlist(pl);
-            // the J task lists of the list on both sides (the common use), so that a call with it
-            // on both sides allocates nothing
-            get_jtasks(pl, pl);
-            build_jtasks_sym(pl);
-        } catch (...) {
-            free_pairlist(pl);
-            delete pl;
-            throw;
-        }
-        g_lists.insert(pl);
-        *out = pl;
+        *out = pairlist_new(b1, b2, symmetric, npair, p1, p2, bound, 0, nullptr);
     });
 }
 
 int lsjk_pairlist_destroy(lsjk_pairlist* pl)
 {
+    return guarded([&] { pairlist_delete(pl); });
+}
+
+int lsjk_batch_create(int nmol, const lsjk_pairlist* const* lists, lsjk_batch** out)
+{
     return guarded([&] {
-        if (!pl) return;
-        g_lists.erase(pl);
-        for (lsjk_pairlist* o : g_lists) drop_caches_for(o, pl->uid);
-        free_pairlist(pl);
-        delete pl;
+        require_init();
+        if (!out || nmol <= 0 || !lists) throw LsjkError(LSJK_E_INVALID, "lsjk_batch_create: null argument");
+        *out = batch_new(nmol, lists);
     });
+}
+
+int lsjk_batch_destroy(lsjk_batch* b)
+{
+    return guarded([&] { batch_delete(b); });
+}
+
+int lsjk_batch_layout(const lsjk_batch* b, int* ld, size_t* nelem, size_t* offsets)
+{
+    return guarded([&] {
+        if (!b || !ld || !nelem) throw LsjkError(LSJK_E_INVALID, "lsjk_batch_layout: null argument");
+        *ld = b->ld;
+        *nelem = b->nelem;
+        if (offsets)
+            for (int m = 0; m < b->nmol; m++) offsets[m] = (size_t)b->cart0[m] * (size_t)(b->ld + 1);
+    });
+}
+
+int lsjk_batch_jk(lsjk_batch* b, const double* D_dev, int nterm, const double* scales, const double* omegas,
+                  float thresp, float thredp, double* J_dev, double* K_dev)
+{
+    return guarded([&] { batch_jk(b, D_dev, nterm, scales, omegas, thresp, thredp, J_dev, K_dev); });
 }
 
 int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
@@ -1437,201 +2091,12 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
     return guarded([&] { coulomb_call(pl12, pl34, D34_dev, nterm, scales, omegas, thresp, thredp, J12_dev); });
 }
 
+
 int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
                   int nterm, const double* scales, const double* omegas, float thresp, float thredp, double* K13_dev)
 {
     return guarded([&] {
-        using namespace lsjk;
-        require_init();
-        if (!pl12 || !pl34 || !D24_dev || !K13_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_exchange: null argument");
-        // mixed precision when thredp > thresp (intbox.hpp:787-793)
-        const bool mixed = thredp > thresp;
-        Ewald ew;
-        check_ewald(nterm, scales, omegas, ew);
-        hipStream_t st = S.stream;
-        const Basis *b1 = pl12->b1, *b2 = pl12->b2, *b3 = pl34->b1, *b4 = pl34->b2;
-        const bool symmode = D24symm && (pl12 == pl34);
-        // mirror-class choices of the symmetric mode (see below); a scratch K while they are timed
-        unsigned tb;
-        std::memcpy(&tb, &thresp, sizeof(tb));
-        // keyed by route as well: the two routes' kernels differ in speed
-        const auto okey = std::make_pair(pl34->uid, ((unsigned long long)tb << 1) | (mixed ? 1ull : 0ull));
-        auto& ocache = const_cast<lsjk_pairlist*>(pl12)->korient;
-        const bool tune = symmode && ew.n > 0 && !ocache.count(okey);
-        std::map<int, int> tuned;  // filled by the tuning below, stored only once it is complete
-        const std::map<int, int>* orient = nullptr;
-        if (symmode && !tune) {
-            auto it = ocache.find(okey);
-            if (it != ocache.end()) orient = &it->second;
-        }
-        double* Dsh = nullptr;
-        double* Ks = nullptr;
-        Workspace wsp;
-        wsp.add(&Dsh, (size_t)b2->nblk * b4->nblk);
-        if (tune) wsp.add(&Ks, (size_t)b1->ncart * b3->ncart);
-        wsp.commit();
-
-        HIPCHK(hipEventRecord(S.ev[0], st));
-        launch_fill(S.d_sc, sizeof(CallScalars), st);
-        launch_k_blockmax(D24_dev, b4->ncart, b2->d_blk_cart, b2->d_blk_L, b2->nblk, b4->d_blk_cart, b4->d_blk_L,
-                          b4->nblk, Dsh, S.d_sc, st);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(S.ev[1], st));
-        S.kid.clear();
-        LaunchTimer lt;
-        if (ew.n > 0) {
-            Fanout fo;
-            struct KL {
-                int L1, L2, L3, L4;
-                const KUnitG* units;
-                int nunits;
-                double cost;
-                KArgs ka;
-                KGroups kg;
-            };
-            // The launch of one class: tri = the triangle rule (row1 rank <= row3 rank, equal row
-            // momenta of one list); K = the output.  False when the class has no work.
-            auto make_kl = [&](int L1, int L2, int L3, int L4, bool tri, double* K, KL& out) {
-                const auto& rows1 = pl12->h_rows[L1][L2];
-                const auto& rows3 = pl34->h_rows[L3][L4];
-                if (rows1.empty() || rows3.empty()) return false;
-                double cost = 0.0;
-                const auto units = get_kunits_g(pl12, pl34, L1, L2, L3, L4, tri, thresp, &cost);
-                if (units.second == 0) return false;
-                const KGroups kg = kgroups_args(get_kgroups(const_cast<lsjk_pairlist*>(pl34), L3, L4));
-                KArgs ka{};
-                ka.rys = S.rys;
-                ka.ew = ew;
-                ka.thresp = thresp;
-                ka.thredp = thredp;
-                ka.count = S.counting ? 1 : 0;
-                ka.symmetric = symmode ? 1 : 0;
-                ka.tri = tri ? 1 : 0;
-                const lsjk_pairlist* sides[2] = {pl12, pl34};
-                KSide* ks[2] = {&ka.s12, &ka.s34};
-                const int Ls[2] = {L1, L3}, Lps[2] = {L2, L4};
-                for (int q = 0; q < 2; q++) {
-                    const lsjk_pairlist* pl = sides[q];
-                    KSide& sd = *ks[q];
-                    sd.row_start = pl->d_row_start;
-                    sd.maxLp = pl->kmaxL2;
-                    sd.eb = pl->d_ebound;
-                    sd.ecart = pl->d_ecart;
-                    sd.eblk = pl->d_eblk;
-                    sd.eg = pl->d_egeom;
-                    sd.eg32 = pl->d_egeom32;
-                    sd.nent = pl->nent;
-                    sd.rows = pl->d_rows[Ls[q]][Lps[q]];
-                    sd.nrows = (int)pl->h_rows[Ls[q]][Lps[q]].size();
-                    sd.x = pl->b1->d_x;
-                    sd.y = pl->b1->d_y;
-                    sd.z = pl->b1->d_z;
-                    sd.cart = pl->b1->d_cart;
-                }
-                ka.D = D24_dev;
-                ka.ldD = b4->ncart;
-                ka.Dsh = Dsh;
-                ka.ldDsh = b4->nblk;
-                ka.K = K;
-                ka.ldK = b3->ncart;
-                ka.maxb12 = pl12->maxbound;
-                ka.maxb34 = pl34->maxbound;
-                ka.sc = S.d_sc;
-                // class cost: candidate pairs x FLOPs of one quartet of the class
-                const int Lt = L1 + L2 + L3 + L4;
-                const double Pi = (double)ncart(L1) * ncart(L2) * ncart(L3) * ncart(L4);
-                const double G = (double)(L1 + L2 + 1) * (L3 + L4 + 1);
-                const double F = 40.0 + (Lt / 2 + 1) * (30.0 + 9.0 * G + 3.0 * Pi) + 2.0 * Pi;
-                out = KL{L1, L2, L3, L4, units.first, units.second, cost * F, ka, kg};
-                return true;
-            };
-            // Symmetric mode (K = K^T): a class and its mirror (L3 L4 | L1 L2) hold the same quartets
-            // with bra and ket swapped, so one of them, over all row pairs and with the transposed
-            // block, gives both.  Which one runs faster depends on the class shapes and the lists;
-            // at first use both are timed once (on the library stream, into a scratch K) and the
-            // faster one is kept for this pair of lists.  A class that is its own mirror takes the
-            // row triangle instead.
-            if (tune) {
-                // events and the counting flag are restored whatever happens in between
-                struct Guard {
-                    hipEvent_t e0 = nullptr, e1 = nullptr;
-                    bool counting = S.counting;
-                    ~Guard()
-                    {
-                        if (e0) (void)hipEventDestroy(e0);
-                        if (e1) (void)hipEventDestroy(e1);
-                        S.counting = counting;
-                    }
-                } g;
-                HIPCHK(hipEventCreate(&g.e0));
-                HIPCHK(hipEventCreate(&g.e1));
-                S.counting = false;
-                auto time_class = [&](int L1, int L2, int L3, int L4) {
-                    KL k;
-                    if (!make_kl(L1, L2, L3, L4, false, Ks, k)) return 0.0f;
-                    k.ka.count = 0;
-                    HIPCHK(hipEventRecord(g.e0, st));
-                    launch_k_ket(L1, L2, L3, L4, mixed ? kRouteMixed : kRoutePlain, k.ka, k.kg, k.units, k.nunits, st);
-                    HIPCHK(hipEventRecord(g.e1, st));
-                    HIPCHK(hipEventSynchronize(g.e1));
-                    float ms = 0.0f;
-                    HIPCHK(hipEventElapsedTime(&ms, g.e0, g.e1));
-                    return ms;
-                };
-                for (int L1 = 0; L1 <= b1->maxL; L1++)
-                    for (int L2 = 0; L2 <= b2->maxL; L2++)
-                        for (int L3 = 0; L3 <= b3->maxL; L3++)
-                            for (int L4 = 0; L4 <= b4->maxL; L4++) {
-                                const int c = kcount_index(L1, L2, L3, L4), m = kcount_index(L3, L4, L1, L2);
-                                if (c >= m) continue;  // each pair once; self-mirrors need no choice
-                                // twice each, alternating; the faster of each pair of runs counts
-                                float tc = time_class(L1, L2, L3, L4), tm = time_class(L3, L4, L1, L2);
-                                tc = std::min(tc, time_class(L1, L2, L3, L4));
-                                tm = std::min(tm, time_class(L3, L4, L1, L2));
-                                tuned[c] = tuned[m] = (tc <= tm) ? c : m;
-                            }
-                orient = &(ocache[okey] = tuned);
-            }
-            std::vector<KL> kls;
-            for (int L1 = 0; L1 <= b1->maxL; L1++)
-                for (int L3 = 0; L3 <= b3->maxL; L3++)
-                    for (int L2 = 0; L2 <= b2->maxL; L2++)
-                        for (int L4 = 0; L4 <= b4->maxL; L4++) {
-                            bool tri = false;
-                            if (symmode) {
-                                const int c = kcount_index(L1, L2, L3, L4), m = kcount_index(L3, L4, L1, L2);
-                                if (c == m) {
-                                    tri = true;
-                                } else {
-                                    // the stored choice, else the class of the lower index (both
-                                    // classes of a pair never run together)
-                                    const auto it = orient ? orient->find(c) : std::map<int, int>::const_iterator();
-                                    const int pick = (orient && it != orient->end()) ? it->second : std::min(c, m);
-                                    if (pick != c) continue;
-                                }
-                            }
-                            KL k;
-                            if (!make_kl(L1, L2, L3, L4, tri, K13_dev, k)) continue;
-                            // evaluated for the mirror class too: its diagonal blocks are the transposes
-                            k.ka.mirror_diag = (symmode && !tri) ? 1 : 0;
-                            kls.push_back(k);
-                        }
-            // heaviest classes first (estimated cost)
-            std::stable_sort(kls.begin(), kls.end(), [](const KL& a, const KL& b) { return a.cost > b.cost; });
-            fo.begin();
-            for (auto& k : kls) {
-                const int idx = kcount_index(k.L1, k.L2, k.L3, k.L4);
-                hipStream_t ss = fo.pick();
-                lt.begin(idx, ss);
-                launch_k_ket(k.L1, k.L2, k.L3, k.L4, mixed ? kRouteMixed : kRoutePlain, k.ka, k.kg, k.units, k.nunits, ss);
-                lt.end();
-            }
-            fo.end();
-            HIPCHK(hipGetLastError());
-        }
-        HIPCHK(hipEventRecord(S.ev[2], st));
-        HIPCHK(hipEventRecord(S.ev[3], st));
-        finish_call(S.counting, S.d_sc);
+        exchange_call(pl12, pl34, D24_dev, D24symm, nterm, scales, omegas, thresp, thredp, K13_dev);
     });
 }
 
@@ -1756,7 +2221,7 @@ int lsjk_exchange_grad(const lsjk_pairlist* pl, const double* D13_dev, const dou
         wsp.commit();
         launch_fill(S.d_sc, sizeof(CallScalars), st);
         launch_k_blockmax(D24_dev, b->ncart, b->d_blk_cart, b->d_blk_L, b->nblk, b->d_blk_cart, b->d_blk_L, b->nblk, Dsh,
-                          S.d_sc, st);
+                          b->nblk, nullptr, nullptr, 0, S.d_sc, st);
         if (ew.n > 0) {
             for (int L1 = 0; L1 <= b->maxL; L1++)
                 for (int L2 = 0; L2 <= b->maxL; L2++)
@@ -1966,7 +2431,7 @@ int lsjk_count_quartets(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, co
         if (D24_dev) {
             launch_fill(S.d_sc, sizeof(CallScalars), st);
             launch_k_blockmax(D24_dev, b4->ncart, b2->d_blk_cart, b2->d_blk_L, b2->nblk, b4->d_blk_cart, b4->d_blk_L,
-                              b4->nblk, Dsh, S.d_sc, st);
+                              b4->nblk, Dsh, b4->nblk, nullptr, nullptr, 0, S.d_sc, st);
             CountKArgs ka{};
             ka.thresp = thresp;
             ka.half = (D24symm && pl12 == pl34) ? 1 : 0;

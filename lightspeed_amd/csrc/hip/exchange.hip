@@ -23,19 +23,29 @@ LSJK_GRAD_EXTERN(1, 0) LSJK_GRAD_EXTERN(1, 1) LSJK_GRAD_EXTERN(1, 2)
 LSJK_GRAD_EXTERN(2, 0) LSJK_GRAD_EXTERN(2, 1) LSJK_GRAD_EXTERN(2, 2)
 #undef LSJK_GRAD_EXTERN
 
-// Block maxima of |D24| per (block2, block4) and the global maximum.
+// Block maxima of |D24| per (block2, block4), Dsh[i2 * ldDsh + i4], and the global maximum.
+// Dense: every (i2, i4), thread per pair.  Banded (blo != nullptr, a batch basis of independent
+// molecules): only the pairs of one molecule, i4 in [blo[i2], bhi[i2]); thread (i2, j) takes
+// i4 = blo[i2] + j, j < band (the largest molecule's block count).
 __global__ void k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2, int nb2, const int* bc4,
-                           const int* bL4, int nb4, double* Dsh, CallScalars* sc)
+                           const int* bL4, int nb4, double* Dsh, int ldDsh, const int* blo, const int* bhi, int band,
+                           CallScalars* sc)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)nb2 * nb4) return;
-    const int i2 = (int)(i / nb4), i4 = (int)(i - (long long)i2 * nb4);
+    const int w = blo ? band : nb4;
+    if (i >= (long long)nb2 * w) return;
+    const int i2 = (int)(i / w);
+    int i4 = (int)(i - (long long)i2 * w);
+    if (blo) {
+        i4 += blo[i2];
+        if (i4 >= bhi[i2]) return;
+    }
     const int o2 = bc2[i2], o4 = bc4[i4];
     const int n2 = ncart(bL2[i2]), n4 = ncart(bL4[i4]);
     double m = 0.0;
     for (int b = 0; b < n2; b++)
         for (int d = 0; d < n4; d++) m = fmax(m, fabs(D[(size_t)(o2 + b) * ldD + (o4 + d)]));
-    Dsh[i] = m;
+    Dsh[(size_t)i2 * ldDsh + i4] = m;
     atomicMax(&sc->dmax_bits, (unsigned long long)__double_as_longlong(m));
 }
 
@@ -73,12 +83,13 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s)
 }
 
 void launch_k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2, int nb2, const int* bc4,
-                       const int* bL4, int nb4, double* Dsh, CallScalars* sc, hipStream_t s)
+                       const int* bL4, int nb4, double* Dsh, int ldDsh, const int* blo, const int* bhi, int band,
+                       CallScalars* sc, hipStream_t s)
 {
-    long long n = (long long)nb2 * nb4;
+    long long n = (long long)nb2 * (blo ? band : nb4);
     if (n == 0) return;
     hipLaunchKernelGGL(k_blockmax, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, ldD, bc2, bL2, nb2, bc4,
-                       bL4, nb4, Dsh, sc);
+                       bL4, nb4, Dsh, ldDsh, blo, bhi, band, sc);
 }
 
 void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,

@@ -24,7 +24,19 @@ struct Basis {
     int *d_L = nullptr, *d_cart = nullptr, *d_blk = nullptr, *d_blk_cart = nullptr, *d_blk_L = nullptr;
     int natom = 0;            // atoms of the basis (lsjk_basis_set_atoms; 0 = not set)
     int* d_atom = nullptr;    // atom index of each primitive
+    // Batch basis (independent molecules, lsjk_batch_create): matrices over it hold only each
+    // molecule's diagonal block, element (i, j) of the global Cartesian (or block) numbering at
+    // i * ld + j (or i * ldblk + j for block maxima); ld >= the largest molecule, so the blocks do
+    // not overlap.  0: plain ncart x ncart (nblk x nblk) matrices.
+    int ld = 0, ldblk = 0;
+    std::vector<int> h_mol;           // molecule of each primitive (batch basis)
+    std::vector<int> h_blk_lo, h_blk_hi;  // per block: its molecule's block range (batch basis)
+    int *d_blk_lo = nullptr, *d_blk_hi = nullptr;
 };
+
+// Row stride of the Cartesian matrices and of the block-maximum matrices over a basis.
+inline int mat_ld(const Basis* b) { return b->ld ? b->ld : b->ncart; }
+inline int blk_ld(const Basis* b) { return b->ldblk ? b->ldblk : b->nblk; }
 
 // ---------------------------------------------------------------- pair list
 // A J task list on the device: tasks of group pair (L12, L34) at [off, off + cnt).
@@ -88,6 +100,13 @@ struct PairList {
     std::vector<long long> h_tri_pre[kMaxL + 1][kMaxL + 1][kMaxL + 1];
     long long* d_tri_pre[kMaxL + 1][kMaxL + 1][kMaxL + 1] = {};
     int* d_tri_start[kMaxL + 1][kMaxL + 1][kMaxL + 1] = {};
+    // batch list (lsjk_batch_create): pairs of one molecule only; nmol molecules, h_mol[p] the
+    // molecule of basis-1 primitive p; in each Hermite group the molecules' pairs are contiguous,
+    // molecule m at [h_hmol[L][m], h_hmol[L][m + 1]), each sorted by bound.  Quartets are formed
+    // within one molecule only (J tasks and K units never pair two molecules).
+    int nmol = 0;
+    std::vector<int> h_mol;
+    std::vector<int> h_hmol[2 * kMaxL + 1];
     // process-unique id (never reused, unlike the object's address)
     unsigned long long uid = 0;
     // cached J task lists keyed by the ket list's uid
@@ -115,6 +134,7 @@ struct PairList {
         std::vector<int> grp_rank;
         std::vector<float> chunk_b;
         std::vector<int> order;     // indices into h_rows[L][Lp] in group order
+        std::vector<int> mol_grp;   // batch list: groups of molecule m at [mol_grp[m], mol_grp[m + 1])
     };
     std::map<int, KGroupData> kgroups;
     // cached K build unit lists (device pointer, count) keyed by (ket list uid, class and

@@ -283,9 +283,11 @@ void launch_j_quartets_grad(int L12p1, int L34, const JQuartetArgs& a, const JTa
 // the device's per-workgroup limit.
 int persistent_grid(const void* kernel, int block, size_t lds, long long nwork_groups);
 
+// blk_lo / blk_hi (nullptr: dense): per block 2, the range of blocks 4 of its molecule (batch
+// basis); band = the largest such range.
 void launch_k_blockmax(const double* D, int ldD, const int* blk_cart2, const int* blk_L2, int nblk2,
-                       const int* blk_cart4, const int* blk_L4, int nblk4, double* Dsh, CallScalars* sc,
-                       hipStream_t s);
+                       const int* blk_cart4, const int* blk_L4, int nblk4, double* Dsh, int ldDsh, const int* blk_lo,
+                       const int* blk_hi, int band, CallScalars* sc, hipStream_t s);
 // Exchange gradient classes (route kRouteGrad) over KUnit tiles.
 void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,
                        hipStream_t s);
@@ -293,6 +295,7 @@ void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a
 void launch_k_ket(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KGroups& g, const KUnitG* units,
                   int nunits, hipStream_t s);
 void launch_fill(void* p, size_t bytes, hipStream_t s);
+
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 }  // namespace lsjk

@@ -270,6 +270,22 @@ PYBIND11_MODULE(_lightspeed, m)
         .def_static("is_hip_ready", [](std::shared_ptr<ResourceList> r, std::shared_ptr<PairList> a,
                                        std::shared_ptr<PairList> b) { return IntBox::is_hip_ready(r, a, b); });
 
+    py::class_<JKBatch, std::shared_ptr<JKBatch>>(m, "JKBatch")
+        .def_static("build", &JKBatch::build, py::arg("resources"), py::arg("pairlists"))
+        .def_property_readonly("nmol", &JKBatch::nmol)
+        .def_property_readonly("ld", &JKBatch::ld)
+        .def_property_readonly("nelem", &JKBatch::nelem)
+        .def_property_readonly("offsets", &JKBatch::offsets)
+        .def_property_readonly("pairlists", &JKBatch::pairlists)
+        .def("jk", &JKBatch::jk, py::arg("ewald"), py::arg("Ds"), py::arg("thresp"), py::arg("thredp"),
+             py::call_guard<py::gil_scoped_release>())
+        .def(
+            "jk_device",
+            [](JKBatch& b, std::shared_ptr<Ewald> ew, uintptr_t D, float thresp, float thredp, uintptr_t J,
+               uintptr_t K) { b.jk_device(ew, (const double*)D, thresp, thredp, (double*)J, (double*)K); },
+            py::arg("ewald"), py::arg("D_ptr"), py::arg("thresp"), py::arg("thredp"), py::arg("J_ptr"),
+            py::arg("K_ptr"), py::call_guard<py::gil_scoped_release>());
+
     m.def("hip_init", &hip_session_init, py::arg("device") = 0);
     m.def("hip_finalize", &hip_session_finalize);
     m.def("device_count", &hip_device_count);

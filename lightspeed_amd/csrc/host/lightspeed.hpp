@@ -12,6 +12,7 @@
 
 struct lsjk_basis;
 struct lsjk_pairlist;
+struct lsjk_batch;
 
 namespace lightspeed {
 
@@ -279,6 +280,45 @@ struct IntBox {
     static bool is_hip_ready(const std::shared_ptr<ResourceList>& resources,
                              const std::shared_ptr<PairList>& pairlist12, const std::shared_ptr<PairList>& pairlist34,
                              std::string* why = nullptr);
+};
+
+// ------------------------------------------------------------------ batches of molecules
+// The J and K builds of many independent molecules in one set of class launches (the HIP library's
+// lsjk_batch_*), replayed from a captured hipGraph after the first call.  No counterpart in the
+// reference, whose callers run one IntBox::coulomb + IntBox::exchange per molecule (reference
+// est/rhf.py:616-643, src/cpp/intbox/mo_eri.cpp:10-250).  Molecule m's J and K equal
+// IntBox::coulomb(res, ewald, pl_m, pl_m, D_m, thresp, thredp) and IntBox::exchange(res, ewald,
+// pl_m, pl_m, D_m, true, thresp, thredp) (D_m symmetric).
+class JKBatch {
+public:
+    static std::shared_ptr<JKBatch> build(const std::shared_ptr<ResourceList>& resources,
+                                          const std::vector<std::shared_ptr<PairList>>& pairlists);
+    ~JKBatch();
+    JKBatch(const JKBatch&) = delete;
+    JKBatch& operator=(const JKBatch&) = delete;
+    size_t nmol() const { return pairlists_.size(); }
+    const std::vector<std::shared_ptr<PairList>>& pairlists() const { return pairlists_; }
+    // Device layout: molecule m's Cartesian (ncart_m x ncart_m) block, element (i, j), at
+    // offsets()[m] + i * ld() + j of an nelem()-double buffer.
+    int ld() const { return ld_; }
+    size_t nelem() const { return nelem_; }
+    const std::vector<size_t>& offsets() const { return offsets_; }
+    // J_dev += J[D], K_dev += K[D] for every molecule, device buffers in the layout above (either
+    // output may be null).
+    void jk_device(const std::shared_ptr<Ewald>& ewald, const double* D_dev, float thresp, float thredp,
+                   double* J_dev, double* K_dev);
+    // Host densities (nao_m x nao_m each) -> fresh J_m, K_m.
+    std::pair<std::vector<std::shared_ptr<Tensor>>, std::vector<std::shared_ptr<Tensor>>> jk(
+        const std::shared_ptr<Ewald>& ewald, const std::vector<std::shared_ptr<Tensor>>& Ds, float thresp,
+        float thredp);
+
+private:
+    JKBatch() = default;
+    std::vector<std::shared_ptr<PairList>> pairlists_;
+    lsjk_batch* dev_ = nullptr;
+    int ld_ = 0;
+    size_t nelem_ = 0;
+    std::vector<size_t> offsets_;
 };
 
 // Session helpers (the HIP library is one process-wide session, like TCIntBox).

@@ -92,6 +92,21 @@ def random_density(nao, nocc, seed):
     return Q @ Q.T
 
 
+def water_cluster_box(n):
+    """Cube edge (angstrom) of a seeded (H2O)_n of the batched workload: 10 (n / 10)^(1/3), at least 4."""
+    return max(4.0, 10.0 * (n / 10.0) ** (1.0 / 3.0))
+
+
+def water_batch(nmol=256, seed=3, nmax=16):
+    """The batched workload: nmol independent water clusters (H2O)_n with n drawn uniformly from
+    1..nmax (seeded), molecule i in a cube of edge water_cluster_box(n) (cluster seed 5000 + i),
+    nocc = 5 n, density seed 7000 + i.  Returns a list of (molecule, nocc, density seed)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(1, nmax + 1, size=nmol)
+    return [(water_cluster(int(n), water_cluster_box(int(n)), 5000 + i), 5 * int(n), 7000 + i)
+            for i, n in enumerate(ns)]
+
+
 # Named workloads: (molecule builder, basis, nocc, box) -- sizes quoted in the docstrings.
 def config(name):
     """Return (molecule, basis_name, nocc) for a named workload.
