@@ -90,6 +90,22 @@ int lsjk_pairlist_destroy(lsjk_pairlist* pl);
 int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
                  const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): lsjk_coulomb with the
+ * same arguments, kernels and results, but its per-call buffers (truncated ket bounds, ket Hermite
+ * densities and their FP32 copy, bra accumulators: about 16 bytes per Hermite component of the
+ * two lists) are taken from a fixed device scratch array of LSJK_NOALLOC_SCRATCH_BYTES that the
+ * library holds on the device from module load until the process ends.  The call allocates and
+ * frees nothing.  Limits: a call that needs more scratch fails with LSJK_E_UNSUPPORTED before any
+ * launch; the J task list of (pl12, pl34) must exist (it is built when a list is created, for that
+ * list on both sides, and by the first lsjk_coulomb call of any other pair), else LSJK_E_INVALID;
+ * per-launch timing (lsjk_set_profiling) is not recorded.  Every call on the device shares the one
+ * array, which keeps the last call's contents (each call clears what it accumulates into); calls
+ * are serialised by the session lock, and calls that overlapped on one GPU would race on it. */
+#define LSJK_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int lsjk_coulomb_noalloc(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
+                         const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev);
+
 /* Replaces IBoxKFockGen / IBoxKFockSym (reference tcintbox.cpp:299-334, called from
  * tc/exchange.cpp:14-75), with the semantics of IntBox::exchange (intbox.hpp:773-828):
  *   K13_dev (ncart1 x ncart3) += sum_24 (12|O|34) D24_dev (ncart2 x ncart4).

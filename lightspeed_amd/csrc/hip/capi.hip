@@ -26,6 +26,14 @@ namespace {
 struct lsjk_basis : public lsjk::Basis {};
 struct lsjk_pairlist : public lsjk::PairList {};
 
+// Device scratch of lsjk_coulomb_noalloc: its per-call J buffers (truncated ket bounds, ket
+// Hermite densities and their FP32 copy, bra accumulators) are carved from this array instead of
+// the session workspace, so that the call allocates nothing.  The runtime places one copy on each
+// device at module load and keeps it until the process ends.  Host code never passes the name
+// itself to a kernel (on the host it is a stand-in, not a device address): lsjk_init looks up the
+// device address once with hipGetSymbolAddress and every use goes through Session::scratch.
+__device__ __align__(256) unsigned char scratch[LSJK_NOALLOC_SCRATCH_BYTES];
+
 
 namespace {
 
@@ -56,6 +64,7 @@ struct Session {
     lsjk::CallScalars* d_sc = nullptr;
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    void* scratch = nullptr;  // device address of `scratch` on the session's device
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool counting = false;
     uint64_t counts[1024] = {0};
@@ -1976,6 +1985,10 @@ int lsjk_init(int device)
         }
         HIPCHK(hipMalloc((void**)&S.d_sc, sizeof(lsjk::CallScalars)));
         for (auto& e : S.ev) HIPCHK(hipEventCreate(&e));
+        // device address of the no-alloc scratch array on this device (the device is current)
+        S.scratch = nullptr;
+        HIPCHK(hipGetSymbolAddress(&S.scratch, HIP_SYMBOL(scratch)));
+        if (!S.scratch) throw LsjkError(LSJK_E_HIP, "lsjk_init: no device address for the scratch array");
 
         S.device = device;
         S.init = true;
@@ -2089,6 +2102,21 @@ int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const dou
                  const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev)
 {
     return guarded([&] { coulomb_call(pl12, pl34, D34_dev, nterm, scales, omegas, thresp, thredp, J12_dev); });
+}
+
+int lsjk_coulomb_noalloc(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
+                         const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev)
+{
+    return guarded([&] {
+        require_init();
+        if (!S.scratch) throw LsjkError(LSJK_E_NOINIT, "lsjk_coulomb_noalloc: scratch array address unknown");
+        CallOpts o;
+        o.ws = S.scratch;
+        o.ws_bytes = LSJK_NOALLOC_SCRATCH_BYTES;
+        o.noalloc = true;
+        o.who = "lsjk_coulomb_noalloc";
+        coulomb_call(pl12, pl34, D34_dev, nterm, scales, omegas, thresp, thredp, J12_dev, o);
+    });
 }
 
 

@@ -27,6 +27,7 @@ def load():
     L.lsjk_pairlist_destroy.argtypes = [vp]
     build = [vp, vp, vp, ci, vp, vp, cf, cf, vp]
     L.lsjk_coulomb.argtypes = build
+    L.lsjk_coulomb_noalloc.argtypes = build
     L.lsjk_exchange.argtypes = [vp, vp, vp, ci, ci, vp, vp, cf, cf, vp]
     L.lsjk_malloc.argtypes = [ctypes.POINTER(vp), csz]
     L.lsjk_free.argtypes = [vp]
@@ -34,7 +35,7 @@ def load():
     L.lsjk_memcpy_dtoh.argtypes = [vp, vp, csz]
     L.lsjk_memset_zero.argtypes = [vp, csz]
     for f in (L.lsjk_init, L.lsjk_basis_create, L.lsjk_basis_destroy, L.lsjk_pairlist_create,
-              L.lsjk_pairlist_destroy, L.lsjk_coulomb, L.lsjk_exchange, L.lsjk_malloc,
+              L.lsjk_pairlist_destroy, L.lsjk_coulomb, L.lsjk_coulomb_noalloc, L.lsjk_exchange, L.lsjk_malloc,
               L.lsjk_free, L.lsjk_memcpy_htod, L.lsjk_memcpy_dtoh, L.lsjk_memset_zero):
         f.restype = ci
     return L
@@ -107,6 +108,11 @@ class Lib:
     def coulomb(self, pl12, pl34, dD, ew, thresp, thredp, dJ):
         s, o = ew
         self.check(self.L.lsjk_coulomb(pl12, pl34, dD, len(s), s.ctypes.data, o.ctypes.data, thresp, thredp, dJ))
+
+    def coulomb_noalloc(self, pl12, pl34, dD, ew, thresp, thredp, dJ):
+        s, o = ew
+        self.check(self.L.lsjk_coulomb_noalloc(pl12, pl34, dD, len(s), s.ctypes.data, o.ctypes.data, thresp, thredp,
+                                               dJ))
 
     def exchange(self, pl12, pl34, dD, D24symm, ew, thresp, thredp, dK):
         s, o = ew
