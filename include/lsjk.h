@@ -143,6 +143,89 @@ int lsjk_batch_layout(const lsjk_batch* b, int* ld, size_t* nelem, size_t* offse
 int lsjk_batch_jk(lsjk_batch* b, const double* D_dev, int nterm, const double* scales, const double* omegas,
                   float thresp, float thredp, double* J_dev, double* K_dev);
 
+/* The learned screen.  The reference sieves exchange quartets by the Schwarz estimate alone,
+ * bound12 * bound34 * max|D24 block| against thresp (src/cpp/intbox/cpu/exchange.cpp:186, contract
+ * src/lightspeed/intbox.hpp:736-742 and 787-793).  The learned route tightens that estimate per
+ * exchange work unit (a bra row primitive against a spatial group of ket rows) with a network
+ * trained on the magnitudes of the unit's quartets, and routes each quartet by
+ * est * tau_u: skipped below thresp, FP32 below thredp, FP64 above.  Schwarz stays the fallback:
+ * tau_u <= 1, and classes the screen does not certify keep the plain FP64 route.
+ *
+ * Network (FP32): ni <= 32 inputs, 3 hidden layers of 64 ReLU units, one linear output; parameter
+ * vector W1 (64 x ni, row-major), b1 (64), W2 (64 x 64), b2, W3 (64 x 64), b3, w4 (64), b4 (1).
+ * lsjk_mlp_forward: out[s] = f(X[s]) for n rows of X (n x ni), device pointers.
+ * lsjk_mlp_grad: grad += d/dparams and loss[0] += L of L = (1/nb) sum_s sw_s r_s^2 (times wu where
+ * r_s < 0), r_s = f(X[idx[s]]) - y[idx[s]], over nb rows (idx NULL: rows 0..nb-1; sw NULL: 1).
+ * lsjk_mlp_adam: one Adam step (bias-corrected, step t >= 1) of params with gradient grad and the
+ * moment buffers m, v. */
+int lsjk_mlp_nparam(int ni, int* nparam);
+int lsjk_mlp_forward(int ni, const float* params_dev, int n, const float* X_dev, float* out_dev);
+int lsjk_mlp_grad(int ni, const float* params_dev, int nb, const int* idx_dev, const float* X_dev, const float* y_dev,
+                  const float* sw_dev, float wu, float* grad_dev, float* loss_dev);
+int lsjk_mlp_adam(int nparam, float* params_dev, const float* grad_dev, float* m_dev, float* v_dev, float lr,
+                  float beta1, float beta2, float eps, int t);
+
+/* Unit features and training targets of the exchange class (L1 L2 | L3 L4) of one list on both
+ * sides (all row pairs; the units lsjk_exchange forms at this thresp, in the same order).
+ * lsjk_unit_features: *nunits, and (feat non-NULL) LSJK_SCREEN_NFEAT floats per unit: L1, L2, L3, L4;
+ * ln of the bra row's exponent, of the smallest exponent among its partners in the unit, of the
+ * smallest exponent among the ket group's rows and among the unit's ket partners; the distance of
+ * the bra row to the ket group's centroid, the group's radius, the largest bra and ket pair extents
+ * (bohr); ln of the unit's largest bra bound and of its first chunk's largest ket bound; ln of the
+ * (ss|ss) ratio to its Schwarz product, sqrt(2) (pq)^(1/4) (p+q)^(-1/2) F0(rho R^2), for the unit's
+ * most diffuse pairs at the closest approach the geometry allows and at the centroid distance.
+ * lsjk_unit_magnitudes: per unit (host array of nunits floats), the largest
+ * max_components |(ab|cd)| / (bound_ab bound_cd) over the unit's quartets with
+ * bound_ab bound_cd >= smin, by the exact Rys evaluation on the GPU (1/r operator; classes with
+ * s and p functions only). */
+#define LSJK_SCREEN_NFEAT 16
+int lsjk_unit_features(const lsjk_pairlist* pl, int L1, int L2, int L3, int L4, float thresp, int* nunits,
+                       float* feat);
+int lsjk_unit_magnitudes(const lsjk_pairlist* pl, int L1, int L2, int L3, int L4, float thresp, float smin,
+                         float* mag);
+
+/* A trained screen: the network (host params), the normalisation of its inputs
+ * (x' = (x - mean) / scale, ni entries each) and, per exchange class index
+ * c = ((L1*5 + L2)*5 + L3)*5 + L4, nbin bins of the network output (ascending edges
+ * fedge[c * (nbin - 1) + b]) with a certified margin delta[c * nbin + b] each (NaN: not certified):
+ * tau_u = min(1, exp(f(x'_u) + delta of f's bin)), applied while thresp >= smin * max|D24|.
+ * lsjk_exchange_screened: lsjk_exchange on the learned route (certified classes with s and p
+ * functions: est * tau_u against thresp and thredp; all other classes: plain FP64 route).  The
+ * screen keeps the factors of each unit list it has seen until it is destroyed. */
+typedef struct lsjk_screen lsjk_screen;
+int lsjk_screen_create(int ni, const float* params, const float* mean, const float* scale, int nbin, const float* fedge,
+                       const double* delta, float smin, lsjk_screen** out);
+int lsjk_screen_destroy(lsjk_screen* screen);
+int lsjk_exchange_screened(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
+                           int nterm, const double* scales, const double* omegas, float thresp, float thredp,
+                           lsjk_screen* screen, double* K13_dev);
+
+/* Autotuner.  No counterpart in the reference, whose GPU tile choices live inside closed TeraChem
+ * (tcintbox.cpp:261-334).  On a symmetric list over one basis, used on both sides with the density
+ * D_dev (symmetric) and the route (thresp, thredp): times the J launches with the ket chunk of every
+ * task scaled by 1/4, 1/2, 1, 2 or 4, and the K launches with the work-unit split target of every
+ * class at 3072, 12288 or 49152 units (class launches serialised on one stream, HIP events around
+ * each, best of two), and keeps the fastest choice per J group pair and per K class for later
+ * builds of the list on both sides (the FP64 and mixed routes share the J choices).
+ * lsjk_tuned_params: the choices, n entries: kind (0 J one-way, 1 J symmetric, 2 K), counter index
+ * (J: L12*5 + L34; K: as lsjk_last_counts), value (chunk scale or split target), and the launch's ms
+ * with the default and with the choice.
+ * lsjk_batch_autotune: the same on a batch's combined list (D_dev in the batch layout); drops the
+ * batch's captured graphs.  lsjk_batch_pairlist: the batch's combined list (for lsjk_tuned_params). */
+int lsjk_autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const double* scales, const double* omegas,
+                  float thresp, float thredp);
+int lsjk_tuned_params(const lsjk_pairlist* pl, int capacity, int* n, int* kind, int* idx, double* value,
+                      double* ms_default, double* ms_tuned);
+int lsjk_batch_autotune(lsjk_batch* b, const double* D_dev, int nterm, const double* scales, const double* omegas,
+                        float thresp, float thredp);
+int lsjk_batch_pairlist(const lsjk_batch* b, const lsjk_pairlist** out);
+
+/* Boys functions F_n(T) = int_0^1 t^(2n) exp(-T t^2) dt for n = 0..nboys (nboys <= 16) at npoint
+ * arguments: F_dev (npoint x (nboys + 1), row-major) from T_dev, FP64.  Replaces the reference's
+ * Boys::compute / Boys::interpolate_F (src/lightspeed/boys.hpp:49-114, 170-181;
+ * src/cpp/quad/boys.cpp:9-49). */
+int lsjk_boys(int nboys, int npoint, const double* T_dev, double* F_dev);
+
 /* Coulomb gradient, bra-centre part (the reference's coulombGrad, intbox.hpp:954-1008 and
  * cpu/coulomb_grad.cpp:19-563, is the sum of this with the densities in both roles):
  *   G_dev (natom x 3) += sum_{ab,cd} Dbra_ab Dket_cd d/dR [bra centres a, b] (ab|O|cd),

@@ -39,6 +39,7 @@ PureTransform = _native.PureTransform
 IntBox = _native.IntBox
 DeviceArray = _native.DeviceArray
 JKBatch = _native.JKBatch
+Boys = _native.Boys
 
 hip_init = _native.hip_init
 hip_finalize = _native.hip_finalize
@@ -52,6 +53,7 @@ synchronize = _native.synchronize
 set_profiling = _native.set_profiling
 last_launches = _native.last_launches
 count_quartets = _native.count_quartets
+autotune = _native.autotune
 
 NATIVE_LIBRARY = _os.path.join(_os.path.dirname(__file__), "liblsjk.so")
 
@@ -96,6 +98,6 @@ Basis.basisdir = _loaders.BASISDIR
 
 __all__ = [
     "Molecule", "Atom", "Tensor", "Primitive", "Basis", "Ewald", "PairList", "PairListL",
-    "ResourceList", "PureTransform", "IntBox", "DeviceArray", "JKBatch", "hip_init", "hip_finalize",
+    "ResourceList", "PureTransform", "IntBox", "DeviceArray", "JKBatch", "Boys", "hip_init", "hip_finalize",
     "device_count",
 ]

@@ -34,7 +34,6 @@ struct lsjk_pairlist : public lsjk::PairList {};
 // device address once with hipGetSymbolAddress and every use goes through Session::scratch.
 __device__ __align__(256) unsigned char scratch[LSJK_NOALLOC_SCRATCH_BYTES];
 
-
 namespace {
 
 struct LsjkError : public std::runtime_error {
@@ -435,6 +434,7 @@ void build_pairlist(lsjk_pairlist* pl)
     pl->d_row_start = dev_upload(pl->h_row_start);
     pl->d_ebound = dev_upload(eb);
     pl->h_eb = eb;
+    pl->h_eprim = eprim;
     pl->d_eprim = dev_upload(eprim);
     pl->d_ecart = dev_upload(ecart);
     pl->d_eblk = dev_upload(eblk);
@@ -553,6 +553,7 @@ void drop_caches_for(lsjk_pairlist* pl, unsigned long long uid)
     erase_if_key(pl->kunits, by_uid, true);
     erase_if_key(pl->kcost, by_uid, false);
     erase_if_key(pl->kunits_g, by_uid, true);
+    pl->kunits_gen++;
     erase_if_key(pl->kcost_g, by_uid, false);
     erase_if_key(pl->korient, by_uid, false);
 }
@@ -569,6 +570,13 @@ constexpr int kJTasksTarget = 16384;
 // takes shorter chunks, down to nket / 16 kets: each task adds its tile's bra potentials with
 // atomics, so more tasks per tile would contend on the same addresses.  For total angular
 // momentum L >= 6 a quartet costs so much more than the atomics that the floor is nket / 64.
+// A ket chunk scaled by an autotuned factor (16 .. 4096 kets).
+long long scaled_chunk(long long c, float scale)
+{
+    if (scale == 1.0f) return c;
+    return std::min<long long>(4096, std::max<long long>(16, (long long)std::llround((double)c * scale)));
+}
+
 long long j_chunk(long long nket, long long ntile, int L)
 {
     long long c = (nket * ntile + kJTasksTarget - 1) / kJTasksTarget;
@@ -749,13 +757,16 @@ const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp)
     G.d_slot = dev_upload(slot);
     G.d_grp_slot = dev_upload(G.grp_slot);
     G.d_grp_rows = dev_upload(grp_rows);
+    G.slot = slot;
+    G.grp_rows = grp_rows;
     G.d_grp_rank = dev_upload(G.grp_rank);
     G.d_chunk_b = dev_upload(G.chunk_b);
     return pl->kgroups.emplace(key, std::move(G)).first->second;
 }
 
 // Work units per K class below which the groups' chunk ranges are split.
-constexpr long long kKUnitsTarget = 12288;
+constexpr long long kKUnitsTargetDefault = 12288;
+constexpr int kcount_index_host(int L1, int L2, int L3, int L4) { return 256 + ((L1 * 5 + L2) * 5 + L3) * 5 + L4; }
 // thresp values whose K unit lists are cached per pair of lists (older ones are dropped)
 constexpr int kKThrespCached = 4;
 
@@ -789,11 +800,18 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
                 auto same = [&](const std::pair<unsigned long long, long long>& k) { return k.first == pl34->uid; };
                 erase_if_key(pl12->kunits_g, same, true);
                 erase_if_key(pl12->kcost_g, same, false);
+                pl12->kunits_gen++;
             }
         }
         const auto& rows1 = pl12->h_rows[L1][L2];
         const auto& G34 = get_kgroups(pl34, L3, L4);
         const auto& G12 = get_kgroups(pl12, L1, L2);
+        // the class's unit-split target: autotuned (this list on both sides) or the default
+        long long kKUnitsTarget = kKUnitsTargetDefault;
+        if (pl12 == pl34) {
+            auto ks = pl12->ksplit.find(kcount_index_host(L1, L2, L3, L4));
+            if (ks != pl12->ksplit.end()) kKUnitsTarget = ks->second;
+        }
         const auto& mrank = get_mrank(pl12);
         const int m12 = pl12->kmaxL2 + 1;
         // chunk bounds per group, and the smallest rank of each group (triangle mode)
@@ -979,6 +997,7 @@ void append_batch_jtasks(const lsjk_pairlist* pl, int L12, int L34, bool sym, st
         work += (double)((hb[m + 1] - hb[m] + 63) / 64) * (double)(hk[m + 1] - hk[m]);
     long long chunk = (long long)std::ceil(work / kJTasksTarget);
     chunk = std::min<long long>(kJChunkMax, std::max<long long>(64, (chunk + 63) / 64 * 64));
+    chunk = scaled_chunk(chunk, pl->jscale[sym ? 1 : 0][L12][L34]);
     for (int m = 0; m < pl->nmol; m++) {
         const int b0 = hb[m], b1 = hb[m + 1], k0 = hk[m], k1 = hk[m + 1];
         if (b1 <= b0 || k1 <= k0) continue;
@@ -1010,7 +1029,8 @@ const lsjk::JTaskList* get_jtasks(const lsjk_pairlist* pl12c, const lsjk_pairlis
                 append_batch_jtasks(pl12, L12, L34, false, tasks);
             } else {
                 const long long ntile = (b1 - b0 + 63) / 64;
-                const long long chunk = j_chunk(k1 - k0, ntile, L12 + L34);
+                const long long chunk = scaled_chunk(j_chunk(k1 - k0, ntile, L12 + L34),
+                                                     pl12 == pl34 ? pl12->jscale[0][L12][L34] : 1.0f);
                 for (int b = b0; b < b1; b += 64)
                     for (int k = k0; k < k1; k += (int)chunk)
                         tasks.push_back({b, std::min(64, b1 - b), k, std::min(k + (int)chunk, k1)});
@@ -1040,7 +1060,8 @@ void build_jtasks_sym(lsjk_pairlist* pl)
                 continue;
             }
             const long long ntile = (b1 - b0 + 63) / 64;
-            const long long chunk = j_chunk((k1 - k0) / (L12 == L34 ? 2 : 1), ntile, L12 + L34);
+            const long long chunk = scaled_chunk(j_chunk((k1 - k0) / (L12 == L34 ? 2 : 1), ntile, L12 + L34),
+                                                 pl->jscale[1][L12][L34]);
             for (int b = b0; b < b1; b += 64) {
                 const int kend = (L12 == L34) ? std::min(k1, b + 64) : k1;
                 for (int k = k0; k < kend; k += (int)chunk)
@@ -1144,6 +1165,209 @@ int lsjk::persistent_grid(const void* kernel, int block, size_t lds, long long n
     return (int)std::max<long long>(1, std::min<long long>(g, nwork_groups));
 }
 
+// ------------------------------------------------------------------ learned screen
+//
+// A screen holds the trained network (lsjk_mlp_* layout), the normalisation of its inputs, and per
+// K class the certified margin delta (log scale; NaN: the class is not certified).  For a work
+// unit u of a certified class the factor tau_u = min(1, exp(f(x_u) + delta)) bounds, for every
+// quartet of the unit with Schwarz product >= smin, the ratio of its largest integral component to
+// its Schwarz product.  The factors are computed once per unit list and kept on the screen.
+namespace {
+struct LearnedScreen {
+    int ni = 0;
+    float* d_params = nullptr;
+    std::vector<float> mean, scale;
+    int nbin = 1;
+    std::vector<float> fedge;    // 625 x (nbin - 1): bin edges of the network output per class
+    std::vector<double> delta;   // 625 x nbin: margin per class and bin (NaN: not certified)
+    float smin = 0.0f;
+    std::map<std::tuple<unsigned long long, unsigned long long, long long>, std::pair<float*, int>> tau;
+    std::map<std::tuple<unsigned long long, unsigned long long, long long>, unsigned long long> tau_gen;
+};
+}  // namespace
+
+struct lsjk_screen : public LearnedScreen {};
+
+namespace {
+
+// Features of the K work units of one class, LSJK_SCREEN_NFEAT per unit, from the lists alone:
+//   L1, L2, L3, L4;
+//   ln(exponent of the bra row), ln(smallest exponent among the bra row's partners of the unit),
+//   ln(smallest exponent among the ket group's rows), ln(smallest exponent among the unit's ket
+//   partner entries);
+//   |bra row - centroid of the ket group's rows|, the group's radius about its centroid,
+//   largest bra pair extent |A - B| among the unit's bra partners, largest ket pair extent among the
+//   unit's slots (bohr);
+//   ln(largest bra bound of the unit), ln(largest ket bound of the unit's first chunk);
+//   ln of the (ss|ss) ratio to its Schwarz product for the unit's most diffuse pairs at the closest
+//   approach (centroid distance less the group radius and both pair extents) and at the centroid
+//   distance.
+// sqrt(2) (pq)^(1/4) (p + q)^(-1/2) F0(rho R^2), rho = pq / (p + q): (ss|ss) over its Schwarz product
+// for pair exponents p, q at centre distance R (F0(T) = sqrt(pi / T) erf(sqrt T) / 2).
+double ss_ratio(double p, double q, double R)
+{
+    const double rho = p * q / (p + q), T = rho * R * R;
+    const double F0 = T < 1e-12 ? 1.0 - T / 3.0 : 0.5 * std::sqrt(M_PI / T) * std::erf(std::sqrt(T));
+    return std::sqrt(2.0) * std::pow(p * q, 0.25) / std::sqrt(p + q) * F0;
+}
+
+void unit_features(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34c, int L1, int L2, int L3, int L4,
+                   const std::vector<lsjk::KUnitG>& units, std::vector<float>& out)
+{
+    constexpr int NF = LSJK_SCREEN_NFEAT;
+    lsjk_pairlist* pl34 = const_cast<lsjk_pairlist*>(pl34c);
+    const auto& G = get_kgroups(pl34, L3, L4);
+    const int gsize = lsjk::kgroup_size(L3, L4);
+    const lsjk::Basis *b1 = pl12->b1, *b2 = pl12->b2, *b3 = pl34->b1, *b4 = pl34->b2;
+    const auto& rows1 = pl12->h_rows[L1][L2];
+    const int m12 = pl12->kmaxL2 + 1;
+    auto dist = [](const double* a, const double* b) {
+        return std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+    };
+    // per group: centroid, radius, smallest row exponent
+    std::vector<double> gc((size_t)G.ngroup * 3, 0.0), grad(G.ngroup, 0.0), gemin(G.ngroup, 1e300);
+    for (int g = 0; g < G.ngroup; g++) {
+        int n = 0;
+        for (int t = 0; t < gsize; t++) {
+            const int r = G.grp_rows[(size_t)g * gsize + t];
+            if (r < 0) continue;
+            for (int d = 0; d < 3; d++) gc[3 * g + d] += b3->h_xyz[3 * r + d];
+            gemin[g] = std::min(gemin[g], b3->h_e[r]);
+            n++;
+        }
+        for (int d = 0; d < 3; d++) gc[3 * g + d] /= std::max(n, 1);
+        for (int t = 0; t < gsize; t++) {
+            const int r = G.grp_rows[(size_t)g * gsize + t];
+            if (r >= 0) grad[g] = std::max(grad[g], dist(&b3->h_xyz[3 * r], &gc[3 * g]));
+        }
+    }
+    // per 64-slot chunk: smallest partner exponent, largest pair extent
+    const size_t nchunk = G.slot.size() / 64;
+    std::vector<double> cemin(nchunk, 1e300), cext(nchunk, 0.0);
+    for (size_t c = 0; c < nchunk; c++) {
+        const int g = (int)(std::upper_bound(G.grp_slot.begin(), G.grp_slot.end(), (int)(c * 64)) - G.grp_slot.begin()) - 1;
+        for (int l = 0; l < 64; l++) {
+            const int2 sl = G.slot[c * 64 + l];
+            if (sl.x < 0) continue;
+            const int r3 = G.grp_rows[(size_t)g * gsize + sl.y];
+            const int p4 = pl34->h_eprim[sl.x];
+            cemin[c] = std::min(cemin[c], b4->h_e[p4]);
+            cext[c] = std::max(cext[c], dist(&b3->h_xyz[3 * r3], &b4->h_xyz[3 * p4]));
+        }
+    }
+    out.assign(units.size() * NF, 0.0f);
+    for (size_t q = 0; q < units.size(); q++) {
+        const lsjk::KUnitG& u = units[q];
+        const int r1 = rows1[u.i1];
+        int a0 = pl12->h_row_start[(size_t)r1 * m12 + L2], a1 = pl12->h_row_start[(size_t)r1 * m12 + L2 + 1];
+        if (u.prange != 0) {
+            a0 = std::min(a1, a0 + (u.prange & 0xffff));
+            if (u.prange >> 16) a1 = std::min(a1, a0 + (u.prange >> 16));
+        }
+        const double* A = &b1->h_xyz[3 * r1];
+        double emin2 = 1e300, ext12 = 0.0;
+        for (int k = a0; k < a1; k++) {
+            const int p2 = pl12->h_eprim[k];
+            emin2 = std::min(emin2, b2->h_e[p2]);
+            ext12 = std::max(ext12, dist(A, &b2->h_xyz[3 * p2]));
+        }
+        const size_t c0 = (size_t)G.grp_slot[u.g] / 64 + (u.crange & 0xffff);
+        const size_t c1 = std::min((size_t)G.grp_slot[u.g + 1] / 64, c0 + (size_t)(u.crange >> 16));
+        double emin4 = 1e300, ext34 = 0.0;
+        for (size_t c = c0; c < c1; c++) {
+            emin4 = std::min(emin4, cemin[c]);
+            ext34 = std::max(ext34, cext[c]);
+        }
+        auto lg = [](double v) { return (float)std::log(std::max(v, 1e-300)); };
+        float* f = &out[q * NF];
+        f[0] = (float)L1;
+        f[1] = (float)L2;
+        f[2] = (float)L3;
+        f[3] = (float)L4;
+        f[4] = lg(b1->h_e[r1]);
+        f[5] = lg(a1 > a0 ? emin2 : b1->h_e[r1]);
+        f[6] = lg(gemin[u.g]);
+        f[7] = lg(c1 > c0 ? emin4 : gemin[u.g]);
+        f[8] = (float)dist(A, &gc[3 * u.g]);
+        f[9] = (float)grad[u.g];
+        f[10] = (float)ext12;
+        f[11] = (float)ext34;
+        f[12] = lg(a1 > a0 ? (double)pl12->h_eb[a0] : 1e-300);
+        f[13] = lg(c0 < nchunk ? (double)G.chunk_b[c0] : 1e-300);
+        // s-type baseline: the exact (ss|ss) ratio to its Schwarz product,
+        // sqrt(2) (pq)^(1/4) (p + q)^(-1/2) F0(rho R^2), for the unit's most diffuse pairs at the
+        // closest approach the geometry allows and at the centroid distance
+        const double p = b1->h_e[r1] + (a1 > a0 ? emin2 : b1->h_e[r1]);
+        const double qq = gemin[u.g] + (c1 > c0 ? emin4 : gemin[u.g]);
+        const double R = f[8], Rmin = std::max(0.0, R - f[9] - f[10] - f[11]);
+        f[14] = lg(ss_ratio(p, qq, Rmin));
+        f[15] = lg(ss_ratio(p, qq, R));
+    }
+}
+
+std::vector<lsjk::KUnitG> download_units(const lsjk::KUnitG* d, int n)
+{
+    std::vector<lsjk::KUnitG> h((size_t)std::max(n, 0));
+    if (n > 0) HIPCHK(hipMemcpy(h.data(), d, (size_t)n * sizeof(lsjk::KUnitG), hipMemcpyDeviceToHost));
+    return h;
+}
+
+// The screen's factors for the units of one class (computed and cached at first use): network
+// over the normalised unit features, tau = min(1, exp(f + delta)).
+const float* screen_tau(lsjk_screen* sc, const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, int L1, int L2,
+                        int L3, int L4, bool tri, float thresp, const lsjk::KUnitG* units, int nunits)
+{
+    unsigned int tb;
+    std::memcpy(&tb, &thresp, sizeof(tb));
+    const long long cls = ((long long)tb << 8) | (((((L1 * 3 + L2) * 3 + L3) * 3 + L4) << 1) | (tri ? 1 : 0));
+    const auto key = std::make_tuple(pl12->uid, pl34->uid, cls);
+    auto it = sc->tau.find(key);
+    if (it != sc->tau.end() && it->second.second == nunits && sc->tau_gen[key] == pl12->kunits_gen)
+        return it->second.first;
+    const int ci = ((L1 * 5 + L2) * 5 + L3) * 5 + L4;
+    std::vector<float> feat;
+    unit_features(pl12, pl34, L1, L2, L3, L4, download_units(units, nunits), feat);
+    constexpr int NF = LSJK_SCREEN_NFEAT;
+    std::vector<float> x((size_t)nunits * sc->ni, 0.0f);
+    for (int u = 0; u < nunits; u++)
+        for (int k = 0; k < sc->ni && k < NF; k++)
+            x[(size_t)u * sc->ni + k] = (feat[(size_t)u * NF + k] - sc->mean[k]) / sc->scale[k];
+    float* dx = dev_upload(x);
+    float* df = nullptr;
+    std::vector<float> f(nunits);
+    try {
+        HIPCHK(hipMalloc((void**)&df, std::max(nunits, 1) * sizeof(float)));
+        lsjk::launch_mlp_forward(sc->d_params, sc->ni, nunits, dx, df, S.stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(S.stream));
+        if (nunits) HIPCHK(hipMemcpy(f.data(), df, nunits * sizeof(float), hipMemcpyDeviceToHost));
+    } catch (...) {
+        dev_free(dx);
+        dev_free(df);
+        throw;
+    }
+    dev_free(dx);
+    dev_free(df);
+    std::vector<float> tau(nunits);
+    for (int u = 0; u < nunits; u++) {
+        // the margin of the output's bin (edges ascending); an uncertified bin keeps tau = 1
+        int b = 0;
+        while (b + 1 < sc->nbin && f[u] >= sc->fedge[(size_t)ci * (sc->nbin - 1) + b]) b++;
+        const double d = sc->delta[(size_t)ci * sc->nbin + b];
+        tau[u] = (d == d) ? (float)std::min(1.0, std::max(1e-30, std::exp((double)f[u] + d))) : 1.0f;
+    }
+    if (it != sc->tau.end()) {
+        dev_free(it->second.first);
+        sc->tau.erase(it);
+    }
+    float* d = dev_upload(tau);
+    sc->tau[key] = {d, nunits};
+    sc->tau_gen[key] = pl12->kunits_gen;
+    return d;
+}
+
+}  // namespace
+
 // How a build call runs.  ws: a fixed workspace of ws_bytes for the per-call buffers (nullptr:
 // the session workspace, grown as needed).  noalloc: allocate no device memory (the J task list
 // must exist already; lsjk_coulomb_noalloc).  graph: only enqueue the launches on the library
@@ -1155,6 +1379,8 @@ struct CallOpts {
     bool noalloc = false;
     bool graph = false;
     const char* who = nullptr;
+    lsjk_screen* screen = nullptr;  // exchange: the learned route (certified classes)
+    bool tune_orient = false;       // exchange: time both orientations of each mirror-class pair
 };
 
 void commit_ws(Workspace& wsp, const CallOpts& o, const char* who)
@@ -1432,9 +1658,12 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
         unsigned tb;
         std::memcpy(&tb, &thresp, sizeof(tb));
         // keyed by route as well: the two routes' kernels differ in speed
-        const auto okey = std::make_pair(pl34->uid, ((unsigned long long)tb << 1) | (mixed ? 1ull : 0ull));
+        const auto okey = std::make_pair(pl34->uid, ((unsigned long long)tb << 2) | (mixed ? 1ull : 0ull) |
+                                                        (o.screen ? 2ull : 0ull));
         auto& ocache = const_cast<lsjk_pairlist*>(pl12)->korient;
-        const bool tune = symmode && ew.n > 0 && !ocache.count(okey) && !o.graph && !o.noalloc;
+        // the mirror-class orientations are timed only when the autotuner asks (o.tune_orient); a
+        // first call otherwise takes the deterministic cost-model choice below
+        const bool tune = symmode && ew.n > 0 && o.tune_orient && !o.graph && !o.noalloc;
         std::map<int, int> tuned;  // filled by the tuning below, stored only once it is complete
         const std::map<int, int>* orient = nullptr;
         if (symmode && !tune) {
@@ -1467,6 +1696,7 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
                 double cost;
                 KArgs ka;
                 KGroups kg;
+                int route;
             };
             // The launch of one class: tri = the triangle rule (row1 rank <= row3 rank, equal row
             // momenta of one list); K = the output.  False when the class has no work.
@@ -1521,15 +1751,54 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
                 const double Pi = (double)ncart(L1) * ncart(L2) * ncart(L3) * ncart(L4);
                 const double G = (double)(L1 + L2 + 1) * (L3 + L4 + 1);
                 const double F = 40.0 + (Lt / 2 + 1) * (30.0 + 9.0 * G + 3.0 * Pi) + 2.0 * Pi;
-                out = KL{L1, L2, L3, L4, units.first, units.second, cost * F, ka, kg};
+                int route = mixed ? kRouteMixed : kRoutePlain;
+                if (o.screen) {
+                    // learned route: certified classes (s and p functions) take their units'
+                    // factors; every other quartet is evaluated in FP64 under the plain test
+                    const int ci = ((L1 * 5 + L2) * 5 + L3) * 5 + L4;
+                    bool cert = false;
+                    for (int b = 0; b < o.screen->nbin; b++) {
+                        const double dl = o.screen->delta[(size_t)ci * o.screen->nbin + b];
+                        cert = cert || dl == dl;
+                    }
+                    if (cert && L1 <= 1 && L2 <= 1 && L3 <= 1 && L4 <= 1) {
+                        ka.tau = screen_tau(o.screen, pl12, pl34, L1, L2, L3, L4, tri, thresp, units.first,
+                                            units.second);
+                        ka.tau_smin = o.screen->smin;
+                    } else {
+                        route = kRoutePlain;
+                    }
+                }
+                out = KL{L1, L2, L3, L4, units.first, units.second, cost * F, ka, kg, route};
                 return true;
             };
             // Symmetric mode (K = K^T): a class and its mirror (L3 L4 | L1 L2) hold the same quartets
             // with bra and ket swapped, so one of them, over all row pairs and with the transposed
-            // block, gives both.  Which one runs faster depends on the class shapes and the lists;
-            // at first use both are timed once (on the library stream, into a scratch K) and the
-            // faster one is kept for this pair of lists.  A class that is its own mirror takes the
-            // row triangle instead.
+            // block, gives both.  Which one runs faster depends on the class shapes and the lists.
+            // By default the orientation with fewer estimated wave steps is kept (the unit lists'
+            // cost: (bra partner, ket chunk) steps that pass the bound test with |D| <= 1), a
+            // deterministic choice; the autotuner times both (twice each, on the library stream,
+            // into a scratch K) and keeps the faster.  A class that is its own mirror takes the row
+            // triangle instead.
+            if (symmode && !tune && !orient && ew.n > 0 && !o.graph) {
+                std::map<int, int> pick;
+                for (int L1 = 0; L1 <= b1->maxL; L1++)
+                    for (int L2 = 0; L2 <= b2->maxL; L2++)
+                        for (int L3 = 0; L3 <= b3->maxL; L3++)
+                            for (int L4 = 0; L4 <= b4->maxL; L4++) {
+                                const int c = kcount_index(L1, L2, L3, L4), m = kcount_index(L3, L4, L1, L2);
+                                if (c >= m) continue;
+                                auto steps = [&](int A, int B, int C, int D_) {
+                                    if (pl12->h_rows[A][B].empty() || pl34->h_rows[C][D_].empty()) return 0.0;
+                                    double cost = 0.0;
+                                    get_kunits_g(pl12, pl34, A, B, C, D_, false, thresp, &cost);
+                                    return cost;
+                                };
+                                const double sc = steps(L1, L2, L3, L4), sm = steps(L3, L4, L1, L2);
+                                pick[c] = pick[m] = (sc <= sm) ? c : m;
+                            }
+                orient = &(ocache[okey] = pick);
+            }
             if (tune) {
                 // events and the counting flag are restored whatever happens in between
                 struct Guard {
@@ -1550,7 +1819,7 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
                     if (!make_kl(L1, L2, L3, L4, false, Ks, k)) return 0.0f;
                     k.ka.count = 0;
                     HIPCHK(hipEventRecord(g.e0, st));
-                    launch_k_ket(L1, L2, L3, L4, mixed ? kRouteMixed : kRoutePlain, k.ka, k.kg, k.units, k.nunits, st);
+                    launch_k_ket(L1, L2, L3, L4, k.route, k.ka, k.kg, k.units, k.nunits, st);
                     HIPCHK(hipEventRecord(g.e1, st));
                     HIPCHK(hipEventSynchronize(g.e1));
                     float ms = 0.0f;
@@ -1602,7 +1871,7 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
                 const int idx = kcount_index(k.L1, k.L2, k.L3, k.L4);
                 hipStream_t ss = fo.pick();
                 lt.begin(idx, ss);
-                launch_k_ket(k.L1, k.L2, k.L3, k.L4, mixed ? kRouteMixed : kRoutePlain, k.ka, k.kg, k.units, k.nunits, ss);
+                launch_k_ket(k.L1, k.L2, k.L3, k.L4, k.route, k.ka, k.kg, k.units, k.nunits, ss);
                 lt.end();
             }
             fo.end();
@@ -1614,6 +1883,136 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
             finish_call(S.counting, S.d_sc);
         }
 }
+
+// ------------------------------------------------------------------ autotuner
+//
+// Launch parameters that the kernels take at run time, timed on a list on both sides with a given
+// density and route: per J group pair the ket chunk of a task (scale of the default chunk: 1/4 ..
+// 4), per K class the work-unit split target (how finely few-unit classes are cut: 3072 .. 49152
+// units).  Class launches are serialised on one stream and timed with HIP events (lsjk's per-launch
+// profiling), each candidate twice (the faster counts); the fastest candidate is kept per group
+// pair / class.  Group size, waves per workgroup and register budgets are compile-time template
+// parameters of the kernels and are not searched here.
+namespace {
+constexpr float kJScales[] = {0.25f, 0.5f, 1.0f, 2.0f, 4.0f};
+constexpr long long kKSplits[] = {3072, 12288, 49152};
+
+void rebuild_jtasks(lsjk_pairlist* pl)
+{
+    auto it = pl->jtasks.find(pl->uid);
+    if (it != pl->jtasks.end()) {
+        dev_free(it->second.d_tasks);
+        pl->jtasks.erase(it);
+    }
+    dev_free(pl->jtasks_sym.d_tasks);
+    pl->jtasks_sym = lsjk::JTaskList();
+    get_jtasks(pl, pl);
+    build_jtasks_sym(pl);
+}
+
+void drop_own_kunits(lsjk_pairlist* pl)
+{
+    auto self = [&](const std::pair<unsigned long long, long long>& k) { return k.first == pl->uid; };
+    erase_if_key(pl->kunits_g, self, true);
+    erase_if_key(pl->kcost_g, self, false);
+    pl->kunits_gen++;
+}
+
+void autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const double* scales, const double* omegas,
+              float thresp, float thredp)
+{
+    using namespace lsjk;
+    if (!pl->symmetric || pl->b1 != pl->b2) throw LsjkError(LSJK_E_INVALID, "lsjk_autotune: a symmetric list over one basis");
+    const size_t n = mat_elems(pl->b1, pl->b1);
+    double *dJ = nullptr, *dK = nullptr;
+    struct Restore {
+        int nside = S.nside;
+        bool profiling = S.profiling, counting = S.counting;
+        double** a;
+        double** b;
+        ~Restore()
+        {
+            S.nside = nside;
+            S.profiling = profiling;
+            S.counting = counting;
+            dev_free(*a);
+            dev_free(*b);
+        }
+    } restore{S.nside, S.profiling, S.counting, &dJ, &dK};
+    HIPCHK(hipMalloc((void**)&dJ, n * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&dK, n * sizeof(double)));
+    S.counting = false;
+    // mirror-class orientations: timed (both, twice each) and stored for this route; then warm
+    // ket groups, task and unit lists of the defaults
+    {
+        auto self = [&](const std::pair<unsigned long long, unsigned long long>& k) { return k.first == pl->uid; };
+        erase_if_key(pl->korient, self, false);
+        CallOpts ot;
+        ot.tune_orient = true;
+        exchange_call(pl, pl, D_dev, 1, nterm, scales, omegas, thresp, thredp, dK, ot);
+    }
+    coulomb_call(pl, pl, D_dev, nterm, scales, omegas, thresp, thredp, dJ);
+    exchange_call(pl, pl, D_dev, 1, nterm, scales, omegas, thresp, thredp, dK);
+    S.nside = 1;
+    S.profiling = true;
+    auto launches = [&](bool J) {
+        std::map<int, double> ms;
+        for (int rep = 0; rep < 2; rep++) {
+            if (J)
+                coulomb_call(pl, pl, D_dev, nterm, scales, omegas, thresp, thredp, dJ);
+            else
+                exchange_call(pl, pl, D_dev, 1, nterm, scales, omegas, thresp, thredp, dK);
+            for (size_t i = 0; i < S.kid.size(); i++) {
+                auto it = ms.find(S.kid[i]);
+                ms[S.kid[i]] = it == ms.end() ? S.kms[i] : std::min(it->second, S.kms[i]);
+            }
+        }
+        return ms;
+    };
+    pl->tuned.clear();
+    // J: one candidate scale for every group pair at a time
+    std::map<int, std::map<float, double>> jt;  // launch id -> scale -> ms
+    for (float c : kJScales) {
+        for (int q = 0; q < 2; q++)
+            for (int a = 0; a < 5; a++)
+                for (int b = 0; b < 5; b++) pl->jscale[q][a][b] = c;
+        rebuild_jtasks(pl);
+        for (auto& kv : launches(true)) jt[kv.first][c] = kv.second;
+    }
+    for (int q = 0; q < 2; q++)
+        for (int a = 0; a < 5; a++)
+            for (int b = 0; b < 5; b++) pl->jscale[q][a][b] = 1.0f;
+    for (auto& kv : jt) {
+        const int id = kv.first;
+        const bool sym = (id >> 24) & 1;
+        const int idx = id & 0xfff, L12 = idx / 5, L34 = idx % 5;
+        if (!kv.second.count(1.0f)) continue;
+        float best = 1.0f;
+        for (auto& c : kv.second)
+            if (c.second < kv.second[best]) best = c.first;
+        pl->jscale[sym ? 1 : 0][L12][L34] = best;
+        pl->tuned.push_back({sym ? 1 : 0, idx, (double)best, kv.second[1.0f], kv.second[best]});
+    }
+    rebuild_jtasks(pl);
+    // K: one candidate split target for every class at a time
+    std::map<int, std::map<long long, double>> kt;  // class counter index -> target -> ms
+    for (long long c : kKSplits) {
+        for (int i = 256; i < 256 + 625; i++) pl->ksplit[i] = c;
+        drop_own_kunits(pl);
+        for (auto& kv : launches(false)) kt[kv.first][c] = kv.second;
+    }
+    pl->ksplit.clear();
+    for (auto& kv : kt) {
+        if (!kv.second.count(kKUnitsTargetDefault)) continue;
+        long long best = kKUnitsTargetDefault;
+        for (auto& c : kv.second)
+            if (c.second < kv.second[best]) best = c.first;
+        if (best != kKUnitsTargetDefault) pl->ksplit[kv.first] = best;
+        pl->tuned.push_back({2, kv.first, (double)best, kv.second[kKUnitsTargetDefault], kv.second[best]});
+    }
+    drop_own_kunits(pl);
+}
+}  // namespace
 
 namespace {
 // A device pair list (lsjk_pairlist_create; also the combined list of a batch, nmol > 0 with the
@@ -1634,6 +2033,9 @@ lsjk_pairlist* pairlist_new(const lsjk_basis* b1, const lsjk_basis* b2, int symm
     lsjk_pairlist* pl = new lsjk_pairlist();
     static unsigned long long next_uid = 0;
     pl->uid = ++next_uid;
+    for (int q = 0; q < 2; q++)
+        for (int a = 0; a < 5; a++)
+            for (int b = 0; b < 5; b++) pl->jscale[q][a][b] = 1.0f;
     try {
         pl->b1 = b1;
         pl->b2 = b2;
@@ -2119,12 +2521,238 @@ int lsjk_coulomb_noalloc(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
     });
 }
 
-
 int lsjk_exchange(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
                   int nterm, const double* scales, const double* omegas, float thresp, float thredp, double* K13_dev)
 {
     return guarded([&] {
         exchange_call(pl12, pl34, D24_dev, D24symm, nterm, scales, omegas, thresp, thredp, K13_dev);
+    });
+}
+
+int lsjk_unit_features(const lsjk_pairlist* pl, int L1, int L2, int L3, int L4, float thresp, int* nunits, float* feat)
+{
+    return guarded([&] {
+        require_init();
+        if (!pl || !nunits || L1 < 0 || L2 < 0 || L3 < 0 || L4 < 0 || L1 > 2 || L2 > 2 || L3 > 2 || L4 > 2)
+            throw LsjkError(LSJK_E_INVALID, "lsjk_unit_features: bad argument");
+        *nunits = 0;
+        if (pl->h_rows[L1][L2].empty() || pl->h_rows[L3][L4].empty()) return;
+        const auto units = get_kunits_g(pl, pl, L1, L2, L3, L4, false, thresp, nullptr);
+        *nunits = units.second;
+        if (!feat) return;
+        std::vector<float> f;
+        unit_features(pl, pl, L1, L2, L3, L4, download_units(units.first, units.second), f);
+        std::memcpy(feat, f.data(), f.size() * sizeof(float));
+    });
+}
+
+int lsjk_unit_magnitudes(const lsjk_pairlist* pl, int L1, int L2, int L3, int L4, float thresp, float smin, float* mag)
+{
+    return guarded([&] {
+        using namespace lsjk;
+        require_init();
+        if (!pl || !mag || L1 < 0 || L2 < 0 || L3 < 0 || L4 < 0 || L1 > 1 || L2 > 1 || L3 > 1 || L4 > 1)
+            throw LsjkError(LSJK_E_INVALID, "lsjk_unit_magnitudes: classes with s and p functions only");
+        if (pl->h_rows[L1][L2].empty() || pl->h_rows[L3][L4].empty()) return;
+        const auto units = get_kunits_g(pl, pl, L1, L2, L3, L4, false, thresp, nullptr);
+        if (units.second == 0) return;
+        const KGroups kg = kgroups_args(get_kgroups(const_cast<lsjk_pairlist*>(pl), L3, L4));
+        KArgs ka{};
+        ka.rys = S.rys;
+        ka.ew.n = 1;
+        ka.ew.plain = 1;
+        ka.ew.alpha[0] = 1.0;
+        ka.ew.omega[0] = -1.0;
+        KSide* ks[2] = {&ka.s12, &ka.s34};
+        const int Ls[2] = {L1, L3}, Lps[2] = {L2, L4};
+        for (int q = 0; q < 2; q++) {
+            KSide& sd = *ks[q];
+            sd.row_start = pl->d_row_start;
+            sd.maxLp = pl->kmaxL2;
+            sd.eb = pl->d_ebound;
+            sd.ecart = pl->d_ecart;
+            sd.eblk = pl->d_eblk;
+            sd.eg = pl->d_egeom;
+            sd.eg32 = pl->d_egeom32;
+            sd.nent = pl->nent;
+            sd.rows = pl->d_rows[Ls[q]][Lps[q]];
+            sd.nrows = (int)pl->h_rows[Ls[q]][Lps[q]].size();
+            sd.x = pl->b1->d_x;
+            sd.y = pl->b1->d_y;
+            sd.z = pl->b1->d_z;
+            sd.cart = pl->b1->d_cart;
+        }
+        float* d = nullptr;
+        HIPCHK(hipMalloc((void**)&d, (size_t)units.second * sizeof(float)));
+        try {
+            launch_k_unit_mag_sp(L1, L2, L3, L4, ka, kg, units.first, units.second, smin, d, S.stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(S.stream));
+            HIPCHK(hipMemcpy(mag, d, (size_t)units.second * sizeof(float), hipMemcpyDeviceToHost));
+        } catch (...) {
+            dev_free(d);
+            throw;
+        }
+        dev_free(d);
+    });
+}
+
+int lsjk_screen_create(int ni, const float* params, const float* mean, const float* scale, int nbin, const float* fedge,
+                       const double* delta, float smin, lsjk_screen** out)
+{
+    return guarded([&] {
+        require_init();
+        if (!out || !params || !mean || !scale || !delta || ni < 1 || ni > lsjk::kMlpMaxIn || ni > LSJK_SCREEN_NFEAT ||
+            nbin < 1 || nbin > 64 || (nbin > 1 && !fedge))
+            throw LsjkError(LSJK_E_INVALID, "lsjk_screen_create: bad argument");
+        for (int k = 0; k < ni; k++)
+            if (!(scale[k] > 0.0f)) throw LsjkError(LSJK_E_INVALID, "lsjk_screen_create: input scales must be positive");
+        lsjk_screen* sc = new lsjk_screen();
+        try {
+            sc->ni = ni;
+            std::vector<float> p(params, params + lsjk::mlp_nparam(ni));
+            sc->d_params = dev_upload(p);
+            sc->mean.assign(mean, mean + ni);
+            sc->scale.assign(scale, scale + ni);
+            sc->nbin = nbin;
+            sc->delta.assign(delta, delta + 625 * (size_t)nbin);
+            if (nbin > 1) sc->fedge.assign(fedge, fedge + 625 * (size_t)(nbin - 1));
+            sc->smin = smin;
+        } catch (...) {
+            dev_free(sc->d_params);
+            delete sc;
+            throw;
+        }
+        *out = sc;
+    });
+}
+
+int lsjk_screen_destroy(lsjk_screen* sc)
+{
+    return guarded([&] {
+        if (!sc) return;
+        for (auto& kv : sc->tau) dev_free(kv.second.first);
+        dev_free(sc->d_params);
+        delete sc;
+    });
+}
+
+int lsjk_exchange_screened(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D24_dev, int D24symm,
+                           int nterm, const double* scales, const double* omegas, float thresp, float thredp,
+                           lsjk_screen* screen, double* K13_dev)
+{
+    return guarded([&] {
+        if (!screen) throw LsjkError(LSJK_E_INVALID, "lsjk_exchange_screened: null screen");
+        CallOpts o;
+        o.screen = screen;
+        o.who = "lsjk_exchange_screened";
+        exchange_call(pl12, pl34, D24_dev, D24symm, nterm, scales, omegas, thresp, thredp, K13_dev, o);
+    });
+}
+
+int lsjk_boys(int nboys, int npoint, const double* T_dev, double* F_dev)
+{
+    return guarded([&] {
+        require_init();
+        if (nboys < 0 || nboys > lsjk::kBoysMax || npoint < 0 || (npoint > 0 && (!T_dev || !F_dev)))
+            throw LsjkError(LSJK_E_INVALID, "lsjk_boys: bad argument (nboys must be 0..16)");
+        lsjk::launch_boys(nboys, npoint, T_dev, F_dev, S.stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(S.stream));
+    });
+}
+
+int lsjk_autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const double* scales, const double* omegas,
+                  float thresp, float thredp)
+{
+    return guarded([&] {
+        require_init();
+        if (!pl || !D_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_autotune: null argument");
+        autotune(pl, D_dev, nterm, scales, omegas, thresp, thredp);
+    });
+}
+
+int lsjk_tuned_params(const lsjk_pairlist* pl, int capacity, int* n, int* kind, int* idx, double* value,
+                      double* ms_default, double* ms_tuned)
+{
+    return guarded([&] {
+        if (!pl || !n) throw LsjkError(LSJK_E_INVALID, "lsjk_tuned_params: null argument");
+        *n = (int)pl->tuned.size();
+        for (int i = 0; i < std::min(capacity, *n); i++) {
+            const auto& t = pl->tuned[i];
+            if (kind) kind[i] = t.kind;
+            if (idx) idx[i] = t.idx;
+            if (value) value[i] = t.value;
+            if (ms_default) ms_default[i] = t.ms_default;
+            if (ms_tuned) ms_tuned[i] = t.ms_tuned;
+        }
+    });
+}
+
+int lsjk_batch_autotune(lsjk_batch* b, const double* D_dev, int nterm, const double* scales, const double* omegas,
+                        float thresp, float thredp)
+{
+    return guarded([&] {
+        require_init();
+        if (!b || !D_dev) throw LsjkError(LSJK_E_INVALID, "lsjk_batch_autotune: null argument");
+        for (auto& g : b->graphs) batch_drop_graph(g);  // they hold the old task and unit lists
+        b->graphs.clear();
+        autotune(b->list, D_dev, nterm, scales, omegas, thresp, thredp);
+    });
+}
+
+int lsjk_batch_pairlist(const lsjk_batch* b, const lsjk_pairlist** out)
+{
+    return guarded([&] {
+        if (!b || !out) throw LsjkError(LSJK_E_INVALID, "lsjk_batch_pairlist: null argument");
+        *out = b->list;
+    });
+}
+
+int lsjk_mlp_nparam(int ni, int* nparam)
+{
+    return guarded([&] {
+        if (!nparam || ni < 1 || ni > lsjk::kMlpMaxIn) throw LsjkError(LSJK_E_INVALID, "lsjk_mlp_nparam: bad input width");
+        *nparam = lsjk::mlp_nparam(ni);
+    });
+}
+
+int lsjk_mlp_forward(int ni, const float* params_dev, int n, const float* X_dev, float* out_dev)
+{
+    return guarded([&] {
+        require_init();
+        if (ni < 1 || ni > lsjk::kMlpMaxIn || n < 0 || !params_dev || (n > 0 && (!X_dev || !out_dev)))
+            throw LsjkError(LSJK_E_INVALID, "lsjk_mlp_forward: bad argument");
+        lsjk::launch_mlp_forward(params_dev, ni, n, X_dev, out_dev, S.stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(S.stream));
+    });
+}
+
+int lsjk_mlp_grad(int ni, const float* params_dev, int nb, const int* idx_dev, const float* X_dev, const float* y_dev,
+                  const float* sw_dev, float wu, float* grad_dev, float* loss_dev)
+{
+    return guarded([&] {
+        require_init();
+        if (ni < 1 || ni > lsjk::kMlpMaxIn || nb < 0 || !params_dev || !grad_dev || !loss_dev ||
+            (nb > 0 && (!X_dev || !y_dev)))
+            throw LsjkError(LSJK_E_INVALID, "lsjk_mlp_grad: bad argument");
+        lsjk::launch_mlp_grad(params_dev, ni, nb, idx_dev, X_dev, y_dev, sw_dev, wu, grad_dev, loss_dev, S.stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(S.stream));
+    });
+}
+
+int lsjk_mlp_adam(int nparam, float* params_dev, const float* grad_dev, float* m_dev, float* v_dev, float lr,
+                  float beta1, float beta2, float eps, int t)
+{
+    return guarded([&] {
+        require_init();
+        if (nparam <= 0 || !params_dev || !grad_dev || !m_dev || !v_dev || t < 1)
+            throw LsjkError(LSJK_E_INVALID, "lsjk_mlp_adam: bad argument");
+        lsjk::launch_mlp_adam(nparam, params_dev, grad_dev, m_dev, v_dev, lr, beta1, beta2, eps, t, S.stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(S.stream));
     });
 }
 

@@ -681,9 +681,14 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
 
     // quartets with est at or above `thresp` here (and, on the FP32 route, below thredp) are evaluated
     const float thresp = (ROUTE == kRouteHi) ? a.thredp : a.thresp;
-    const double thr = (double)thresp;
-    const double thredp = (double)a.thredp;
     const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
+    // learned route: the unit's certified factor tau <= 1 (est tau bounds every quartet's
+    // contribution), so the thresholds scale by 1 / tau.  It holds for quartets whose Schwarz
+    // product is at least tau_smin; quartets below that fail the plain test whenever
+    // thresp >= tau_smin max|D|, else the factor is not applied.
+    const double tau = (a.tau && (double)thresp >= (double)a.tau_smin * dmax) ? (double)ld_const(a.tau + u) : 1.0;
+    const double thr = (double)thresp / tau;
+    const double thredp = (double)a.thredp / tau;
     const float thre12 = (float)(thr / ((double)a.maxb34 * dmax));
     const float thre34 = (float)(thr / ((double)a.maxb12 * dmax));
     const float b12top = ld_const(a.s12.eb + a0);
@@ -822,6 +827,73 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
             if (nq32) atomicAdd(&a.sc->counts32[kcount_index(L1, L2, L3, L4)], nq32);
         }
     }
+}
+
+// Training data of the learned screen: for each K work unit, the largest ratio M_q / (b12 b34)
+// over the unit's quartets q with Schwarz product b12 b34 >= smin, M_q = max over the Cartesian
+// components of |(ab|O|cd)| (the exact Rys evaluation of k_quartet, one component pair (b, d) at a
+// time against a unit D block).  The unit's quartets are those k_ket would visit without any
+// density screening: the bra row's partners (its walk piece) x the unit's slots (triangle rank
+// rule included).  One wave per unit.
+template <int L1, int L2, int L3, int L4>
+__global__ __launch_bounds__(64 * kKWaves) void k_unit_mag(KArgs a, KGroups kg, const KUnitG* __restrict__ units,
+                                                           int nunits, float smin, float* __restrict__ out)
+{
+    constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4);
+    constexpr int G = kgroup_size(L3, L4);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int u = blockIdx.x * kKWaves + w;
+    if (u >= nunits) return;
+    const int i1 = ld_const(&units[u].i1), g = ld_const(&units[u].g), rank1 = ld_const(&units[u].rank1);
+    const int crange = ld_const(&units[u].crange);
+    const int r1 = ld_const(a.s12.rows + i1);
+    int a0 = ld_const(a.s12.row_start + r1 * (a.s12.maxLp + 1) + L2);
+    int a1 = ld_const(a.s12.row_start + r1 * (a.s12.maxLp + 1) + L2 + 1);
+    const int prange = ld_const(&units[u].prange);
+    if (prange != 0) {
+        a0 = min(a1, a0 + (prange & 0xffff));
+        if (prange >> 16) a1 = min(a1, a0 + (prange >> 16));
+    }
+    const int s0 = ld_const(kg.grp_slot + g) + 64 * (crange & 0xffff);
+    const int s1 = min(ld_const(kg.grp_slot + g + 1), s0 + 64 * (crange >> 16));
+    float best = 0.0f;
+    for (int c0 = s0; c0 < s1; c0 += 64) {
+        const int2 sl = kg.slot[c0 + lane];
+        const int e4 = max(sl.x, 0);
+        const bool valid = sl.x >= 0 && kg.grp_rank[g * G + sl.y] >= rank1;
+        const double b34 = (double)a.s34.eb[e4];
+        const KEnt<double> E34 = load_ent(a.s34.eg, e4);
+        for (int k = a0; k < a1; k++) {
+            const double b12 = (double)ld_const(a.s12.eb + k);
+            const KEnt<double> E12 = load_ent_uniform(a.s12.eg, k);
+            if (valid && b12 * b34 >= (double)smin) {
+                double m = 0.0;
+                for (int b = 0; b < N2; b++)
+                    for (int d = 0; d < N4; d++) {
+                        double Db[N2][N4];
+#pragma unroll
+                        for (int bb = 0; bb < N2; bb++)
+#pragma unroll
+                            for (int dd = 0; dd < N4; dd++) Db[bb][dd] = (bb == b && dd == d) ? 1.0 : 0.0;
+                        double acc[N1][N3];
+#pragma unroll
+                        for (int i = 0; i < N1; i++)
+#pragma unroll
+                            for (int j = 0; j < N3; j++) acc[i][j] = 0.0;
+                        k_quartet<double, L1, L2, L3, L4, false>(a.rys, nullptr, a.ew, E12, E34, Db, acc);
+#pragma unroll
+                        for (int i = 0; i < N1; i++)
+#pragma unroll
+                            for (int j = 0; j < N3; j++) m = fmax(m, fabs(acc[i][j]));
+                    }
+                best = fmaxf(best, (float)(m / (b12 * b34)));
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
+    if (lane == 0) out[u] = best;
 }
 
 // Launch of one K build class at row angular momentum L1 (instantiated in exchange_l<L1>.hip).

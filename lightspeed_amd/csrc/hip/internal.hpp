@@ -107,6 +107,21 @@ struct PairList {
     int nmol = 0;
     std::vector<int> h_mol;
     std::vector<int> h_hmol[2 * kMaxL + 1];
+    // autotuned launch parameters of this list on both sides (lsjk_autotune): J ket-chunk scale per
+    // group pair of the one-way task list [0] and of the symmetric kernel's list [1] (1: the
+    // default chunk), K unit-split target per class counter index (absent: the default)
+    float jscale[2][5][5];
+    std::map<int, long long> ksplit;
+    // generation of the K unit lists (raised whenever cached unit lists are dropped), so that data
+    // cached per unit list elsewhere can tell a rebuilt list from the old one
+    unsigned long long kunits_gen = 0;
+    // the tuned choices, for reporting: kind (0 J one-way, 1 J symmetric, 2 K), counter index, value,
+    // ms with the default and with the choice
+    struct Tuned {
+        int kind, idx;
+        double value, ms_default, ms_tuned;
+    };
+    std::vector<Tuned> tuned;
     // process-unique id (never reused, unlike the object's address)
     unsigned long long uid = 0;
     // cached J task lists keyed by the ket list's uid
@@ -117,8 +132,9 @@ struct PairList {
     std::map<std::pair<unsigned long long, int>, std::pair<void*, int>> kunits;
     // cost of the class's units (sum over units of partner-count products), same key
     std::map<std::pair<unsigned long long, int>, double> kcost;
-    // host copy of the entry bounds (K view order)
+    // host copies of the entry bounds and partner primitives (K view order)
     std::vector<float> h_eb;
+    std::vector<int> h_eprim;
     // spatial (Morton) rank of each basis-1 primitive: the order of the ket groups, and the total
     // order of the triangle mode; built on first use
     std::vector<int> h_mrank;
@@ -135,6 +151,8 @@ struct PairList {
         std::vector<float> chunk_b;
         std::vector<int> order;     // indices into h_rows[L][Lp] in group order
         std::vector<int> mol_grp;   // batch list: groups of molecule m at [mol_grp[m], mol_grp[m + 1])
+        std::vector<int2> slot;     // host copies of the slots and group rows
+        std::vector<int> grp_rows;
     };
     std::map<int, KGroupData> kgroups;
     // cached K build unit lists (device pointer, count) keyed by (ket list uid, class and

@@ -215,6 +215,10 @@ struct KArgs {
     double* K;           // K13, ncart1 x ncart3
     int ldK;
     float maxb12, maxb34;  // max_bound of the two lists
+    // learned route (nullptr: none): per unit the certified factor tau in (0, 1] of the Schwarz
+    // estimate, applied while thresp >= tau_smin max|D24|
+    const float* tau;
+    float tau_smin;
     CallScalars* sc;
     // exchange gradient (kRouteGrad): G (natom x 3) += d/dR sum D13_ac D24_bd (ab|cd)
     const double* D13;
@@ -291,11 +295,33 @@ void launch_k_blockmax(const double* D, int ldD, const int* blk_cart2, const int
 // Exchange gradient classes (route kRouteGrad) over KUnit tiles.
 void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,
                        hipStream_t s);
+// Learned-screen training data: per K unit, the largest M_q / (b12 b34) over its quartets with
+// b12 b34 >= smin (classes with s and p functions only).
+void launch_k_unit_mag_sp(int L1, int L2, int L3, int L4, const KArgs& a, const KGroups& kg, const KUnitG* units,
+                          int nunits, float smin, float* out, hipStream_t s);
 // K build classes (routes plain, FP64 part, FP32 part) over ket-group units.
 void launch_k_ket(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KGroups& g, const KUnitG* units,
                   int nunits, hipStream_t s);
 void launch_fill(void* p, size_t bytes, hipStream_t s);
 
+// Boys functions F_0..F_nboys (nboys <= kBoysMax) of npoint arguments T, F row-major (npoint x
+// (nboys + 1)), FP64 (boys.hip).
+constexpr int kBoysMax = 16;
+void launch_boys(int nboys, int npoint, const double* T, double* F, hipStream_t s);
+
+// The learned screen's network (mlp.hip): ni <= kMlpMaxIn inputs, 3 hidden layers of kMlpHidden
+// ReLU units, one linear output, FP32; kMlpTile samples per workgroup.
+constexpr int kMlpHidden = 64;
+constexpr int kMlpMaxIn = 32;
+constexpr int kMlpTile = 64;
+int mlp_nparam(int ni);
+void launch_mlp_forward(const float* params, int ni, int n, const float* X, float* out, hipStream_t s);
+// grad += d/dparams of (1/nb) sum_s sw_s r_s^2 (times wu where r_s < 0), r_s = f(X[idx[s]]) - y[idx[s]]
+// (idx null: s itself; sw null: 1); loss[0] += that loss.
+void launch_mlp_grad(const float* params, int ni, int nb, const int* idx, const float* X, const float* y,
+                     const float* sw, float wu, float* grad, float* loss, hipStream_t s);
+void launch_mlp_adam(int np, float* params, const float* grad, float* m, float* v, float lr, float b1, float b2,
+                     float eps, int t, hipStream_t s);
 void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 }  // namespace lsjk

@@ -54,6 +54,25 @@ void JKBatch::jk_device(const std::shared_ptr<Ewald>& ewald, const double* D_dev
     check(lsjk_batch_jk(dev_, D_dev, (int)s.size(), s.data(), o.data(), thresp, thredp, J_dev, K_dev));
 }
 
+std::vector<std::tuple<int, int, double, double, double>> JKBatch::autotune(const std::shared_ptr<Ewald>& ewald,
+                                                                            const double* D_dev, float thresp,
+                                                                            float thredp)
+{
+    const auto& s = ewald->scales();
+    const auto& o = ewald->omegas();
+    check(lsjk_batch_autotune(dev_, D_dev, (int)s.size(), s.data(), o.data(), thresp, thredp));
+    const lsjk_pairlist* pl = nullptr;
+    check(lsjk_batch_pairlist(dev_, &pl));
+    int n = 0;
+    check(lsjk_tuned_params(pl, 0, &n, nullptr, nullptr, nullptr, nullptr, nullptr));
+    std::vector<int> kind(std::max(n, 1)), idx(std::max(n, 1));
+    std::vector<double> val(std::max(n, 1)), m0(std::max(n, 1)), m1(std::max(n, 1));
+    check(lsjk_tuned_params(pl, n, &n, kind.data(), idx.data(), val.data(), m0.data(), m1.data()));
+    std::vector<std::tuple<int, int, double, double, double>> out;
+    for (int i = 0; i < n; i++) out.emplace_back(kind[i], idx[i], val[i], m0[i], m1[i]);
+    return out;
+}
+
 std::pair<std::vector<std::shared_ptr<Tensor>>, std::vector<std::shared_ptr<Tensor>>> JKBatch::jk(
     const std::shared_ptr<Ewald>& ewald, const std::vector<std::shared_ptr<Tensor>>& Ds, float thresp, float thredp)
 {
@@ -105,6 +124,29 @@ std::pair<std::vector<std::shared_ptr<Tensor>>, std::vector<std::shared_ptr<Tens
         }
     }
     return {Js, Ks};
+}
+
+std::shared_ptr<Tensor> Boys::compute(int nboys, const std::shared_ptr<Tensor>& T)
+{
+    if (nboys < 0 || nboys > 16) throw std::runtime_error("Boys::compute: nboys must be 0..16");
+    const size_t n = T->size();
+    hip_session_init(0);
+    auto F = std::make_shared<Tensor>(std::vector<size_t>{n, (size_t)nboys + 1}, "F");
+    void *dT = nullptr, *dF = nullptr;
+    try {
+        check(lsjk_malloc(&dT, std::max<size_t>(n, 1) * sizeof(double)));
+        check(lsjk_malloc(&dF, std::max<size_t>(n * (nboys + 1), 1) * sizeof(double)));
+        if (n) check(lsjk_memcpy_htod(dT, T->data().data(), n * sizeof(double)));
+        check(lsjk_boys(nboys, (int)n, (const double*)dT, (double*)dF));
+        if (n) check(lsjk_memcpy_dtoh(F->data().data(), dF, n * (nboys + 1) * sizeof(double)));
+    } catch (...) {
+        if (dT) lsjk_free(dT);
+        if (dF) lsjk_free(dF);
+        throw;
+    }
+    lsjk_free(dT);
+    lsjk_free(dF);
+    return F;
 }
 
 }  // namespace lightspeed

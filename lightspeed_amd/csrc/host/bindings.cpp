@@ -196,6 +196,8 @@ PYBIND11_MODULE(_lightspeed, m)
         .def_property_readonly("pairlists", &PairList::pairlists)
         .def_property_readonly("npair", &PairList::npair)
         .def("max_bound", [](const std::shared_ptr<PairList>& p) { return PairListUtil::max_bound(p); })
+        .def_property_readonly("device_ptr",
+                               [](const std::shared_ptr<PairList>& p) { return (uintptr_t)p->device_handle(); })
         // every stride-th pair of each class, order kept (a bounded sample of a list for tests
         // against the CPU oracle at full-size workloads)
         .def("subsample", [](const std::shared_ptr<PairList>& p, int stride) {
@@ -284,7 +286,41 @@ PYBIND11_MODULE(_lightspeed, m)
             [](JKBatch& b, std::shared_ptr<Ewald> ew, uintptr_t D, float thresp, float thredp, uintptr_t J,
                uintptr_t K) { b.jk_device(ew, (const double*)D, thresp, thredp, (double*)J, (double*)K); },
             py::arg("ewald"), py::arg("D_ptr"), py::arg("thresp"), py::arg("thredp"), py::arg("J_ptr"),
-            py::arg("K_ptr"), py::call_guard<py::gil_scoped_release>());
+            py::arg("K_ptr"), py::call_guard<py::gil_scoped_release>())
+        .def(
+            "autotune",
+            [](JKBatch& b, std::shared_ptr<Ewald> ew, uintptr_t D, float thresp, float thredp) {
+                return b.autotune(ew, (const double*)D, thresp, thredp);
+            },
+            py::arg("ewald"), py::arg("D_ptr"), py::arg("thresp"), py::arg("thredp"));
+
+    py::class_<Boys>(m, "Boys").def_static("compute", &Boys::compute, py::arg("nboys"), py::arg("T"));
+
+    auto tuned_list = [](const lsjk_pairlist* d) {
+        int n = 0;
+        check(lsjk_tuned_params(d, 0, &n, nullptr, nullptr, nullptr, nullptr, nullptr));
+        std::vector<int> kind(std::max(n, 1)), idx(std::max(n, 1));
+        std::vector<double> val(std::max(n, 1)), m0(std::max(n, 1)), m1(std::max(n, 1));
+        check(lsjk_tuned_params(d, n, &n, kind.data(), idx.data(), val.data(), m0.data(), m1.data()));
+        py::list out;
+        for (int i = 0; i < n; i++) out.append(py::make_tuple(kind[i], idx[i], val[i], m0[i], m1[i]));
+        return out;
+    };
+    m.def(
+        "autotune",
+        [tuned_list](std::shared_ptr<PairList> pl, std::shared_ptr<Ewald> ew, uintptr_t D, float thresp, float thredp) {
+            lsjk_pairlist* d = pl->device_handle();
+            const auto& sc = ew->scales();
+            const auto& om = ew->omegas();
+            {
+                py::gil_scoped_release rel;
+                check(lsjk_autotune(d, (const double*)D, (int)sc.size(), sc.data(), om.data(), thresp, thredp));
+            }
+            return tuned_list(d);
+        },
+        py::arg("pairlist"), py::arg("ewald"), py::arg("D_ptr"), py::arg("thresp"), py::arg("thredp"),
+        "Autotune the J chunk and K split of a symmetric list on both sides; returns "
+        "[(kind, index, value, ms_default, ms_tuned), ...]");
 
     m.def("hip_init", &hip_session_init, py::arg("device") = 0);
     m.def("hip_finalize", &hip_session_finalize);

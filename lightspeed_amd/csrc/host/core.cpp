@@ -427,66 +427,74 @@ std::vector<double> PureTransform::solid_harmonics(int L)
 }
 
 namespace {
-// Per-shell transform blocks of a basis: (aoIdx, cartIdx, L, pure)
+// The shells of a basis with their transform blocks: C (npure x ncart, the real solid harmonics)
+// for a pure shell, none (identity) for a Cartesian one.
 struct ShellInfo {
-    int ao, cart, L;
-    bool pure;
+    int ao, cart, L, nao, nc;
+    const std::vector<double>* C;  // nullptr: Cartesian shell
 };
-std::vector<ShellInfo> shells_of(const Basis& b)
+std::vector<ShellInfo> shells_of(const Basis& b, std::map<int, std::vector<double>>& cache)
 {
     std::vector<ShellInfo> s;
     int last = -1;
     for (auto& p : b.primitives())
         if (p.shellIdx() != last) {
-            s.push_back({p.aoIdx(), p.cartIdx(), p.L(), p.is_pure()});
+            const int L = p.L(), nc = (L + 1) * (L + 2) / 2;
+            const std::vector<double>* C = nullptr;
+            if (p.is_pure()) {
+                if (!cache.count(L)) cache[L] = PureTransform::solid_harmonics(L);
+                C = &cache[L];
+            }
+            s.push_back({p.aoIdx(), p.cartIdx(), L, p.is_pure() ? 2 * L + 1 : nc, nc, C});
             last = p.shellIdx();
         }
     return s;
 }
-// Dense nao x ncart transformation matrix (identity blocks for Cartesian shells).
-std::vector<double> transform_matrix(const Basis& b)
-{
-    std::vector<double> T(b.nao() * b.ncart(), 0.0);
-    std::map<int, std::vector<double>> cache;
-    for (auto& s : shells_of(b)) {
-        int nc = (s.L + 1) * (s.L + 2) / 2;
-        if (s.pure) {
-            if (!cache.count(s.L)) cache[s.L] = PureTransform::solid_harmonics(s.L);
-            const auto& C = cache[s.L];
-            for (int r = 0; r < 2 * s.L + 1; r++)
-                for (int c = 0; c < nc; c++) T[(size_t)(s.ao + r) * b.ncart() + s.cart + c] = C[r * nc + c];
-        } else {
-            for (int c = 0; c < nc; c++) T[(size_t)(s.ao + c) * b.ncart() + s.cart + c] = 1.0;
-        }
-    }
-    return T;
-}
 }  // namespace
 
+// Both transforms work shell pair by shell pair, as the reference's (pure_transform.cpp:46-247):
+// a block of one shell pair maps to the block of the same pair, so the cost is O(nshell^2) small
+// products, not dense nao x ncart matrices.
 std::shared_ptr<Tensor> PureTransform::pureToCart2(const std::shared_ptr<Basis>& b1, const std::shared_ptr<Basis>& b2,
                                                    const std::shared_ptr<Tensor>& Tpure)
 {
     Tpure->shape_error({b1->nao(), b2->nao()});
     if (!b1->has_pure() && !b2->has_pure()) return Tpure;
-    const size_t n1 = b1->nao(), n2 = b2->nao(), c1 = b1->ncart(), c2 = b2->ncart();
-    std::vector<double> T1 = transform_matrix(*b1), T2 = transform_matrix(*b2);
-    // X = Tpure T2 (n1 x c2), Tcart = T1^T X
-    std::vector<double> X(n1 * c2, 0.0);
+    std::map<int, std::vector<double>> cache;
+    const auto S1 = shells_of(*b1, cache), S2 = shells_of(*b2, cache);
+    const size_t n2 = b2->nao(), c2 = b2->ncart();
+    auto out = std::make_shared<Tensor>(std::vector<size_t>{b1->ncart(), c2});
     const double* P = Tpure->data().data();
-    for (size_t i = 0; i < n1; i++)
-        for (size_t k = 0; k < n2; k++) {
-            double v = P[i * n2 + k];
-            if (v == 0.0) continue;
-            const double* t = &T2[k * c2];
-            for (size_t j = 0; j < c2; j++) X[i * c2 + j] += v * t[j];
-        }
-    auto out = std::make_shared<Tensor>(std::vector<size_t>{c1, c2});
     double* O = out->data().data();
-    for (size_t k = 0; k < n1; k++)
-        for (size_t i = 0; i < c1; i++) {
-            double t = T1[k * c1 + i];
-            if (t == 0.0) continue;
-            for (size_t j = 0; j < c2; j++) O[i * c2 + j] += t * X[k * c2 + j];
+    size_t m = 1;
+    for (const auto& x : S1) m = std::max(m, (size_t)x.nc);
+    for (const auto& x : S2) m = std::max(m, (size_t)x.nc);
+    std::vector<double> X(m * m), Y(m * m);
+    for (const auto& s1 : S1)
+        for (const auto& s2 : S2) {
+            // X (nao1 x nc2) = P_block C2, Y (nc1 x nc2) = C1^T X
+            for (int i = 0; i < s1.nao; i++)
+                for (int b = 0; b < s2.nc; b++) {
+                    double v = 0.0;
+                    if (s2.C) {
+                        for (int k = 0; k < s2.nao; k++) v += P[(size_t)(s1.ao + i) * n2 + s2.ao + k] * (*s2.C)[k * s2.nc + b];
+                    } else {
+                        v = P[(size_t)(s1.ao + i) * n2 + s2.ao + b];
+                    }
+                    X[i * s2.nc + b] = v;
+                }
+            for (int a = 0; a < s1.nc; a++)
+                for (int b = 0; b < s2.nc; b++) {
+                    double v = 0.0;
+                    if (s1.C) {
+                        for (int i = 0; i < s1.nao; i++) v += (*s1.C)[i * s1.nc + a] * X[i * s2.nc + b];
+                    } else {
+                        v = X[a * s2.nc + b];
+                    }
+                    Y[a * s2.nc + b] = v;
+                }
+            for (int a = 0; a < s1.nc; a++)
+                for (int b = 0; b < s2.nc; b++) O[(size_t)(s1.cart + a) * c2 + s2.cart + b] = Y[a * s2.nc + b];
         }
     return out;
 }
@@ -499,24 +507,38 @@ std::shared_ptr<Tensor> PureTransform::cartToPure2(const std::shared_ptr<Basis>&
     Tcart->shape_error({b1->ncart(), b2->ncart()});
     if (!b1->has_pure() && !b2->has_pure()) return Tcart;
     auto out = Tpure ? Tpure : std::make_shared<Tensor>(std::vector<size_t>{b1->nao(), b2->nao()});
-    const size_t n1 = b1->nao(), n2 = b2->nao(), c1 = b1->ncart(), c2 = b2->ncart();
-    std::vector<double> T1 = transform_matrix(*b1), T2 = transform_matrix(*b2);
-    // X = Tcart T2^T (c1 x n2), out += T1 X
-    std::vector<double> X(c1 * n2, 0.0);
+    std::map<int, std::vector<double>> cache;
+    const auto S1 = shells_of(*b1, cache), S2 = shells_of(*b2, cache);
+    const size_t n2 = b2->nao(), c2 = b2->ncart();
     const double* C = Tcart->data().data();
-    for (size_t i = 0; i < c1; i++)
-        for (size_t k = 0; k < n2; k++) {
-            const double* t = &T2[k * c2];
-            double s = 0.0;
-            for (size_t j = 0; j < c2; j++) s += C[i * c2 + j] * t[j];
-            X[i * n2 + k] = s;
-        }
     double* O = out->data().data();
-    for (size_t r = 0; r < n1; r++)
-        for (size_t i = 0; i < c1; i++) {
-            double t = T1[r * c1 + i];
-            if (t == 0.0) continue;
-            for (size_t k = 0; k < n2; k++) O[r * n2 + k] += t * X[i * n2 + k];
+    size_t m = 1;
+    for (const auto& x : S1) m = std::max(m, (size_t)x.nc);
+    for (const auto& x : S2) m = std::max(m, (size_t)x.nc);
+    std::vector<double> X(m * m);
+    for (const auto& s1 : S1)
+        for (const auto& s2 : S2) {
+            // X (nc1 x nao2) = T_block C2^T, out_block += C1 X
+            for (int a = 0; a < s1.nc; a++)
+                for (int k = 0; k < s2.nao; k++) {
+                    double v = 0.0;
+                    if (s2.C) {
+                        for (int b = 0; b < s2.nc; b++) v += C[(size_t)(s1.cart + a) * c2 + s2.cart + b] * (*s2.C)[k * s2.nc + b];
+                    } else {
+                        v = C[(size_t)(s1.cart + a) * c2 + s2.cart + k];
+                    }
+                    X[a * s2.nao + k] = v;
+                }
+            for (int r = 0; r < s1.nao; r++)
+                for (int k = 0; k < s2.nao; k++) {
+                    double v = 0.0;
+                    if (s1.C) {
+                        for (int a = 0; a < s1.nc; a++) v += (*s1.C)[r * s1.nc + a] * X[a * s2.nao + k];
+                    } else {
+                        v = X[r * s2.nao + k];
+                    }
+                    O[(size_t)(s1.ao + r) * n2 + s2.ao + k] += v;
+                }
         }
     return out;
 }

@@ -8,6 +8,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 struct lsjk_basis;
@@ -307,6 +308,11 @@ public:
     // output may be null).
     void jk_device(const std::shared_ptr<Ewald>& ewald, const double* D_dev, float thresp, float thredp,
                    double* J_dev, double* K_dev);
+    // lsjk_batch_autotune on the batch's combined list with a density in the batch layout; returns
+    // the tuned table (kind, counter index, value, ms default, ms tuned).
+    std::vector<std::tuple<int, int, double, double, double>> autotune(const std::shared_ptr<Ewald>& ewald,
+                                                                       const double* D_dev, float thresp,
+                                                                       float thredp);
     // Host densities (nao_m x nao_m each) -> fresh J_m, K_m.
     std::pair<std::vector<std::shared_ptr<Tensor>>, std::vector<std::shared_ptr<Tensor>>> jk(
         const std::shared_ptr<Ewald>& ewald, const std::vector<std::shared_ptr<Tensor>>& Ds, float thresp,
@@ -319,6 +325,13 @@ private:
     int ld_ = 0;
     size_t nelem_ = 0;
     std::vector<size_t> offsets_;
+};
+
+// ------------------------------------------------------------------ Boys function
+// Reference src/lightspeed/boys.hpp:170-181 (Boys::compute): an (npoint, nboys + 1) Tensor of
+// F_0..F_nboys at the npoint arguments of T (nboys <= 16), computed on the GPU.
+struct Boys {
+    static std::shared_ptr<Tensor> compute(int nboys, const std::shared_ptr<Tensor>& T);
 };
 
 // Session helpers (the HIP library is one process-wide session, like TCIntBox).

@@ -47,6 +47,29 @@ class Integrals:
             self.ls.IntBox.exchange(self.res, self.ewald, self.pairs, self.pairs, Dt, True, thresp, thredp).np)
 
 
+class ScreenedIntegrals(Integrals):
+    """Integrals whose exchange runs on the learned route (lightspeed_amd.screen.Screen); J stays on
+    the FP64 route."""
+
+    def __init__(self, basis, screen, threpq=1.0e-14):
+        super().__init__(basis, threpq)
+        self.screen = screen
+        n = basis.ncart
+        self.dD = self.ls.DeviceArray([n, n])
+        self.dK = self.ls.DeviceArray([n, n])
+        if basis.has_pure:
+            raise ValueError("ScreenedIntegrals: Cartesian bases only")
+
+    def coulomb(self, D, thresp, thredp):
+        return super().coulomb(D, thresp, thresp)
+
+    def exchange(self, D, thresp, thredp):
+        self.dD.upload(np.ascontiguousarray(D))
+        self.dK.zero()
+        self.screen.exchange_device(self.ewald, self.pairs, self.pairs, self.dD.ptr, True, thresp, thredp, self.dK.ptr)
+        return self.dK.download()
+
+
 def nuclear_repulsion(xyzZ):
     """sum_{A<B} Z_A Z_B / R_AB (bohr)."""
     xyzZ = np.asarray(xyzZ, dtype=float)
@@ -62,7 +85,7 @@ class RHF:
     """Closed-shell RHF.  After run(): energy, converged, iterations, C, eps, D, history."""
 
     def __init__(self, molecule, integrals, charge=0, thresp=1.0e-14, thredp=1.0e-14, maxiter=100, e_conv=1.0e-12,
-                 g_conv=1.0e-9, diis_max=8, fixed_iterations=None):
+                 g_conv=1.0e-9, diis_max=8, fixed_iterations=None, log=None):
         self.molecule = molecule
         self.ints = integrals
         nel = int(round(sum(a.Z for a in molecule.atoms))) - charge
@@ -73,6 +96,7 @@ class RHF:
         self.maxiter, self.e_conv, self.g_conv, self.diis_max = maxiter, e_conv, g_conv, diis_max
         self.fixed_iterations = fixed_iterations
         self.history = []
+        self.log = log  # optional callable: one progress line per iteration
 
     def run(self):
         xyzZ = self.molecule.xyzZ
@@ -101,6 +125,8 @@ class RHF:
             G = F @ D @ S - S @ D @ F
             gnorm = float(np.max(np.abs(G)))
             self.history.append((it, E, gnorm))
+            if self.log:
+                self.log("RHF iteration %d: E = %.10f  max|FDS - SDF| = %.2e" % (it, E, gnorm))
             if self.fixed_iterations is None and it > 1 and abs(E - E_old) < self.e_conv and gnorm < self.g_conv:
                 self.converged = True
                 break
@@ -131,3 +157,4 @@ class RHF:
         self.iterations = len(self.history)
         self.C, self.eps, self.D = C, eps, D
         return self.energy
+

@@ -988,6 +988,150 @@ int oracle_exchange(const ob_basis* b1, const ob_basis* b2, const ob_basis* b3, 
     return 0;
 }
 
+/* ---------------------------------------------------------------- quartet magnitudes */
+
+/* All Cartesian components (ab|O|cd) of one primitive quartet, I[((a*N2 + b)*N3 + c)*N4 + d]: the
+ * per-quartet evaluation of IntBox::exchangeCPU (src/cpp/intbox/cpu/exchange.cpp:191-324: the
+ * prefactor, the Rys roots, the rotated nrys x nrys Gauss-Hermite 2-D integrals and their
+ * assembly), without the contraction with D24. */
+static void quartet_tensor(const ob_basis* b1, int P1, const ob_basis* b2, int P2, const ob_basis* b3, int P3,
+                           const ob_basis* b4, int P4, int nterm, const double* alphas, const double* omegas,
+                           double* I)
+{
+    int L1 = b1->L[P1], L2 = b2->L[P2], L3 = b3->L[P3], L4 = b4->L[P4];
+    int V1 = L1 + 1, V2 = L2 + 1, V3 = L3 + 1, V4 = L4 + 1;
+    int N1 = ncart_of(L1), N2 = ncart_of(L2), N3 = ncart_of(L3), N4 = ncart_of(L4);
+    int nrys = (L1 + L2 + L3 + L4) / 2 + 1;
+    const double* tgh = lsjk_gh_x + nrys * (nrys - 1) / 2;
+    const double* wgh = lsjk_gh_w + nrys * (nrys - 1) / 2;
+    memset(I, 0, sizeof(double) * N1 * N2 * N3 * N4);
+    double e1 = b1->e[P1], e2 = b2->e[P2], e3 = b3->e[P3], e4 = b4->e[P4];
+    const double *x1 = b1->xyz + 3 * P1, *x2 = b2->xyz + 3 * P2, *x3 = b3->xyz + 3 * P3, *x4 = b4->xyz + 3 * P4;
+    double r12 = 0.0, r34 = 0.0;
+    for (int d = 0; d < 3; d++) {
+        r12 += pow(x1[d] - x2[d], 2);
+        r34 += pow(x3[d] - x4[d], 2);
+    }
+    double K12 = b1->c[P1] * b2->c[P2] * exp(-e1 * e2 / (e1 + e2) * r12);
+    double K34 = b3->c[P3] * b4->c[P4] * exp(-e3 * e4 / (e3 + e4) * r34);
+    double e12 = e1 + e2, e34 = e3 + e4, e12_m1 = 1.0 / e12, e34_m1 = 1.0 / e34;
+    double xP[3], xQ[3], rPQ_2 = 0.0;
+    for (int d = 0; d < 3; d++) {
+        xP[d] = (e1 * x1[d] + e2 * x2[d]) * e12_m1;
+        xQ[d] = (e3 * x3[d] + e4 * x4[d]) * e34_m1;
+        rPQ_2 += pow(xP[d] - xQ[d], 2);
+    }
+    double rho = e12 * e34 / (e12 + e34);
+    for (int it = 0; it < nterm; it++) {
+        double alpha = alphas[it], omega = omegas[it];
+        if (alpha == 0.0 || omega == 0.0) continue;
+        double pref = alpha * K12 * K34 * 2.0 * pow(M_PI, -0.5) * e12_m1 * e34_m1 * pow(e12 + e34, -0.5);
+        double d2 = 1.0;
+        if (omega != -1.0) {
+            d2 = omega * omega / (rho + omega * omega);
+            pref *= sqrt(d2);
+        }
+        double rx[9], rw[9];
+        rys_eval(nrys, rho * d2 * rPQ_2, rx, rw);
+        for (int r = 0; r < nrys; r++) {
+            double t2 = rx[r] * d2, w = rw[r];
+            double J2[3][(OB_MAXL + 1) * (OB_MAXL + 1) * (OB_MAXL + 1) * (OB_MAXL + 1)];
+            memset(J2, 0, sizeof(J2));
+            double gamma = rho * t2 / (1.0 - t2);
+            double delta_m1 = (e34 - e12) / (2.0 * gamma);
+            double Qv = delta_m1 * pow(1.0 + delta_m1 * delta_m1, -0.5);
+            double c2 = 0.5 + 0.5 * Qv, s2 = 0.5 - 0.5 * Qv, cs = 0.5 * pow(1.0 + delta_m1 * delta_m1, -0.5);
+            double cv = sqrt(c2), sv = sqrt(s2);
+            double a1 = c2 * (e12 + gamma) + s2 * (e34 + gamma) - 2.0 * cs * gamma;
+            double a2 = s2 * (e12 + gamma) + c2 * (e34 + gamma) + 2.0 * cs * gamma;
+            double X11 = cv * pow(a1, -0.5), X12 = -sv * pow(a2, -0.5);
+            double X21 = sv * pow(a1, -0.5), X22 = cv * pow(a2, -0.5);
+            double xA[3], xB[3];
+            for (int d = 0; d < 3; d++) {
+                xA[d] = xP[d] + rho * t2 * e12_m1 * (xQ[d] - xP[d]);
+                xB[d] = xQ[d] + rho * t2 * e34_m1 * (xP[d] - xQ[d]);
+            }
+            for (int v1 = 0; v1 < nrys; v1++)
+                for (int v2 = 0; v2 < nrys; v2++) {
+                    double z1 = X11 * tgh[v1] + X12 * tgh[v2];
+                    double z2 = X21 * tgh[v1] + X22 * tgh[v2];
+                    double wv = wgh[v1] * wgh[v2];
+                    for (int d = 0; d < 3; d++) {
+                        double* out = J2[d];
+                        double p1 = 1.0;
+                        for (int l1 = 0; l1 < V1; l1++) {
+                            double p2 = 1.0;
+                            for (int l2 = 0; l2 < V2; l2++) {
+                                double p3 = 1.0;
+                                for (int l3 = 0; l3 < V3; l3++) {
+                                    double p4 = 1.0;
+                                    for (int l4 = 0; l4 < V4; l4++) {
+                                        (*out++) += wv * p1 * p2 * p3 * p4;
+                                        p4 *= z2 + xB[d] - x4[d];
+                                    }
+                                    p3 *= z2 + xB[d] - x3[d];
+                                }
+                                p2 *= z1 + xA[d] - x2[d];
+                            }
+                            p1 *= z1 + xA[d] - x1[d];
+                        }
+                    }
+                }
+            /* Cartesian component (a, b, c, d) of each centre: exponents (lx, ly, lz) in lexical order */
+            int lx[4][15], ly[4][15], lz[4][15];
+            const int Ls[4] = {L1, L2, L3, L4};
+            for (int k = 0; k < 4; k++) {
+                int idx = 0;
+                for (int ai = 0; ai <= Ls[k]; ai++)
+                    for (int bi = 0; bi <= ai; bi++, idx++) {
+                        lx[k][idx] = Ls[k] - ai;
+                        ly[k][idx] = ai - bi;
+                        lz[k][idx] = bi;
+                    }
+            }
+            for (int a = 0; a < N1; a++)
+                for (int b = 0; b < N2; b++)
+                    for (int c = 0; c < N3; c++)
+                        for (int d = 0; d < N4; d++) {
+                            int ix = ((lx[0][a] * V2 + lx[1][b]) * V3 + lx[2][c]) * V4 + lx[3][d];
+                            int iy = ((ly[0][a] * V2 + ly[1][b]) * V3 + ly[2][c]) * V4 + ly[3][d];
+                            int iz = ((lz[0][a] * V2 + lz[1][b]) * V3 + lz[2][c]) * V4 + lz[3][d];
+                            I[((a * N2 + b) * N3 + c) * N4 + d] += pref * w * J2[0][ix] * J2[1][iy] * J2[2][iz];
+                        }
+        }
+    }
+}
+
+/* Largest |component| of each of n primitive quartets (P1[k] P2[k] | P3[k] P4[k]): the magnitude
+ * the learned screen is trained to bound (data generation checks the GPU against this). */
+int oracle_quartet_max(const ob_basis* b1, const ob_basis* b2, const ob_basis* b3, const ob_basis* b4, int64_t n,
+                       const int* P1, const int* P2, const int* P3, const int* P4, int nterm, const double* alphas,
+                       const double* omegas, double* out)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < n; k++) {
+        double I[1296];
+        quartet_tensor(b1, P1[k], b2, P2[k], b3, P3[k], b4, P4[k], nterm, alphas, omegas, I);
+        int N = ncart_of(b1->L[P1[k]]) * ncart_of(b2->L[P2[k]]) * ncart_of(b3->L[P3[k]]) * ncart_of(b4->L[P4[k]]);
+        double m = 0.0;
+        for (int i = 0; i < N; i++) m = fmax(m, fabs(I[i]));
+        out[k] = m;
+    }
+    return 0;
+}
+
+/* The full component tensors of n quartets, out[k * 1296 + ...] in the quartet_tensor layout
+ * (pinning of quartet_tensor against the oracle's J). */
+int oracle_quartet_tensor(const ob_basis* b1, const ob_basis* b2, const ob_basis* b3, const ob_basis* b4, int64_t n,
+                          const int* P1, const int* P2, const int* P3, const int* P4, int nterm,
+                          const double* alphas, const double* omegas, double* out)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t k = 0; k < n; k++)
+        quartet_tensor(b1, P1[k], b2, P2[k], b3, P3[k], b4, P4[k], nterm, alphas, omegas, out + k * 1296);
+    return 0;
+}
+
 /* ---------------------------------------------------------------- one-electron (S, T, V) */
 
 /* IntBox::overlapCPU and IntBox::kineticCPU (src/cpp/intbox/cpu/overlap.cpp:14-147,

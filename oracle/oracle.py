@@ -67,6 +67,9 @@ def lib():
         L.oracle_potential.argtypes = [ctypes.POINTER(_CBasis)] * 2 + [
             ctypes.POINTER(_CPairList), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
             ctypes.c_void_p, ctypes.c_void_p]
+        for f in (L.oracle_quartet_max, L.oracle_quartet_tensor):
+            f.argtypes = [ctypes.POINTER(_CBasis)] * 4 + [ctypes.c_int64] + [ctypes.c_void_p] * 4 + [
+                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -304,6 +307,33 @@ def exchange(ewald, pl12, pl34, D24, D24symm, thresp, K13=None, counts=None):
             raise RuntimeError("Tensor: shape error")
         K = K13 + K
     return K
+
+
+def _quartet_call(fn, width, ewald, bases, P):
+    P = [np.ascontiguousarray(x, dtype=np.int32).reshape(-1) for x in P]
+    n = P[0].size
+    out = np.zeros(max(n, 1) * width)
+    sc, om = _ewald_arrays(ewald)
+    fn(*[ctypes.byref(b.cstruct) for b in bases], n, *[x.ctypes.data for x in P], len(sc), sc.ctypes.data,
+       om.ctypes.data, out.ctypes.data)
+    return out[:n * width]
+
+
+def quartet_max(ewald, b1, b2, b3, b4, P1, P2, P3, P4):
+    """max over Cartesian components of |(P1 P2 | O | P3 P4)| for each primitive quartet (the
+    per-quartet evaluation of exchange.cpp:191-324 without the D contraction)."""
+    return _quartet_call(lib().oracle_quartet_max, 1, ewald, (b1, b2, b3, b4), (P1, P2, P3, P4))
+
+
+def quartet_tensor(ewald, b1, b2, b3, b4, P1, P2, P3, P4):
+    """All Cartesian components of each primitive quartet, shape (n, N1, N2, N3, N4)."""
+    P = [np.ascontiguousarray(x, dtype=np.int32).reshape(-1) for x in (P1, P2, P3, P4)]
+    flat = _quartet_call(lib().oracle_quartet_tensor, 1296, ewald, (b1, b2, b3, b4), P).reshape(-1, 1296)
+    out = []
+    for k in range(flat.shape[0]):
+        N = [(int(b.L[p[k]]) + 1) * (int(b.L[p[k]]) + 2) // 2 for b, p in zip((b1, b2, b3, b4), P)]
+        out.append(flat[k, :N[0] * N[1] * N[2] * N[3]].reshape(N))
+    return out
 
 
 def rys(n, T):

@@ -172,6 +172,20 @@ def _mixed_problem(ls, name, seed=1):
     return bas, D
 
 
+def _fold_mirrors(counts):
+    """K counters (lsjk_last_counts layout) summed over each pair of mirror classes
+    (L1 L2 | L3 L4) / (L3 L4 | L1 L2), keyed by the lower index."""
+    out = {}
+    for L1 in range(3):
+        for L2 in range(3):
+            for L3 in range(3):
+                for L4 in range(3):
+                    c = 256 + ((L1 * 5 + L2) * 5 + L3) * 5 + L4
+                    m = 256 + ((L3 * 5 + L4) * 5 + L1) * 5 + L2
+                    out[min(c, m)] = out.get(min(c, m), 0) + int(counts[c])
+    return np.array([out[k] for k in sorted(out)], np.int64)
+
+
 def test_mixed_route_h2o10_against_oracle(product_backend, oracle_backend):
     """thredp > thresp: FP64 for est >= thredp, FP32 for thresp <= est < thredp, FP64 accumulation
     (intbox.hpp:736-742).  Against the FP64 oracle: max|dJ|, max|dK| <= 1e-7 at thredp = 1e-6 (the
@@ -199,7 +213,9 @@ def test_mixed_route_h2o10_against_oracle(product_backend, oracle_backend):
         ck = ls.last_counts().astype(np.int64)
     finally:
         ls.set_counting(False)
-    assert np.array_equal(cj, cj0) and np.array_equal(ck, ck0)
+    # the symmetric K mode evaluates each pair of mirror classes in one orientation, chosen per route
+    # by timing, so the two routes' counts agree per pair of mirror classes
+    assert np.array_equal(cj, cj0) and np.array_equal(_fold_mirrors(ck), _fold_mirrors(ck0))
     dJ, dK = np.abs(J - Jo).max(), np.abs(K - Ko).max()
     _say("  max|dJ| = %.3e  max|dK| = %.3e" % (dJ, dK))
     assert dJ <= 1.0e-7 and dK <= 1.0e-7

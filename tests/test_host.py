@@ -195,3 +195,22 @@ def test_bench_refuses_several_gpus():
     import subprocess
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_pure_transforms_per_block_against_dense(product_backend, oracle_backend):
+    """PureTransform.pureToCart2 / cartToPure2 work shell pair by shell pair; they equal the dense
+    products with the oracle's nao x ncart transform matrix (C4H10 / cc-pVDZ, spherical d)."""
+    ls = product_backend.ls
+    O = oracle_backend.O
+    from lightspeed_amd import workloads as W
+    bas = ls.Basis.from_gbs_file(W.alkane(4), "cc-pvdz")
+    assert bas.has_pure and bas.nao < bas.ncart
+    T = O.OBasis(bas).transform()
+    rng = np.random.default_rng(0)
+    P = rng.standard_normal((bas.nao, bas.nao))
+    C = np.array(ls.PureTransform.pureToCart2(bas, bas, ls.Tensor.array(P)).np)
+    assert np.abs(C - T.T @ P @ T).max() < 1e-13
+    Kc = rng.standard_normal((bas.ncart, bas.ncart))
+    Q0 = rng.standard_normal((bas.nao, bas.nao))
+    Q = np.array(ls.PureTransform.cartToPure2(bas, bas, ls.Tensor.array(Kc), ls.Tensor.array(Q0.copy())).np)
+    assert np.abs(Q - (Q0 + T @ Kc @ T.T)).max() < 1e-13

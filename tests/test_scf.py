@@ -108,3 +108,28 @@ def test_rhf_h2o64_mixed_route_energy_error(product_backend):
     print("(H2O)64/6-31G**: max |dE| mixed vs fp64 over 10 iterations = %.3e" % dE, flush=True)
     assert np.isfinite(e64.energy) and e64.energy < -4000.0  # 64 waters, about -76 Eh each
     assert dE <= 1.0e-6
+
+
+@pytest.mark.gpu
+def test_rhf_h2o256_mixed_route_energy_error(product_backend):
+    """(H2O)256/6-31G* (4864 functions, the largest workload), 10 iterations from the core guess: the
+    mixed route (thredp = 1e-6) stays within 1e-6 Eh of the FP64 route at every iteration (dense SCF
+    algebra on the host, J and K through IntBox; one progress line per iteration)."""
+    ls = product_backend.ls
+    from lightspeed_amd import workloads as W
+
+    def say(m):
+        print(m, flush=True)
+
+    mol, bname, _ = W.config("h2o256_631gs")
+    bas = ls.Basis.from_gbs_file(mol, bname)
+    ints = scf.Integrals(bas, 1.0e-12)
+    e64 = scf.RHF(mol, ints, thresp=1.0e-12, thredp=1.0e-12, fixed_iterations=10, log=say)
+    e64.run()
+    say("(H2O)256/6-31G* FP64 route: E10 = %.10f" % e64.energy)
+    emx = scf.RHF(mol, ints, thresp=1.0e-12, thredp=1.0e-6, fixed_iterations=10, log=say)
+    emx.run()
+    dE = max(abs(a[1] - b[1]) for a, b in zip(e64.history, emx.history))
+    say("(H2O)256/6-31G*: max |dE| mixed vs fp64 over 10 iterations = %.3e" % dE)
+    assert np.isfinite(e64.energy) and e64.energy < -19000.0  # 256 waters, about -76 Eh each
+    assert dE <= 1.0e-6
