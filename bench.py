@@ -33,7 +33,8 @@ nrys = (L1+L2+L3+L4)/2 + 1), against the peak of the launch's mix: its FP64 FLOP
 78.6 TFLOP/s and its FP32 FLOPs (the launch's evaluated FP32 share) at the FP32 peak 157.3 TFLOP/s.
 jk_frac = [sum_FP64 N F / 78.6e12 + sum_FP32 N F / 157.3e12] / t_JK (algorithmic counts);
 jk_exec_frac = the same over the quartets the kernels evaluated (their own counters).
-`--table PATH` writes the per-launch table (N, F, evaluated, ms, TFLOP/s, fraction).
+`--table PATH` writes the per-launch table (N, F, evaluated, ms, TFLOP/s, fraction, lane use = the
+share of the wave evaluating integrals in its busy steps).
 cpu_baseline: the reference's own CPU IntBox::coulombCPU + IntBox::exchangeCPU, compiled from
 its sources into oracle/_ref/ref_jk, timed on a bounded sample of the same workload (the bra list
 keeps every `--cpu-stride`-th pair of each class, the ket list is whole), all threads of the host.
@@ -339,7 +340,7 @@ def main():
         log("no trained screen at %s: learned routes not measured" % screen_path)
 
     # evaluated counts per route and results for the accuracy checks (whole matrices vs the FP64 route)
-    res_JK, cnt = {}, {}
+    res_JK, cnt, occ = {}, {}, {}
     ls.set_counting(True)
     for name, r in routes.items():
         wl.dJ.zero()
@@ -347,13 +348,16 @@ def main():
         thredp, scr = r
         ls.IntBox.coulomb_device(wl.ew, pl, pl, wl.dD.ptr, thresp, thresp if scr else thredp, wl.dJ.ptr)
         cj, cj32 = ls.last_counts().astype(np.int64), ls.last_counts_fp32().astype(np.int64)
+        wsj, wlj = (x.astype(np.int64) for x in ls.last_wave_steps())
         if scr:
             scr.exchange_device(wl.ew, pl, pl, wl.dD.ptr, True, thresp, thredp, wl.dK.ptr)
         else:
             ls.IntBox.exchange_device(wl.ew, pl, pl, wl.dD.ptr, True, thresp, thredp, wl.dK.ptr)
         ck, ck32 = ls.last_counts().astype(np.int64), ls.last_counts_fp32().astype(np.int64)
+        wsk, wlk = (x.astype(np.int64) for x in ls.last_wave_steps())
         res_JK[name] = (wl.dJ.download(), wl.dK.download())
         cnt[name] = (cj, cj32, ck, ck32)
+        occ[name] = (wsj, wlj, wsk, wlk)
     ls.set_counting(False)
     J64, K64 = res_JK["fp64"]
     acc = {}
@@ -476,8 +480,12 @@ def main():
         c = lid - 256
         m = mirror(lid)
         N = float(NK[c] + (NK[m - 256] if m != lid else 0))
-        f32 = ck32[lid] / ck[lid] if ck[lid] else 0.0
-        return N, N * cls_flops(c) * (1.0 - f32), N * cls_flops(c) * f32, int(ck[lid])
+        # the counting call ran before the autotuner, which may have turned the mirror pair around:
+        # its counts sit at one of the two indices
+        n_ev = int(ck[lid] + (ck[m] if m != lid else 0))
+        n_32 = int(ck32[lid] + (ck32[m] if m != lid else 0))
+        f32 = n_32 / n_ev if n_ev else 0.0
+        return N, N * cls_flops(c) * (1.0 - f32), N * cls_flops(c) * f32, n_ev
 
     def launch_name(kind, lid):
         # template arguments as the profiler prints them: route 0 plain FP64, 4 mixed (FP64 and FP32
@@ -486,8 +494,21 @@ def main():
             if (lid >> 24) & 1:
                 return "j_quartets_sym<%d, %d>" % ((lid & 0xfff) // 5, (lid & 0xfff) % 5)
             return "j_quartets<%d, %d, %d>" % (lid // 5, lid % 5, 4 if j_mixed else 0)
-        k_route = 4 if (routes[head][0] > thresp and ck32[lid] > 0) else 0
+        m = mirror(lid)
+        k_route = 4 if (routes[head][0] > thresp and ck32[lid] + (ck32[m] if m != lid else 0) > 0) else 0
         return "k_ket<%d, %d, %d, %d, %d>" % (cls_of(lid - 256) + (k_route,))
+
+    def lane_use(kind, lid):
+        """Share of the wave's lanes evaluating an integral in the launch's busy wave steps."""
+        wsj, wlj, wsk, wlk = occ[head]
+        if kind == "J":
+            i = lid & 0xfff
+            steps, lanes = wsj[i], wlj[i]
+        else:
+            m = mirror(lid)
+            steps = wsk[lid] + (wsk[m] if m != lid else 0)
+            lanes = wlk[lid] + (wlk[m] if m != lid else 0)
+        return float(lanes) / (64.0 * steps) if steps else float("nan")
 
     means = {k: float(np.mean(v)) for k, v in launch_ms.items()}
     log("serialised quartet kernels per step: J %.3f ms, K %.3f ms" % (
@@ -498,14 +519,15 @@ def main():
         fl = f64 + f32
         pk = fl / (f64 / FP64_PEAK + f32 / FP32_PEAK) if fl > 0 else FP64_PEAK
         rows.append((launch_name(*k), N, fl / max(N, 1.0), ev, means[k], fl / (means[k] * 1e-3) / 1e12,
-                     fl / (means[k] * 1e-3) / pk))
+                     fl / (means[k] * 1e-3) / pk, lane_use(*k)))
         log("  launch %-36s %9.3f ms  %8.3f TFLOP/s" % (rows[-1][0], means[k], rows[-1][5]))
     if args.table:
         with open(args.table, "w") as f:
-            f.write("| launch | N (reference convention) | FLOP/quartet | evaluated | ms | TFLOP/s | frac |\n")
-            f.write("|---|---:|---:|---:|---:|---:|---:|\n")
+            f.write("| launch | N (reference convention) | FLOP/quartet | evaluated | ms | TFLOP/s | frac "
+                    "| lane use |\n")
+            f.write("|---|---:|---:|---:|---:|---:|---:|---:|\n")
             for r in rows:
-                f.write("| %s | %d | %.0f | %d | %.3f | %.2f | %.3f |\n" % (r[0], r[1], r[2], r[3], r[4], r[5], r[6]))
+                f.write("| %s | %d | %.0f | %d | %.3f | %.2f | %.3f | %.3f |\n" % r)
     dom = max(means, key=lambda k: means[k])
     Nd, dom64, dom32, _ = launch_work(*dom)
     dom_fl = dom64 + dom32

@@ -279,6 +279,12 @@ int lsjk_set_counting(int enable);
 int lsjk_last_counts(uint64_t* counts1024);
 /* The part of lsjk_last_counts evaluated on the FP32 route (mixed precision calls), same layout. */
 int lsjk_last_counts_fp32(uint64_t* counts1024);
+/* Lane occupancy of the quartet kernels in the last counted call, same layout: steps1024[i] =
+ * wave steps of class i that evaluated at least one integral, lanes1024[i] = the lanes that
+ * evaluated one, summed over those steps (lanes / (64 steps) = the share of the wave doing
+ * integral work; the symmetric J kernel counts one lane per integral, serving both J blocks, at
+ * the bra class's index).  No counterpart in the reference (instrumentation of this library). */
+int lsjk_last_wave_steps(uint64_t* steps1024, uint64_t* lanes1024);
 int lsjk_last_timing(double* ms2);
 
 /* Quartet counts under the reference's screening conventions, no integrals evaluated (the

@@ -47,6 +47,7 @@ device_count = _native.device_count
 set_counting = _native.set_counting
 last_counts = _native.last_counts
 last_counts_fp32 = _native.last_counts_fp32
+last_wave_steps = _native.last_wave_steps
 set_streams = _native.set_streams
 last_timing = _native.last_timing
 synchronize = _native.synchronize

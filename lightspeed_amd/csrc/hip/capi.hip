@@ -68,6 +68,8 @@ struct Session {
     bool counting = false;
     uint64_t counts[1024] = {0};
     uint64_t counts32[1024] = {0};
+    uint64_t wsteps[1024] = {0};
+    uint64_t wlanes[1024] = {0};
     double ms[2] = {0.0, 0.0};
     hipStream_t side[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t fork = nullptr;
@@ -1135,9 +1137,13 @@ void finish_call(bool count, const lsjk::CallScalars* sc)
         HIPCHK(hipMemcpy(&h, sc, sizeof(h), hipMemcpyDeviceToHost));
         std::memcpy(S.counts, h.counts, sizeof(S.counts));
         std::memcpy(S.counts32, h.counts32, sizeof(S.counts32));
+        std::memcpy(S.wsteps, h.wsteps, sizeof(S.wsteps));
+        std::memcpy(S.wlanes, h.wlanes, sizeof(S.wlanes));
     } else {
         std::memset(S.counts, 0, sizeof(S.counts));
         std::memset(S.counts32, 0, sizeof(S.counts32));
+        std::memset(S.wsteps, 0, sizeof(S.wsteps));
+        std::memset(S.wlanes, 0, sizeof(S.wlanes));
     }
 }
 
@@ -1424,7 +1430,7 @@ static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
         commit_ws(wsp, o, who);
         // one list on both sides (FP64 route): the symmetric kernel takes the group pairs
         // js_enabled admits, each integral once for both J blocks
-        const bool jsym = (pl12 == pl34) && !mixed;
+        const bool jsym = (pl12 == pl34);
         const JTaskList* segp = get_jtasks(pl12, pl34, !noalloc && !o.graph);
         if (!segp)
             throw LsjkError(LSJK_E_INVALID, std::string(who) + ": no J task list for this pair of lists (one is built "
@@ -1514,7 +1520,8 @@ static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
                         const JTask* tk = (const JTask*)sseg.d_tasks + sseg.off[L12][L34];
                         const int lid = idx | (1 << 24) | (L12 != L34 ? (jcount_index(L34, L12) + 1) << 12 : 0);
                         lt.begin(lid, ss);
-                        launch_j_quartets_sym(L12, L34, qa, tk, sseg.cnt[L12][L34], &sc->queue[idx], ss);
+                        launch_j_quartets_sym(L12, L34, mixed ? kRouteMixed : kRoutePlain, qa, tk, sseg.cnt[L12][L34],
+                                              &sc->queue[idx], ss);
                         lt.end();
                     } else {
                         const JTask* tk = (const JTask*)seg.d_tasks + seg.off[L12][L34];
@@ -2255,6 +2262,8 @@ void batch_jk(lsjk_batch* b, const double* D_dev, int nterm, const double* scale
         S.kms.clear();
         std::memset(S.counts, 0, sizeof(S.counts));
         std::memset(S.counts32, 0, sizeof(S.counts32));
+        std::memset(S.wsteps, 0, sizeof(S.wsteps));
+        std::memset(S.wlanes, 0, sizeof(S.wlanes));
         return;
     }
     // eager run: the results of this call, and every cache the launches need (task and unit lists,
@@ -3219,6 +3228,14 @@ int lsjk_last_counts_fp32(uint64_t* counts)
     return guarded([&] {
         if (!counts) throw LsjkError(LSJK_E_INVALID, "lsjk_last_counts_fp32: null pointer");
         std::memcpy(counts, S.counts32, sizeof(S.counts32));
+    });
+}
+int lsjk_last_wave_steps(uint64_t* steps, uint64_t* lanes)
+{
+    return guarded([&] {
+        if (!steps || !lanes) throw LsjkError(LSJK_E_INVALID, "lsjk_last_wave_steps: null pointer");
+        std::memcpy(steps, S.wsteps, sizeof(S.wsteps));
+        std::memcpy(lanes, S.wlanes, sizeof(S.wlanes));
     });
 }
 int lsjk_last_timing(double* ms)

@@ -116,7 +116,7 @@ __device__ __forceinline__ Real rrsqrt(Real x)
     if constexpr (std::is_same<Real, float>::value)
         return __frsqrt_rn(x);
     else
-        return rsqrt(x);
+        return rsqrt_pos(x);
 }
 
 template <typename Real, int NR>
@@ -312,6 +312,7 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
     const float thre_bra = thresp / max34;
     const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
     unsigned long long nq = 0, nq32 = 0;
+    unsigned int nsteps = 0;  // wave steps with an evaluated quartet (counting mode)
 
     for (;;) {
         const int t = (int)wave_take(qctr);
@@ -360,6 +361,7 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
             if constexpr (LO) take = take && !(est >= a.thredp);
             bool step32 = LO;
             if constexpr (MIXED) step32 = __ballot(take && est >= a.thredp) == 0ull;
+            if (a.count && __ballot(take) != 0ull) nsteps++;
             if (!take) continue;
             nq++;
             if (step32) nq32++;
@@ -428,37 +430,44 @@ __global__ __launch_bounds__(256) void j_quartets(JQuartetArgs a, const JTask* _
         if (lane == 0 && nq) {
             atomicAdd(&a.sc->counts[a.count_cls], nq);
             if (nq32) atomicAdd(&a.sc->counts32[a.count_cls], nq32);
+            atomicAdd(&a.sc->wsteps[a.count_cls], (unsigned long long)nsteps);
+            atomicAdd(&a.sc->wlanes[a.count_cls], nq);
         }
     }
 }
 
-// Symmetric J (one list on both sides, FP64): each integral (x|y) of bra group L12 and ket group
+// Symmetric J (one list on both sides): each integral (x|y) of bra group L12 and ket group
 // L34 (L12 >= L34; y < x by Hermite position within one group) is evaluated once and used twice,
 // as the reference's two quartets:
 //   A. J_x += (x|y) d_y  when x passes the bra truncation and float(B_x B'_y) >= thresp;
 //   B. J_y += (y|x) d_x  when y passes the bra truncation and float(B_y B'_x) >= thresp
 //      (not for y = x, where it is A again),
 // B' being the truncated (D-weighted) bounds.  Lanes own bra pairs x as in j_quartets; the B
-// contributions of a ket step are summed over the wave and added to J_y by one lane.
-template <int L12, int L34>
+// contributions of a ket step are summed over the wave and added to J_y by one lane.  ROUTE
+// kRoutePlain: FP64; kRouteMixed: a ket step in which no lane's estimate reaches thredp, in either
+// use, is evaluated in FP32 (table, R, contraction; FP64 accumulation), the others in FP64.
+template <int L12, int L34, int ROUTE>
 __global__ __launch_bounds__(256) void j_quartets_sym(JQuartetArgs a, const JTask* __restrict__ tasks, int ntask,
                                                       unsigned int* qctr)
 {
-    using Lds = JLds<L12, L34, kRoutePlain>;
+    using Lds = JLds<L12, L34, ROUTE>;
+    constexpr bool MIXED = ROUTE == kRouteMixed;
     constexpr int H12 = nherm(L12), H34 = nherm(L34), L = L12 + L34, HL = nherm(L), NR = L / 2 + 1;
     constexpr bool SAME = L12 == L34;
     extern __shared__ double s_dyn[];
     double* s_tab64 = s_dyn;
-    if constexpr (Lds::STAGED64) {
-        rys_stage_lds<NR>(a.rys, s_tab64);
-        __syncthreads();
-    }
+    float* s_tab32 = reinterpret_cast<float*>(reinterpret_cast<char*>(s_dyn) + Lds::bytes64(a.rys));
+    if constexpr (Lds::STAGED64) rys_stage_lds<NR>(a.rys, s_tab64);
+    if constexpr (Lds::HAS32) rys32_stage_lds<NR>(a.rys, s_tab32);
+    if constexpr (Lds::STAGED64 || Lds::HAS32) __syncthreads();
     const int lane = threadIdx.x & 63;
     const float thre_lo = a.thresp;
     const float max34 = __uint_as_float(a.sc->max34_bits);
     const float thre_bra = a.thresp / max34;
     const double dmax = __longlong_as_double((long long)a.sc->dmax_bits);
-    unsigned long long nqa = 0, nqb = 0;
+    unsigned long long nqa = 0, nqb = 0, nq32a = 0, nq32b = 0;
+    unsigned long long nlanes = 0;  // integrals evaluated (one serves both uses; counting mode)
+    unsigned int nsteps = 0;
 
     for (;;) {
         const int t = (int)wave_take(qctr);
@@ -508,6 +517,10 @@ __global__ __launch_bounds__(256) void j_quartets_sym(JQuartetArgs a, const JTas
             const bool takeB = active && (By >= thre_bra) && (!SAME || k < x) && By * kbx >= thre_lo;
             const unsigned long long mB = __ballot(takeB);
             if ((__ballot(takeA) | mB) == 0ull) continue;
+            if (a.count) nsteps++;
+            bool step32 = false;
+            if constexpr (MIXED)
+                step32 = __ballot((takeA && Bx * kby >= a.thredp) || (takeB && By * kbx >= a.thredp)) == 0ull;
             const double q = ld_const(a.ket_geom + k);
             const double c34 = ld_const(a.ket_geom + 4 * a.ket_n + k);
             const double X = Px - ld_const(a.ket_geom + a.ket_n + k);
@@ -523,26 +536,44 @@ __global__ __launch_bounds__(256) void j_quartets_sym(JQuartetArgs a, const JTas
             if (takeA || takeB) {
                 nqa += takeA ? 1 : 0;
                 nqb += takeB ? 1 : 0;
-                double R[HL];
-                hermite_R<double, L>(a.rys, s_tab64, a.ew, p12, c12, q, c34, X, Y, Z, R);
-                const double wa = takeA ? 1.0 : 0.0, wb = takeB ? 1.0 : 0.0;
-                static_for<H34>([&](auto U_) {
-                    constexpr int u = decltype(U_)::value;
-                    constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
-                    const double du = wa * dy[u];
-                    double s = 0.0;
-                    static_for<H12>([&](auto T_) {
-                        constexpr int tt = decltype(T_)::value;
-                        constexpr int idx = herm_index(herm_tx(tt) + ux, herm_ty(tt) + uy, herm_tz(tt) + uz);
-                        jp[tt] = fma(R[idx], du, jp[tt]);
-                        // (y|x) at Q - P = -(P - Q): R_{t+u}(-X) = (-1)^{|t|+|u|} R_{t+u}(X)
-                        if constexpr (((herm_tx(tt) + herm_ty(tt) + herm_tz(tt) + ux + uy + uz) & 1) != 0)
-                            s = fma(-R[idx], dx[tt], s);
-                        else
-                            s = fma(R[idx], dx[tt], s);
+                nlanes++;
+                // one integral in the arithmetic Real, both uses; the A part lands in accA
+                auto both = [&](auto* R, auto* accA, const void* tab) __attribute__((always_inline)) {
+                    using Real = typename std::remove_reference<decltype(R[0])>::type;
+                    hermite_R<Real, L>(a.rys, tab, a.ew, p12, c12, q, c34, X, Y, Z,
+                                       *reinterpret_cast<Real(*)[HL]>(R));
+                    const Real wa = takeA ? Real(1) : Real(0), wb = takeB ? Real(1) : Real(0);
+                    static_for<H34>([&](auto U_) {
+                        constexpr int u = decltype(U_)::value;
+                        constexpr int ux = herm_tx(u), uy = herm_ty(u), uz = herm_tz(u);
+                        const Real du = wa * (Real)dy[u];
+                        Real s = Real(0);
+                        static_for<H12>([&](auto T_) {
+                            constexpr int tt = decltype(T_)::value;
+                            constexpr int idx = herm_index(herm_tx(tt) + ux, herm_ty(tt) + uy, herm_tz(tt) + uz);
+                            accA[tt] = fma(R[idx], du, accA[tt]);
+                            // (y|x) at Q - P = -(P - Q): R_{t+u}(-X) = (-1)^{|t|+|u|} R_{t+u}(X)
+                            if constexpr (((herm_tx(tt) + herm_ty(tt) + herm_tz(tt) + ux + uy + uz) & 1) != 0)
+                                s = fma(-R[idx], (Real)dx[tt], s);
+                            else
+                                s = fma(R[idx], (Real)dx[tt], s);
+                        });
+                        jy[u] = (double)(wb * s);
                     });
-                    jy[u] = wb * s;
-                });
+                };
+                if (MIXED && step32) {
+                    float R[HL], ja[H12];
+#pragma unroll
+                    for (int u = 0; u < H12; u++) ja[u] = 0.0f;
+                    both(R, ja, s_tab32);
+#pragma unroll
+                    for (int u = 0; u < H12; u++) jp[u] += (double)ja[u];
+                    nq32a += takeA ? 1 : 0;
+                    nq32b += takeB ? 1 : 0;
+                } else {
+                    double R[HL];
+                    both(R, jp, s_tab64);
+                }
             }
             if (mB != 0ull) {
                 // sum of the B contributions over the wave (all lanes active here: the ket loop
@@ -568,27 +599,47 @@ __global__ __launch_bounds__(256) void j_quartets_sym(JQuartetArgs a, const JTas
         for (int o = 32; o > 0; o >>= 1) {
             nqa += __shfl_xor(nqa, o);
             nqb += __shfl_xor(nqb, o);
+            nlanes += __shfl_xor(nlanes, o);
         }
         if (lane == 0) {
             if (nqa) atomicAdd(&a.sc->counts[a.count_cls], nqa);
             if (nqb) atomicAdd(&a.sc->counts[a.count_cls2], nqb);
+            if (nlanes) {
+                atomicAdd(&a.sc->wsteps[a.count_cls], (unsigned long long)nsteps);
+                atomicAdd(&a.sc->wlanes[a.count_cls], nlanes);
+            }
+        }
+        if constexpr (MIXED) {
+            for (int o = 32; o > 0; o >>= 1) {
+                nq32a += __shfl_xor(nq32a, o);
+                nq32b += __shfl_xor(nq32b, o);
+            }
+            if (lane == 0) {
+                if (nq32a) atomicAdd(&a.sc->counts32[a.count_cls], nq32a);
+                if (nq32b) atomicAdd(&a.sc->counts32[a.count_cls2], nq32b);
+            }
         }
     }
 }
 
-void launch_j_quartets_sym(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask,
+void launch_j_quartets_sym(int L12, int L34, int route, const JQuartetArgs& a, const JTask* tasks, int ntask,
                            unsigned int* qctr, hipStream_t s)
 {
     if (ntask <= 0) return;
+#define ONE(X, Y, R)                                                                                     \
+    {                                                                                                    \
+        const size_t lds = JLds<X, Y, R>::bytes(a.rys);                                                  \
+        const int grid = persistent_grid((const void*)j_quartets_sym<X, Y, R>, 256, lds, ((long long)ntask + 3) / 4); \
+        hipLaunchKernelGGL((j_quartets_sym<X, Y, R>), dim3(grid), dim3(256), lds, s, a, tasks, ntask, qctr); \
+        return;                                                                                          \
+    }
 #define CASE(X, Y)                                                                                       \
     if (L12 == X && L34 == Y) {                                                                          \
-        const size_t lds = JLds<X, Y, kRoutePlain>::bytes(a.rys);                                        \
-        const int grid = persistent_grid((const void*)j_quartets_sym<X, Y>, 256, lds, ((long long)ntask + 3) / 4); \
-        hipLaunchKernelGGL((j_quartets_sym<X, Y>), dim3(grid), dim3(256), lds, s, a, tasks, ntask, qctr); \
-        return;                                                                                          \
+        if (route == kRouteMixed) ONE(X, Y, kRouteMixed) else ONE(X, Y, kRoutePlain)                    \
     }
     CASE(0, 0) CASE(1, 0) CASE(2, 0) CASE(3, 0) CASE(4, 0)
 #undef CASE
+#undef ONE
     throw std::runtime_error("lsjk: j_quartets_sym class not instantiated");
 }
 

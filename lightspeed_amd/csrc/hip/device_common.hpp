@@ -296,6 +296,17 @@ __device__ __forceinline__ void rys32_stage_lds(const RysTable& tab, float* s_co
     }
 }
 
+// 1/sqrt(x) for finite x > 0 (sums of Gaussian exponents): the hardware estimate y refined by one
+// second-order step, 1/sqrt(x) = y (1 - e)^(-1/2) ~ y (1 + e/2 + 3 e^2 / 8) with e = 1 - x y^2.
+// These are the operations the device library's rsqrt performs, without its selection of
+// special values (zero, infinity, denormals), so for such x the two agree bit for bit.
+__device__ __forceinline__ double rsqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 // Load through the constant address space: a wave-uniform address becomes a scalar (SMEM) load,
 // a divergent one an ordinary vector load.  Only for data no kernel of the launch writes.
 template <typename T>

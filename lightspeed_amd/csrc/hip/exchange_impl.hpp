@@ -174,7 +174,7 @@ __device__ __forceinline__ Real krsqrt(Real x)
     if constexpr (std::is_same<Real, float>::value)
         return __frsqrt_rn(x);
     else
-        return rsqrt(x);
+        return rsqrt_pos(x);
 }
 
 // One primitive quartet: acc[a][c] += sum_{b,d} (ab|cd) Db[b][d], evaluated in the route's
@@ -300,7 +300,7 @@ __device__ __forceinline__ void k_quartet_grad(const RysTable& tab, const Ewald&
     constexpr int NR = (L1 + L2 + L3 + L4 + 1) / 2 + 1;
     const double p = e12.p, q = e34.p;
     const double pq = p + q;
-    const double rpq = rsqrt(pq);
+    const double rpq = rsqrt_pos(pq);
     const double ipq = rpq * rpq;
     const double rho = p * q * ipq;
     const double PQx = e12.Px - e34.Px, PQy = e12.Py - e34.Py, PQz = e12.Pz - e34.Pz;
@@ -568,9 +568,11 @@ __device__ __forceinline__ KEnt<float> load_ent32_uniform(const double* __restri
     return e;
 }
 
-// LDS of one k_ket workgroup: kKWaves group accumulators (G x N1 x N3 doubles each), then
-// the staged table blocks: FP64 (routes with FP64 quartets, nrys <= 3), then FP32 (routes with
-// FP32 quartets, nrys <= 5).
+// LDS of one k_ket workgroup: the staged table blocks first, FP64 (routes with FP64 quartets,
+// nrys <= 3) then FP32 (routes with FP32 quartets, nrys <= 5), then kKWaves group accumulators
+// (G x N1 x N3 doubles each).  The FP64 table at address 0 keeps a bin's coefficient offsets
+// (0..8 nrys * 24 bytes) inside the immediate offset field of the paired LDS reads, so the table
+// lookup needs one address per bin.
 template <int L1, int L2, int L3, int L4, int ROUTE>
 struct KKetLds {
     static constexpr int NR = (L1 + L2 + L3 + L4) / 2 + 1;
@@ -584,12 +586,13 @@ struct KKetLds {
     {
         return STAGED64 ? rys_lds_doubles(tab.nbin[NR], NR) * sizeof(double) : 0;
     }
-    static size_t bytes(const RysTable& tab)
+    // byte offset of the accumulators: after both tables, 8-byte aligned
+    static __host__ __device__ size_t acc_offset(const RysTable& tab)
     {
-        size_t b = (size_t)kKWaves * ACC * sizeof(double) + bytes64(tab);
-        if (STAGED32) b += rys32_lds_floats(tab.nbin[NR], NR) * sizeof(float);
-        return b;
+        const size_t b = bytes64(tab) + (STAGED32 ? rys32_lds_floats(tab.nbin[NR], NR) * sizeof(float) : 0);
+        return (b + 7) & ~(size_t)7;
     }
+    static size_t bytes(const RysTable& tab) { return acc_offset(tab) + (size_t)kKWaves * ACC * sizeof(double); }
 };
 
 // An FP64 entry rounded to the FP32 route's arithmetic (the values of its FP32 record).
@@ -656,9 +659,9 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
     extern __shared__ double s_dyn[];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    double* s_acc = s_dyn + w * Lds::ACC;
-    double* s_tab64 = s_dyn + kKWaves * Lds::ACC;
-    float* s_tab32 = reinterpret_cast<float*>(reinterpret_cast<char*>(s_tab64) + Lds::bytes64(a.rys));
+    double* s_tab64 = s_dyn;
+    float* s_tab32 = reinterpret_cast<float*>(reinterpret_cast<char*>(s_dyn) + Lds::bytes64(a.rys));
+    double* s_acc = reinterpret_cast<double*>(reinterpret_cast<char*>(s_dyn) + Lds::acc_offset(a.rys)) + w * Lds::ACC;
     if constexpr (Lds::STAGED64) rys_stage_lds<NR>(a.rys, s_tab64);
     if constexpr (Lds::STAGED32) rys32_stage_lds<NR>(a.rys, s_tab32);
     for (int q = lane; q < Lds::ACC; q += 64) s_acc[q] = 0.0;
@@ -694,6 +697,7 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
     const float b12top = ld_const(a.s12.eb + a0);
 
     unsigned long long nq = 0, nq32 = 0;
+    unsigned int nsteps = 0;  // wave steps with an evaluated quartet (counting mode)
     bool any = false;
     for (int c0 = s0; c0 < s1; c0 += 64) {
         const float b34max = ld_const(kg.chunk_b + (c0 >> 6));
@@ -748,6 +752,7 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
             }
             const unsigned long long m = __ballot(pass);
             if (m == 0ull) continue;
+            if (a.count) nsteps++;
             if constexpr (ROUTE == kRoutePlain || ROUTE == kRouteHi) {
                 const KEnt<double> E12 = load_ent_uniform(a.s12.eg, k);
                 if (pass) {
@@ -825,6 +830,8 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
         if (lane == 0 && nq) {
             atomicAdd(&a.sc->counts[kcount_index(L1, L2, L3, L4)], nq);
             if (nq32) atomicAdd(&a.sc->counts32[kcount_index(L1, L2, L3, L4)], nq32);
+            atomicAdd(&a.sc->wsteps[kcount_index(L1, L2, L3, L4)], (unsigned long long)nsteps);
+            atomicAdd(&a.sc->wlanes[kcount_index(L1, L2, L3, L4)], nq);
         }
     }
 }

@@ -182,6 +182,8 @@ struct CallScalars {
     unsigned long long dmax_bits;  // max |D| (double bits)
     unsigned long long counts[1024]; // evaluated primitive quartets (see jcount_index / kcount_index)
     unsigned long long counts32[1024]; // the part of them evaluated on the FP32 route
+    unsigned long long wsteps[1024];   // wave steps that evaluated at least one integral
+    unsigned long long wlanes[1024];   // lanes that evaluated an integral, summed over those steps
     unsigned int queue[1024];        // work-queue counters of the persistent kernels
 };
 

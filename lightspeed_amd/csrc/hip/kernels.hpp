@@ -272,7 +272,7 @@ void launch_j_bra_scatter(int LA, int LB, const JScatterArgs& a, int k0, int nk,
 // reduce over the wave per ket); with p or d kets the per-ket reduction costs what it saves
 // (measured again with the DPP reduction: (2, 1) 1.21 ms against 0.60 + 0.55 ms one-way).
 __host__ __device__ constexpr bool js_enabled(int L12, int L34) { return L12 >= L34 && L34 == 0; }
-void launch_j_quartets_sym(int L12, int L34, const JQuartetArgs& a, const JTask* tasks, int ntask,
+void launch_j_quartets_sym(int L12, int L34, int route, const JQuartetArgs& a, const JTask* tasks, int ntask,
                            unsigned int* qctr, hipStream_t s);
 void launch_j_quartets(int L12, int L34, int route, const JQuartetArgs& a, const JTask* tasks, int ntask,
                        unsigned int* qctr, hipStream_t s);

@@ -340,6 +340,11 @@ PYBIND11_MODULE(_lightspeed, m)
         check(lsjk_last_counts_fp32((uint64_t*)a.mutable_data()));
         return a;
     });
+    m.def("last_wave_steps", []() {
+        py::array_t<uint64_t> s(1024), l(1024);
+        check(lsjk_last_wave_steps((uint64_t*)s.mutable_data(), (uint64_t*)l.mutable_data()));
+        return py::make_tuple(s, l);
+    });
     m.def("last_timing", []() {
         double ms[2];
         check(lsjk_last_timing(ms));

@@ -4,7 +4,7 @@ into SCF energy errors.
 It follows the structure of the reference's RHF (est/rhf.py:596-723; the coverage test
 tests/coverage/rhf.py:607-646 quotes the H2O/STO-3G energy -74.9420799265908784): one-electron
 Hamiltonian H = T + V from the nuclear charges, symmetric orthogonalisation S^-1/2, a core-Hamiltonian
-guess, Fock matrix F = H + 2 J[D] - K[D] with D = C_occ C_occ^T, energy
+guess (or a given starting density, e.g. `fragment_guess`'s superposition of fragment densities), Fock matrix F = H + 2 J[D] - K[D] with D = C_occ C_occ^T, energy
 E = tr[D (H + F)] + E_nuc, and DIIS extrapolation on the commutator FDS - SDF (Pulay, Chem. Phys.
 Lett. 73, 393 (1980)).  Dense linear algebra runs on the host (numpy).
 
@@ -85,7 +85,8 @@ class RHF:
     """Closed-shell RHF.  After run(): energy, converged, iterations, C, eps, D, history."""
 
     def __init__(self, molecule, integrals, charge=0, thresp=1.0e-14, thredp=1.0e-14, maxiter=100, e_conv=1.0e-12,
-                 g_conv=1.0e-9, diis_max=8, fixed_iterations=None, log=None):
+                 g_conv=1.0e-9, diis_max=8, fixed_iterations=None, log=None, D0=None, level_shift=0.0,
+                 diis_start=None, damping=0.0):
         self.molecule = molecule
         self.ints = integrals
         nel = int(round(sum(a.Z for a in molecule.atoms))) - charge
@@ -97,6 +98,14 @@ class RHF:
         self.fixed_iterations = fixed_iterations
         self.history = []
         self.log = log  # optional callable: one progress line per iteration
+        self.D0 = D0  # optional starting density (nao x nao); None = core-Hamiltonian guess
+        # level_shift sigma > 0 raises the virtual orbitals by sigma (F + sigma (S - S D S), Saunders &
+        # Hillier, Int. J. Quantum Chem. 7, 699 (1973)) while max|FDS - SDF| > diis_start; below it
+        # (or from the first iteration when diis_start is None) DIIS extrapolates without the shift
+        self.level_shift, self.diis_start = level_shift, diis_start
+        # damping a in [0, 1): while DIIS is off, the next density is (1 - a) D_new + a D (the
+        # remedy for charge sloshing between the fragments of a cluster started from the core guess)
+        self.damping = damping
 
     def run(self):
         xyzZ = self.molecule.xyzZ
@@ -113,6 +122,10 @@ class RHF:
 
         eps, C = diagonalise(H)
         D = C[:, :self.nocc] @ C[:, :self.nocc].T
+        if self.D0 is not None:
+            if self.D0.shape != D.shape:
+                raise ValueError("RHF: D0 has shape %s, the basis needs %s" % (self.D0.shape, D.shape))
+            D = np.array(self.D0, dtype=float)
         Fs, Es = [], []
         E_old = 0.0
         self.converged = False
@@ -137,7 +150,10 @@ class RHF:
             if len(Fs) > self.diis_max:
                 Fs.pop(0)
                 Es.pop(0)
-            if len(Fs) >= 2:
+            use_diis = self.diis_start is None or gnorm <= self.diis_start
+            if not use_diis and self.level_shift > 0.0:
+                F = F + self.level_shift * (S - S @ D @ S)
+            elif len(Fs) >= 2:
                 n = len(Fs)
                 B = -np.ones((n + 1, n + 1))
                 B[n, n] = 0.0
@@ -152,9 +168,47 @@ class RHF:
                 except np.linalg.LinAlgError:
                     pass
             eps, C = diagonalise(F)
-            D = C[:, :self.nocc] @ C[:, :self.nocc].T
+            Dn = C[:, :self.nocc] @ C[:, :self.nocc].T
+            D = Dn if use_diis or self.damping == 0.0 else (1.0 - self.damping) * Dn + self.damping * D
         self.energy = self.history[-1][1]
         self.iterations = len(self.history)
         self.C, self.eps, self.D = C, eps, D
         return self.energy
 
+
+
+def fragment_guess(molecule, natom_fragment, make_integrals, thre=1.0e-12, e_conv=1.0e-10, g_conv=1.0e-7,
+                   maxiter=60):
+    """Starting density for a molecule made of neutral closed-shell fragments of `natom_fragment`
+    consecutive atoms each (a water cluster: O, H, H per water): D = diag(D_1, ..., D_n) with D_m the
+    converged RHF density of fragment m alone (its own nuclei and electrons).
+
+    The basis functions of fragment m are a consecutive block of the molecule's, because the basis
+    loader emits the shells atom by atom in atom order.  `make_integrals(fragment)` returns an
+    `Integrals`-like object for the fragment in the molecule's basis set.
+
+    For a large cluster the core-Hamiltonian guess puts the electrons in the unscreened potential of
+    every nucleus and DIIS from there wanders for many iterations; this guess starts one interaction
+    energy away from the answer."""
+    from .loaders import Molecule
+    if molecule.natom % natom_fragment:
+        raise ValueError("fragment_guess: %d atoms is not a whole number of %d-atom fragments" %
+                         (molecule.natom, natom_fragment))
+    blocks = []
+    for a0 in range(0, molecule.natom, natom_fragment):
+        atoms = [a._replace(atomIdx=i) for i, a in enumerate(molecule.atoms[a0:a0 + natom_fragment])]
+        frag = Molecule("%s_fragment_%d" % (molecule.name, a0 // natom_fragment), atoms)
+        hf = RHF(frag, make_integrals(frag), thresp=thre, thredp=thre, e_conv=e_conv, g_conv=g_conv,
+                 maxiter=maxiter)
+        hf.run()
+        if not hf.converged:
+            raise RuntimeError("fragment_guess: fragment %d did not converge" % (a0 // natom_fragment))
+        blocks.append(hf.D)
+    n = sum(b.shape[0] for b in blocks)
+    D = np.zeros((n, n))
+    o = 0
+    for b in blocks:
+        k = b.shape[0]
+        D[o:o + k, o:o + k] = b
+        o += k
+    return D
