@@ -356,8 +356,9 @@ void build_pairlist(lsjk_pairlist* pl)
     const int ne = (int)ents.size();
     pl->nent = ne;
     std::vector<float> eb(ne);
-    std::vector<int> eprim(ne), ecart(ne), eblk(ne);
-    // packed entries (kKEntDoubles per entry, see KEnt): p, K, P, P - A, A - B, {cart, blk}
+    std::vector<int> eprim(ne), ecart(ne), eblk(ne), keblk(ne);
+    // packed entries (kKEntDoubles per entry, see KEnt): p, K, P, P - A, A - B, {cart, blk}, 1/(2p),
+    // {cart, blk} in the partner basis's K view order
     std::vector<double> eg((size_t)kKEntDoubles * (size_t)std::max(ne, 1), 0.0);
     std::vector<double> eg32((size_t)kKEnt32Doubles * (size_t)std::max(ne, 1), 0.0);
     for (int q = 0; q < ne; q++) {
@@ -388,6 +389,9 @@ void build_pairlist(lsjk_pairlist* pl)
         const int ids[2] = {ecart[q], eblk[q]};
         std::memcpy(&g[11], ids, sizeof(ids));
         g[12] = 0.5 * ip;
+        keblk[q] = b2->h_kblk[eblk[q]];
+        const int kids[2] = {b2->h_kcart[ecart[q]], keblk[q]};
+        std::memcpy(&g[13], kids, sizeof(kids));
         // FP32-route copy: P in FP64, the rest rounded to FP32
         double* h = eg32.data() + (size_t)kKEnt32Doubles * q;
         h[0] = g[2];
@@ -397,6 +401,7 @@ void build_pairlist(lsjk_pairlist* pl)
                              (float)g[8], (float)g[9], (float)g[10], (float)g[12], 0.0f};
         std::memcpy(&h[3], f, sizeof(f));
         std::memcpy(&h[8], ids, sizeof(ids));
+        std::memcpy(&h[9], kids, sizeof(kids));
     }
     for (int L = 0; L <= kMaxL; L++)
         for (int Lp = 0; Lp <= kMaxL; Lp++) pl->h_rows[L][Lp].clear();
@@ -440,6 +445,7 @@ void build_pairlist(lsjk_pairlist* pl)
     pl->d_eprim = dev_upload(eprim);
     pl->d_ecart = dev_upload(ecart);
     pl->d_eblk = dev_upload(eblk);
+    pl->d_keblk = dev_upload(keblk);
     pl->d_egeom = dev_upload(eg);
     pl->d_egeom32 = dev_upload(eg32);
     for (int L = 0; L <= kMaxL; L++)
@@ -490,6 +496,7 @@ void free_pairlist(lsjk_pairlist* pl)
     dev_free(pl->d_eprim);
     dev_free(pl->d_ecart);
     dev_free(pl->d_eblk);
+    dev_free(pl->d_keblk);
     dev_free(pl->d_egeom);
     dev_free(pl->d_egeom32);
     for (int L = 0; L <= lsjk::kMaxL; L++)
@@ -1584,12 +1591,75 @@ void basis_free(lsjk_basis* b)
     dev_free(b->d_blk_L);
     dev_free(b->d_blk_lo);
     dev_free(b->d_blk_hi);
+    dev_free(b->d_kcart);
+    dev_free(b->d_kblk);
     delete b;
+}
+
+// K view order of a basis's blocks (Basis::kperm): by angular momentum, then by the Morton code
+// of the shell centre (10 bits per axis over the basis's bounding box), then by block index; the
+// components of a block stay contiguous.  The identity when kview is false or when the blocks do
+// not tile [0, ncart).
+void set_kview(lsjk::Basis* b, bool kview)
+{
+    const int nblk = b->nblk, ncart = b->ncart;
+    b->h_kblk.resize(nblk);
+    b->h_kcart.resize(ncart);
+    for (int k = 0; k < nblk; k++) b->h_kblk[k] = k;
+    for (int i = 0; i < ncart; i++) b->h_kcart[i] = i;
+    b->kperm = false;
+    if (!kview || nblk == 0) return;
+    std::vector<int> cover(ncart, 0);
+    for (int k = 0; k < nblk; k++)
+        for (int c = 0; c < lsjk::ncart(b->h_blk_L[k]); c++) cover[b->h_blk_cart[k] + c]++;
+    for (int i = 0; i < ncart; i++)
+        if (cover[i] != 1) return;
+    std::vector<int> first(nblk, -1);
+    for (int i = 0; i < b->nprim; i++)
+        if (first[b->h_blk[i]] < 0) first[b->h_blk[i]] = i;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < b->nprim; i++)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], b->h_xyz[3 * (size_t)i + d]);
+            hi[d] = std::max(hi[d], b->h_xyz[3 * (size_t)i + d]);
+        }
+    auto spread = [](unsigned v) {  // 10 bits -> every third bit
+        unsigned long long x = v & 1023u, r = 0;
+        for (int t = 0; t < 10; t++) r |= ((x >> t) & 1ull) << (3 * t);
+        return r;
+    };
+    std::vector<unsigned long long> code(nblk);
+    for (int k = 0; k < nblk; k++) {
+        unsigned long long m = 0;
+        for (int d = 0; d < 3; d++) {
+            const double w = hi[d] - lo[d];
+            const double t = w > 0.0 ? (b->h_xyz[3 * (size_t)first[k] + d] - lo[d]) / w : 0.0;
+            m |= spread((unsigned)std::min(1023.0, std::max(0.0, t * 1023.0))) << d;
+        }
+        code[k] = m;
+    }
+    std::vector<int> order(nblk);
+    for (int k = 0; k < nblk; k++) order[k] = k;
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        if (b->h_blk_L[x] != b->h_blk_L[y]) return b->h_blk_L[x] < b->h_blk_L[y];
+        if (code[x] != code[y]) return code[x] < code[y];
+        return x < y;
+    });
+    int pos = 0;
+    for (int r = 0; r < nblk; r++) {
+        const int k = order[r];
+        b->h_kblk[k] = r;
+        for (int c = 0; c < lsjk::ncart(b->h_blk_L[k]); c++) b->h_kcart[b->h_blk_cart[k] + c] = pos + c;
+        pos += lsjk::ncart(b->h_blk_L[k]);
+    }
+    for (int i = 0; i < ncart; i++) b->kperm = b->kperm || b->h_kcart[i] != i;
+    b->d_kcart = dev_upload(b->h_kcart);
+    b->d_kblk = dev_upload(b->h_kblk);
 }
 
 // A device basis from primitive arrays (lsjk_basis_create; also the combined basis of a batch).
 lsjk_basis* basis_new(int nprim, const double* c, const double* e, const double* xyz, const int* L, const int* cart,
-                      int ncart)
+                      int ncart, bool kview = true)
 {
     lsjk_basis* b = new lsjk_basis();
     try {
@@ -1636,6 +1706,7 @@ lsjk_basis* basis_new(int nprim, const double* c, const double* e, const double*
         b->d_blk = dev_upload(b->h_blk);
         b->d_blk_cart = dev_upload(b->h_blk_cart);
         b->d_blk_L = dev_upload(b->h_blk_L);
+        set_kview(b, kview);
     } catch (...) {
         basis_free(b);
         throw;
@@ -1679,8 +1750,16 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
         }
         double* Dsh = nullptr;
         double* Ks = nullptr;
+        // K view copies of D24 and of its block maxima (Basis::kperm): the kernel gathers from them
+        double* Dk = nullptr;
+        double* Dshk = nullptr;
+        const bool kview = b2->kperm || b4->kperm;
         Workspace wsp;
         wsp.add(&Dsh, blk_elems(b2, b4));
+        if (kview) {
+            wsp.add(&Dk, mat_elems(b2, b4));
+            wsp.add(&Dshk, blk_elems(b2, b4));
+        }
         if (tune) wsp.add(&Ks, mat_elems(b1, b3));
         commit_ws(wsp, o, who);
         const bool timed = !o.graph;
@@ -1689,6 +1768,15 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
         launch_fill(S.d_sc, sizeof(CallScalars), st);
         launch_k_blockmax(D24_dev, mat_ld(b4), b2->d_blk_cart, b2->d_blk_L, b2->nblk, b4->d_blk_cart, b4->d_blk_L,
                           b4->nblk, Dsh, blk_ld(b4), b2->d_blk_lo, b2->d_blk_hi, b2->ldblk, S.d_sc, st);
+        if (kview) {
+            launch_permute(D24_dev, mat_ld(b4), b2->ncart, b4->ncart, b2->kperm ? b2->d_kcart : nullptr,
+                           b4->kperm ? b4->d_kcart : nullptr, Dk, mat_ld(b4), st);
+            launch_permute(Dsh, blk_ld(b4), b2->nblk, b4->nblk, b2->kperm ? b2->d_kblk : nullptr,
+                           b4->kperm ? b4->d_kblk : nullptr, Dshk, blk_ld(b4), st);
+        } else {
+            Dk = const_cast<double*>(D24_dev);
+            Dshk = Dsh;
+        }
         HIPCHK(hipGetLastError());
         if (timed) HIPCHK(hipEventRecord(S.ev[1], st));
         S.kid.clear();
@@ -1733,7 +1821,7 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
                     sd.maxLp = pl->kmaxL2;
                     sd.eb = pl->d_ebound;
                     sd.ecart = pl->d_ecart;
-                    sd.eblk = pl->d_eblk;
+                    sd.eblk = pl->d_keblk;  // the K view order of Dshk
                     sd.eg = pl->d_egeom;
                     sd.eg32 = pl->d_egeom32;
                     sd.nent = pl->nent;
@@ -1744,9 +1832,10 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
                     sd.z = pl->b1->d_z;
                     sd.cart = pl->b1->d_cart;
                 }
-                ka.D = D24_dev;
+                // k_ket gathers D and the block maxima at the entries' K view ids
+                ka.D = Dk;
                 ka.ldD = mat_ld(b4);
-                ka.Dsh = Dsh;
+                ka.Dsh = Dshk;
                 ka.ldDsh = blk_ld(b4);
                 ka.K = K;
                 ka.ldK = mat_ld(b3);
@@ -2178,7 +2267,7 @@ lsjk_batch* batch_new(int nmol, const lsjk_pairlist* const* lists)
         prim0[nmol] = (int)L.size();
         if ((double)std::max(nc, 1) * (double)(b->ld + 1) > 4.0e18)
             throw LsjkError(LSJK_E_UNSUPPORTED, "lsjk_batch_create: batch too large");
-        b->basis = basis_new((int)L.size(), c.data(), e.data(), xyz.data(), L.data(), cart.data(), nc);
+        b->basis = basis_new((int)L.size(), c.data(), e.data(), xyz.data(), L.data(), cart.data(), nc, false);
         Basis* bb = b->basis;
         bb->ld = std::max(b->ld, 1);
         bb->h_mol = mol;

@@ -49,6 +49,18 @@ __global__ void k_blockmax(const double* D, int ldD, const int* bc2, const int* 
     atomicMax(&sc->dmax_bits, (unsigned long long)__double_as_longlong(m));
 }
 
+// out[pr[i] * ldo + pc[j]] = in[i * ldi + j] for i < nr, j < nc (pr, pc permutations; null: the
+// identity on that side).  One thread per element, rows of `in` read coalesced.
+__global__ void k_permute(const double* __restrict__ in, int ldi, int nr, int nc, const int* __restrict__ pr,
+                          const int* __restrict__ pc, double* __restrict__ out, int ldo)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nr * nc) return;
+    const int i = (int)(t / nc), j = (int)(t - (long long)i * nc);
+    const int oi = pr ? pr[i] : i, oj = pc ? pc[j] : j;
+    out[(size_t)oi * ldo + oj] = in[(size_t)i * ldi + j];
+}
+
 __global__ void fill_zero(unsigned int* p, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -90,6 +102,15 @@ void launch_k_blockmax(const double* D, int ldD, const int* bc2, const int* bL2,
     if (n == 0) return;
     hipLaunchKernelGGL(k_blockmax, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, D, ldD, bc2, bL2, nb2, bc4,
                        bL4, nb4, Dsh, ldDsh, blo, bhi, band, sc);
+}
+
+void launch_permute(const double* in, int ldi, int nr, int nc, const int* pr, const int* pc, double* out, int ldo,
+                    hipStream_t s)
+{
+    const long long n = (long long)nr * nc;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_permute, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, ldi, nr, nc, pr, pc, out,
+                       ldo);
 }
 
 void launch_k_quartets(int L1, int L2, int L3, int L4, int route, const KArgs& a, const KUnit* units, int nunits,

@@ -111,7 +111,9 @@ struct KEnt {
     int cart, blk;
 };
 
-// One packed FP64 entry: a 96-byte, 16-byte aligned record (p, K, P, P - A, A - B, {cart, blk}).
+// One packed FP64 entry: a 128-byte, 16-byte aligned record (p, K, P, P - A, A - B, {cart, blk}, 1/(2p),
+// K view {cart, blk}); KV: take the K view ids.
+template <bool KV = false>
 __device__ __forceinline__ KEnt<double> load_ent(const double* __restrict__ eg, int i)
 {
     const double* g = static_cast<const double*>(__builtin_assume_aligned(eg + (size_t)i * kKEntDoubles, 16));
@@ -127,7 +129,7 @@ __device__ __forceinline__ KEnt<double> load_ent(const double* __restrict__ eg, 
     e.ABx = g[8];
     e.ABy = g[9];
     e.ABz = g[10];
-    const int* ids = reinterpret_cast<const int*>(g + 11);
+    const int* ids = reinterpret_cast<const int*>(g + (KV ? 13 : 11));
     e.cart = ids[0];
     e.blk = ids[1];
     e.i2p = g[12];
@@ -135,7 +137,8 @@ __device__ __forceinline__ KEnt<double> load_ent(const double* __restrict__ eg, 
 }
 
 // One packed FP32-route entry: an 80-byte record (P as 3 doubles; p, Kp, P - A, A - B, 1/(2p) as
-// floats; {cart, blk}).
+// floats; {cart, blk}; K view {cart, blk}); KV: take the K view ids.
+template <bool KV = false>
 __device__ __forceinline__ KEnt<float> load_ent32(const double* __restrict__ eg32, int i)
 {
     const double* g = static_cast<const double*>(__builtin_assume_aligned(eg32 + (size_t)i * kKEnt32Doubles, 16));
@@ -153,7 +156,7 @@ __device__ __forceinline__ KEnt<float> load_ent32(const double* __restrict__ eg3
     e.ABy = f[6];
     e.ABz = f[7];
     e.i2p = f[8];
-    const int* ids = reinterpret_cast<const int*>(g + 8);
+    const int* ids = reinterpret_cast<const int*>(g + (KV ? 9 : 8));
     e.cart = ids[0];
     e.blk = ids[1];
     return e;
@@ -521,7 +524,7 @@ __global__ __launch_bounds__(64) void k_tiles_grad(KArgs a, const KUnit* __restr
     if (lane < qn) evaluate(lane);
 }
 
-// One packed FP64 entry at a wave-uniform index, through scalar loads.
+// One packed FP64 entry at a wave-uniform index, through scalar loads (K view ids).
 __device__ __forceinline__ KEnt<double> load_ent_uniform(const double* __restrict__ eg, int i)
 {
     const double* g = eg + (size_t)i * kKEntDoubles;
@@ -537,14 +540,14 @@ __device__ __forceinline__ KEnt<double> load_ent_uniform(const double* __restric
     e.ABx = ld_const(g + 8);
     e.ABy = ld_const(g + 9);
     e.ABz = ld_const(g + 10);
-    const int* ids = reinterpret_cast<const int*>(g + 11);
+    const int* ids = reinterpret_cast<const int*>(g + 13);
     e.cart = ld_const(ids);
     e.blk = ld_const(ids + 1);
     e.i2p = ld_const(g + 12);
     return e;
 }
 
-// One packed FP32-route entry at a wave-uniform index, through scalar loads.
+// One packed FP32-route entry at a wave-uniform index, through scalar loads (K view ids).
 __device__ __forceinline__ KEnt<float> load_ent32_uniform(const double* __restrict__ eg32, int i)
 {
     const double* g = eg32 + (size_t)i * kKEnt32Doubles;
@@ -562,7 +565,7 @@ __device__ __forceinline__ KEnt<float> load_ent32_uniform(const double* __restri
     e.ABy = ld_const(f + 6);
     e.ABz = ld_const(f + 7);
     e.i2p = ld_const(f + 8);
-    const int* ids = reinterpret_cast<const int*>(g + 8);
+    const int* ids = reinterpret_cast<const int*>(g + 9);
     e.cart = ld_const(ids);
     e.blk = ld_const(ids + 1);
     return e;
@@ -711,9 +714,9 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
         KEnt<double> E34;
         KEnt<float> E34f;
         if constexpr (LO) {
-            E34f = load_ent32(a.s34.eg32, e4);
+            E34f = load_ent32<true>(a.s34.eg32, e4);
         } else {
-            E34 = load_ent(a.s34.eg, e4);
+            E34 = load_ent<true>(a.s34.eg, e4);
             if constexpr (SPLIT) E34f = ent_to_f32(E34);
         }
         const int cart4 = LO ? E34f.cart : E34.cart;

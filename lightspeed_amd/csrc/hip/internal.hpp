@@ -32,6 +32,14 @@ struct Basis {
     std::vector<int> h_mol;           // molecule of each primitive (batch basis)
     std::vector<int> h_blk_lo, h_blk_hi;  // per block: its molecule's block range (batch basis)
     int *d_blk_lo = nullptr, *d_blk_hi = nullptr;
+    // K view order: the exchange kernel reads D and its block maxima from copies whose blocks are
+    // ordered by angular momentum, then spatially (Morton order of the shell centres), so that the
+    // partners of a group of nearby rows are neighbouring columns.  h_kcart[i] = position of
+    // Cartesian function i, h_kblk[k] = position of block k; kperm = false: the identity (batch
+    // bases, whose matrices are banded per molecule).
+    bool kperm = false;
+    std::vector<int> h_kcart, h_kblk;
+    int *d_kcart = nullptr, *d_kblk = nullptr;
 };
 
 // Row stride of the Cartesian matrices and of the block-maximum matrices over a basis.
@@ -82,6 +90,7 @@ struct PairList {
     int* d_eprim = nullptr;   // partner primitive
     int* d_ecart = nullptr;   // partner Cartesian offset
     int* d_eblk = nullptr;    // partner block id
+    int* d_keblk = nullptr;   // partner block id in the K view order (Basis::h_kblk)
     double* d_egeom = nullptr; // packed entries (12 doubles each: p, K, P, P-A, A-B, {cart, blk})
     double* d_egeom32 = nullptr; // FP32-route entries (64 bytes each: P in FP64, 8 floats, {cart, blk})
     // pivot view for counting (rows = basis-2 primitives, partners = basis-1 primitives, bounds

@@ -172,13 +172,14 @@ struct KUnitG {
     int i1, g, rank1, crange, prange;
 };
 // A packed partner entry (row primitive A, partner primitive B), 128 bytes: p, Kp, P (3),
-// P - A (3), A - B (3), {the partner's Cartesian offset, its block id} (two ints), 1/(2p), 3 zero
-// pads.  Kp = K_AB sqrt(2 pi^(5/2)) / p with K_AB = c_A c_B exp(-a b / p |A - B|^2).
+// P - A (3), A - B (3), {the partner's Cartesian offset, its block id} (two ints), 1/(2p), the
+// same two ids in the partner basis's K view order (Basis::h_kcart, h_kblk), 2 zero pads.
+// Kp = K_AB sqrt(2 pi^(5/2)) / p with K_AB = c_A c_B exp(-a b / p |A - B|^2).
 constexpr int kKEntDoubles = 16;
 // sqrt(2 pi^(5/2))
 constexpr double kKPrefRoot = 5.914967172795613;
 // The FP32-route copy, 80 bytes: P (3 doubles), p, Kp, P - A, A - B, 1/(2p), 0 (10 floats),
-// {cart, blk}, pad.
+// {cart, blk}, {cart, blk} in the K view order.
 constexpr int kKEnt32Doubles = 10;
 struct KSide {
     const int* row_start;  // [nrow * (maxLp+1) + 1]
@@ -289,6 +290,9 @@ int persistent_grid(const void* kernel, int block, size_t lds, long long nwork_g
 
 // blk_lo / blk_hi (nullptr: dense): per block 2, the range of blocks 4 of its molecule (batch
 // basis); band = the largest such range.
+// out[pr[i] * ldo + pc[j]] = in[i * ldi + j] (the K view copies of D and of its block maxima).
+void launch_permute(const double* in, int ldi, int nr, int nc, const int* pr, const int* pc, double* out, int ldo,
+                    hipStream_t s);
 void launch_k_blockmax(const double* D, int ldD, const int* blk_cart2, const int* blk_L2, int nblk2,
                        const int* blk_cart4, const int* blk_L4, int nblk4, double* Dsh, int ldDsh, const int* blk_lo,
                        const int* blk_hi, int band, CallScalars* sc, hipStream_t s);
