@@ -775,6 +775,11 @@ const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp)
 
 // Work units per K class below which the groups' chunk ranges are split.
 constexpr long long kKUnitsTargetDefault = 12288;
+// Unit tiles inside an XCD's range (get_kunits_g): bra rows x ket groups.  Measured on (H2O)256 /
+// 6-31G* (K 247 -> 199 ms serialised against cost order alone) and (H2O)64 / 6-31G** (no loss);
+// 64 x 2 .. whole band x 2 measured within 2 % of each other.
+constexpr int kKTileRows = 256;
+constexpr int kKTileGroups = 2;
 constexpr int kcount_index_host(int L1, int L2, int L3, int L4) { return 256 + ((L1 * 5 + L2) * 5 + L3) * 5 + L4; }
 // thresp values whose K unit lists are cached per pair of lists (older ones are dropped)
 constexpr int kKThrespCached = 4;
@@ -785,8 +790,9 @@ constexpr int kKThrespCached = 4;
 // test with |D| <= 1 (an ordering heuristic only; the kernel screens with the actual D).  Units
 // are dispatched in coarse cost buckets, heaviest first; inside a bucket in (spatial row, group)
 // order, split into 8 contiguous ranges that are interleaved by workgroup, so that with the
-// round-robin workgroup-to-XCD dispatch each XCD (one L2) works through neighbouring bra rows.
-// (Group-major and 2-D tiled orders were measured as well: no less L2 fill traffic, no faster.)
+// round-robin workgroup-to-XCD dispatch each XCD (one L2) works through neighbouring bra rows, in
+// tiles of rows x groups (below).  The tiles pay off with the K view copy of D (neighbouring
+// shells in neighbouring columns); in the basis order they measured no faster.
 std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, const lsjk_pairlist* pl34c, int L1,
                                                  int L2, int L3, int L4, bool tri, float thresp, double* cost_out)
 {
@@ -935,11 +941,16 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
             return bucket(a.first) > bucket(b.first);
         });
         // Workgroup w (units kKWaves w .. kKWaves w + kKWaves - 1) runs on XCD w % 8.  Each bucket is
-        // cut into 8 contiguous ranges; workgroups take kKWaves consecutive units of range w % 8, so
-        // an XCD stays on neighbouring rows.  Inside a range the units are sorted by cost, so the
-        // waves of one workgroup (which holds its slots until its last wave ends) cost about the
-        // same.  Units that do not fill a whole workgroup of their range go at the bucket's end.
+        // cut into 8 contiguous ranges (bands of neighbouring bra rows); workgroups take kKWaves
+        // consecutive units of range w % 8.  Inside a range the units go in tiles of kKTileRows
+        // rows x kKTileGroups ket groups (spatial order both ways), so that an XCD's L2 holds the
+        // tile's block of the K view D: rows = the tile's bra partners, columns = the groups'
+        // partners.  Inside a tile the units are sorted by cost, so the waves of one workgroup
+        // (which holds its slots until its last wave ends) cost about the same.  Units that do not
+        // fill a whole workgroup of their range go at the bucket's end.
         constexpr size_t W = lsjk::kKWaves;
+        std::vector<int> pos1(rows1.size(), 0);  // spatial position of each bra row
+        for (size_t q = 0; q < G12.order.size(); q++) pos1[G12.order[q]] = (int)q;
         std::vector<KUnitG> u;
         u.reserve(v.size());
         for (size_t s0 = 0; s0 < v.size();) {
@@ -949,8 +960,14 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
             for (size_t x = 0; x < 8; x++) {
                 const size_t b = std::min(n, x * per), e = std::min(n, (x + 1) * per);
                 std::stable_sort(v.begin() + s0 + b, v.begin() + s0 + e,
-                                 [](const std::pair<long long, KUnitG>& a, const std::pair<long long, KUnitG>& c) {
-                                     return a.first > c.first;
+                                 [&](const std::pair<long long, KUnitG>& a, const std::pair<long long, KUnitG>& c) {
+                                     const int ra = pos1[a.second.i1] / kKTileRows, rc = pos1[c.second.i1] / kKTileRows;
+                                     if (ra != rc) return ra < rc;
+                                     const int ga = a.second.g / kKTileGroups, gc = c.second.g / kKTileGroups;
+                                     if (ga != gc) return ga < gc;
+                                     if (a.first != c.first) return a.first > c.first;
+                                     if (pos1[a.second.i1] != pos1[c.second.i1]) return pos1[a.second.i1] < pos1[c.second.i1];
+                                     return a.second.g < c.second.g;
                                  });
             }
             std::vector<KUnitG> rest;
