@@ -17,8 +17,9 @@ Every route's J and K are compared with the FP64 route (whole matrices, and the 
 E2 = 2 tr(DJ) - tr(DK)) and, on the CPU-baseline leg's bra-pair sample, with the FP64 oracle.  The
 headline `value` is the FP64 route unless another route meets the accuracy bar (max|dJ|, max|dK|
 <= 1e-7, |dE2| <= 1e-6) and is at least 5 % faster (the run-to-run spread is about 2 %).  Before the
-timed region the autotuner picks the J ket chunks and K unit splits on this workload (the default
-parameters' FP64 time is reported beside it; --no-autotune skips it).
+timed region the autotuner picks the J ket chunks and K unit splits on this workload, then for the
+mixed routes the faster of the mixed and the FP64 kernel per launch (the default parameters' FP64
+time is reported beside it; --no-autotune skips it).
 
 value = ERI quartets/s = (N_J + N_K) / t_JK with N counted under the reference's screening
 conventions (lsjk_count_quartets: the plain FP64 route's quartets, for every route), t_JK the wall
@@ -42,7 +43,8 @@ The sample's quartet count comes from the CPU FP64 oracle run on the same sample
 equal the reference's evaluated quartets); where the reference build is absent the oracle's own
 time is reported (kind "port").
 Other workloads (config.other_workloads; --quick skips them): C20H42/cc-pVDZ and (H2O)256/6-31G*
-(FP64 and mixed ms per step, quartets/s, jk_frac, reference CPU rate on a bra-pair sample), and the
+(autotuned as above; FP64 and mixed ms per step, quartets/s, jk_frac, reference CPU rate on a
+bra-pair sample), and the
 batched workload, 256 seeded (H2O)n, n = 1..16, 6-31G*, all J and K builds in one graph replay
 (molecules/s, FP64 with default and autotuned parameters, and mixed).
 
@@ -202,11 +204,19 @@ class Workload:
         return t
 
 
-def measure_extra(ls, W, name, steps, cpu_stride, thredp):
+def measure_extra(ls, W, name, steps, cpu_stride, thredp, tune=True):
     """FP64 and mixed routes of another single-molecule workload: ms per step, quartets/s, the
-    whole-step roofline fraction, and the reference CPU rate on a bra-pair sample."""
+    whole-step roofline fraction, and the reference CPU rate on a bra-pair sample.  With tune, the
+    autotuner runs first (FP64 launch shapes, then the mixed route's kernel per launch)."""
     wl = Workload(ls, W, name, 1.0e-12)
     routes = {"fp64": (wl.thresp, None), "mixed": (thredp, None)}
+    t_tune = None
+    if tune:
+        t0 = time.perf_counter()
+        ls.autotune(wl.pl, wl.ew, wl.dD.ptr, wl.thresp, wl.thresp)
+        ls.autotune(wl.pl, wl.ew, wl.dD.ptr, wl.thresp, thredp)
+        t_tune = time.perf_counter() - t0
+        log("%s autotune: %.1f s" % (name, t_tune))
     res = {}
     for r in routes:
         wl.dJ.zero()
@@ -221,7 +231,7 @@ def measure_extra(ls, W, name, steps, cpu_stride, thredp):
            "fp64_ms_per_step": 1000 * timing["fp64"], "fp64_quartets_per_s": wl.quartets / timing["fp64"],
            "fp64_jk_frac": wl.ideal_time() / timing["fp64"],
            "mixed_ms_per_step": 1000 * timing["mixed"], "mixed_quartets_per_s": wl.quartets / timing["mixed"],
-           "mixed_max_abs_dJ_vs_fp64": dJ, "mixed_max_abs_dK_vs_fp64": dK}
+           "mixed_max_abs_dJ_vs_fp64": dJ, "mixed_max_abs_dK_vs_fp64": dK, "autotune_s": t_tune}
     if cpu_stride:
         log("%s: cpu baseline (sample stride %d)" % (name, cpu_stride))
         cb, _ = cpu_baseline(wl.bas, wl.D, cpu_stride, wl.thresp)
@@ -261,6 +271,7 @@ def measure_batch(ls, W, nmol, steps, thredp, tune=True):
         if name == "fp64" and tune:
             t0 = time.perf_counter()
             table = batch.autotune(ew, dD.ptr, 1e-12, 1e-12)
+            batch.autotune(ew, dD.ptr, 1e-12, thredp)  # the mixed route's kernel per launch
             out["autotune_s"] = time.perf_counter() - t0
             out["autotune_changed"] = sum(1 for k, i, v, _, _ in table if (k < 2 and v != 1.0) or (k == 2 and v != 12288))
             log("batch autotune: %.1f s, %d of %d entries changed" % (out["autotune_s"], out["autotune_changed"], len(table)))
@@ -405,10 +416,16 @@ def main():
         t_def = wl.time_routes({"fp64_default": routes["fp64"]}, args.steps, args.warmup)["fp64_default"]
         t0 = time.perf_counter()
         table = ls.autotune(pl, wl.ew, wl.dD.ptr, thresp, thresp)
+        # the mixed route: per launch, its mixed kernel or the FP64 kernel, whichever is faster
+        table_mixed = ls.autotune(pl, wl.ew, wl.dD.ptr, thresp, args.thredp)
+        n_plain = sum(1 for k, i, v, _, _ in table_mixed if k == 3 and v == 1.0)
+        log("autotune, mixed route: %d of %d launches on the FP64 kernel" % (
+            n_plain, sum(1 for k, i, v, _, _ in table_mixed if k == 3)))
         t_tune = time.perf_counter() - t0
         changed = [(k, i, v) for k, i, v, _, _ in table if (k < 2 and v != 1.0) or (k == 2 and v != 12288)]
         log("autotune: %.1f s, %d entries, %d changed" % (t_tune, len(table), len(changed)))
         tune_info = {"seconds": t_tune, "entries": len(table), "changed": len(changed),
+                     "mixed_route_fp64_kernel_launches": n_plain,
                      "fp64_default_ms_per_step": 1000 * t_def,
                      "table": [[int(k), int(i), float(v), float(a), float(b)] for k, i, v, a, b in table]}
     timing = wl.time_routes(routes, args.steps, args.warmup)
@@ -604,9 +621,11 @@ def main():
     if not args.quick:
         others = {}
         others["c20h42_ccpvdz"] = measure_extra(ls, W, "c20h42_ccpvdz", 3,
-                                                0 if args.no_cpu_baseline else 16, args.thredp)
+                                                0 if args.no_cpu_baseline else 16, args.thredp,
+                                                not args.no_autotune)
         others["h2o256_631gs"] = measure_extra(ls, W, "h2o256_631gs", 2,
-                                               0 if args.no_cpu_baseline else 4096, args.thredp)
+                                               0 if args.no_cpu_baseline else 4096, args.thredp,
+                                               not args.no_autotune)
         others["water_batch_256"] = measure_batch(ls, W, 256, 2, args.thredp, not args.no_autotune)
         config["other_workloads"] = others
     print(json.dumps(out), flush=True)

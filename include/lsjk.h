@@ -85,8 +85,9 @@ int lsjk_pairlist_destroy(lsjk_pairlist* pl);
  * Precision (intbox.hpp:736-742): with thredp <= thresp every quartet with est >= thresp is FP64;
  * with thredp > thresp quartets with est >= thredp are FP64 and those with thresp <= est < thredp
  * are evaluated in FP32 -- except that quartets evaluated together with an FP64 one in the same
- * wave step are evaluated in FP64 as well (never less precision than asked for); accumulation is
- * FP64 throughout.  lsjk_last_counts_fp32 reports how many ran in FP32. */
+ * wave step are evaluated in FP64 as well, and so are all quartets of the launches that
+ * lsjk_autotune found faster with the FP64 kernel (never less precision than asked for);
+ * accumulation is FP64 throughout.  lsjk_last_counts_fp32 reports how many ran in FP32. */
 int lsjk_coulomb(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, const double* D34_dev, int nterm,
                  const double* scales, const double* omegas, float thresp, float thredp, double* J12_dev);
 
@@ -206,9 +207,13 @@ int lsjk_exchange_screened(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34,
  * task scaled by 1/4, 1/2, 1, 2 or 4, and the K launches with the work-unit split target of every
  * class at 3072, 12288 or 49152 units (class launches serialised on one stream, HIP events around
  * each, best of two), and keeps the fastest choice per J group pair and per K class for later
- * builds of the list on both sides (the FP64 and mixed routes share the J choices).
- * lsjk_tuned_params: the choices, n entries: kind (0 J one-way, 1 J symmetric, 2 K), counter index
- * (J: L12*5 + L34; K: as lsjk_last_counts), value (chunk scale or split target), and the launch's ms
+ * builds of the list on both sides (the FP64 and mixed routes share these choices).  Called with a
+ * mixed route (thredp > thresp), it instead times each launch of that route with its mixed kernel
+ * and with the FP64 kernel (the same quartets, all in FP64) and keeps the faster per launch, and
+ * times the route's mirror-class orientations; call it after the FP64 route's tuning.
+ * lsjk_tuned_params: the choices, n entries: kind (0 J one-way, 1 J symmetric, 2 K, 3 mixed-route
+ * kernel), counter index (J: L12*5 + L34, bit 24 for the symmetric kernel; K: as lsjk_last_counts),
+ * value (chunk scale, split target, or 1 = FP64 kernel / 0 = mixed kernel), and the launch's ms
  * with the default and with the choice.
  * lsjk_batch_autotune: the same on a batch's combined list (D_dev in the batch layout); drops the
  * batch's captured graphs.  lsjk_batch_pairlist: the batch's combined list (for lsjk_tuned_params). */

@@ -1533,6 +1533,11 @@ static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
                     qa.thredp = thredp;
                     qa.sc = sc;
                     int idx = jcount_index(L12, L34);
+                    // the mixed route's kernel, or the FP64 one where the autotuner found it faster
+                    auto jroute = [&](int lid) {
+                        const bool plain = pl12 == pl34 && (pl12->force_plain || pl12->plain_on_mixed.count(lid));
+                        return mixed && !plain ? kRouteMixed : kRoutePlain;
+                    };
                     hipStream_t ss = fo.pick();
                     if (sym) {
                         // the bra group's densities and the ket group's accumulators, for the
@@ -1544,14 +1549,12 @@ static void coulomb_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, c
                         const JTask* tk = (const JTask*)sseg.d_tasks + sseg.off[L12][L34];
                         const int lid = idx | (1 << 24) | (L12 != L34 ? (jcount_index(L34, L12) + 1) << 12 : 0);
                         lt.begin(lid, ss);
-                        launch_j_quartets_sym(L12, L34, mixed ? kRouteMixed : kRoutePlain, qa, tk, sseg.cnt[L12][L34],
-                                              &sc->queue[idx], ss);
+                        launch_j_quartets_sym(L12, L34, jroute(lid), qa, tk, sseg.cnt[L12][L34], &sc->queue[idx], ss);
                         lt.end();
                     } else {
                         const JTask* tk = (const JTask*)seg.d_tasks + seg.off[L12][L34];
                         lt.begin(idx, ss);
-                        launch_j_quartets(L12, L34, mixed ? kRouteMixed : kRoutePlain, qa, tk, seg.cnt[L12][L34],
-                                          &sc->queue[idx], ss);
+                        launch_j_quartets(L12, L34, jroute(idx), qa, tk, seg.cnt[L12][L34], &sc->queue[idx], ss);
                         lt.end();
                     }
                 }
@@ -1865,6 +1868,10 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
                 const double G = (double)(L1 + L2 + 1) * (L3 + L4 + 1);
                 const double F = 40.0 + (Lt / 2 + 1) * (30.0 + 9.0 * G + 3.0 * Pi) + 2.0 * Pi;
                 int route = mixed ? kRouteMixed : kRoutePlain;
+                // the FP64 kernel where the autotuner found it faster on the mixed route
+                if (mixed && pl12 == pl34 &&
+                    (pl12->force_plain || pl12->plain_on_mixed.count(kcount_index(L1, L2, L3, L4))))
+                    route = kRoutePlain;
                 if (o.screen) {
                     // learned route: certified classes (s and p functions) take their units'
                     // factors; every other quartet is evaluated in FP64 under the plain test
@@ -2055,6 +2062,12 @@ void autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const double* s
     HIPCHK(hipMalloc((void**)&dJ, n * sizeof(double)));
     HIPCHK(hipMalloc((void**)&dK, n * sizeof(double)));
     S.counting = false;
+    struct ForcePlain {
+        lsjk_pairlist* pl;
+        ~ForcePlain() { pl->force_plain = false; }
+    } force_plain_reset{pl};
+    const bool mixed = thredp > thresp;
+    if (mixed) pl->plain_on_mixed.clear();
     // mirror-class orientations: timed (both, twice each) and stored for this route; then warm
     // ket groups, task and unit lists of the defaults
     {
@@ -2082,6 +2095,29 @@ void autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const double* s
         }
         return ms;
     };
+    if (mixed) {
+        // The mixed route: per launch, the mixed kernel against the FP64 kernel (the same quartets,
+        // all in FP64), on the launch shapes the FP64 route was tuned to; the FP64 one is kept
+        // where it is faster (kind 3 in the table: value 1 = FP64 kernel)
+        pl->tuned.erase(std::remove_if(pl->tuned.begin(), pl->tuned.end(),
+                                       [](const lsjk_pairlist::Tuned& t) { return t.kind == 3; }),
+                        pl->tuned.end());
+        std::map<int, double> mj = launches(true), mk = launches(false);
+        pl->force_plain = true;
+        std::map<int, double> pj = launches(true), pk = launches(false);
+        pl->force_plain = false;
+        const std::pair<std::map<int, double>*, std::map<int, double>*> sides[2] = {{&mj, &pj}, {&mk, &pk}};
+        for (const auto& pr : sides) {
+            for (auto& kv : *pr.first) {
+                auto it = pr.second->find(kv.first);
+                if (it == pr.second->end()) continue;
+                const bool plain = it->second < kv.second;
+                if (plain) pl->plain_on_mixed.insert(kv.first);
+                pl->tuned.push_back({3, kv.first, plain ? 1.0 : 0.0, kv.second, plain ? it->second : kv.second});
+            }
+        }
+        return;
+    }
     pl->tuned.clear();
     // J: one candidate scale for every group pair at a time
     std::map<int, std::map<float, double>> jt;  // launch id -> scale -> ms

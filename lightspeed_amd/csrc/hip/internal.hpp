@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <map>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -121,6 +122,11 @@ struct PairList {
     // default chunk), K unit-split target per class counter index (absent: the default)
     float jscale[2][5][5];
     std::map<int, long long> ksplit;
+    // mixed route (thredp > thresp), this list on both sides: the launches (J and K launch ids, as
+    // timed by lsjk_autotune) that run the FP64 kernel instead, because it measured faster (the
+    // same quartets, all in FP64); force_plain: every launch, while the autotuner times them
+    std::set<int> plain_on_mixed;
+    bool force_plain = false;
     // generation of the K unit lists (raised whenever cached unit lists are dropped), so that data
     // cached per unit list elsewhere can tell a rebuilt list from the old one
     unsigned long long kunits_gen = 0;

@@ -67,3 +67,40 @@ def test_autotune_batch(product_backend):
     for a, b, c, d in zip(Js, Js0, Ks, Ks0):
         assert np.abs(np.array(a.np) - np.array(b.np)).max() < 1e-12
         assert np.abs(np.array(c.np) - np.array(d.np)).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_autotune_mixed_route_kernels(product_backend):
+    """lsjk_autotune on a mixed route: per launch, the mixed or the FP64 kernel, whichever timed
+    faster (kind 3 entries, value 1 = FP64 kernel).  Results stay within the mixed route's bar of
+    the FP64 build, and the K launches moved to the FP64 kernel evaluate no quartet in FP32."""
+    ls = product_backend.ls
+    from lightspeed_amd import workloads as W
+    mol = W.water_cluster(8, 9.0, 43)
+    bas = ls.Basis.from_gbs_file(mol, "6-31gss")
+    D = W.random_density(bas.nao, 40, 44)
+    pl = ls.PairList.build_schwarz(bas, bas, True, THRE)
+    res = ls.ResourceList.build()
+    ew = ls.Ewald([1.0], [-1.0])
+    Dt = ls.Tensor.array(D)
+    J0 = np.array(ls.IntBox.coulomb(res, ew, pl, pl, Dt, THRE, THRE).np)
+    K0 = np.array(ls.IntBox.exchange(res, ew, pl, pl, Dt, True, THRE, THRE).np)
+    dD = ls.DeviceArray([bas.ncart, bas.ncart])
+    dD.upload(D)
+    ls.autotune(pl, ew, dD.ptr, THRE, THRE)
+    table = ls.autotune(pl, ew, dD.ptr, THRE, 1e-6)
+    routes = [(i, v, m0, m1) for k, i, v, m0, m1 in table if k == 3]
+    print("mixed-route launches: %d, on the FP64 kernel: %d" % (len(routes), sum(1 for r in routes if r[1] == 1.0)),
+          flush=True)
+    assert routes
+    for i, v, m0, m1 in routes:
+        assert v in (0.0, 1.0) and m1 <= m0 + 1e-9
+    ls.set_counting(True)
+    J = np.array(ls.IntBox.coulomb(res, ew, pl, pl, Dt, THRE, 1e-6).np)
+    K = np.array(ls.IntBox.exchange(res, ew, pl, pl, Dt, True, THRE, 1e-6).np)
+    c32 = ls.last_counts_fp32()
+    ls.set_counting(False)
+    assert np.abs(J - J0).max() <= 1e-7 and np.abs(K - K0).max() <= 1e-7
+    for i, v, _, _ in routes:
+        if v == 1.0 and 256 <= i < 256 + 625:
+            assert c32[i] == 0, i
