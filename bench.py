@@ -15,8 +15,9 @@ Routes, each timed over `--steps` steps after `--warmup`:
     (lightspeed_amd/data/screen_h2o_631gss.npz, or --screen) at thredp = thresp / `--thredp`.
 Every route's J and K are compared with the FP64 route (whole matrices, and the two-electron energy
 E2 = 2 tr(DJ) - tr(DK)) and, on the CPU-baseline leg's bra-pair sample, with the FP64 oracle.  The
-headline `value` is the FP64 route unless another route meets the accuracy bar (max|dJ|, max|dK|
-<= 1e-7, |dE2| <= 1e-6) and is at least 5 % faster (the run-to-run spread is about 2 %).  Before the
+headline `value` is the mixed route when it meets the accuracy bar (max|dJ|, max|dK| <= 1e-7,
+|dE2| <= 1e-6; the metric compares it with the FP64 route), else the FP64 route; `roofline` is the
+headline route's and `roofline_fp64_route` the FP64 route's when they differ.  Before the
 timed region the autotuner picks the J ket chunks and K unit splits on this workload, then for the
 mixed routes the faster of the mixed and the FP64 kernel per launch (the default parameters' FP64
 time is reported beside it; --no-autotune skips it).
@@ -273,7 +274,7 @@ def measure_batch(ls, W, nmol, steps, thredp, tune=True):
             table = batch.autotune(ew, dD.ptr, 1e-12, 1e-12)
             batch.autotune(ew, dD.ptr, 1e-12, thredp)  # the mixed route's kernel per launch
             out["autotune_s"] = time.perf_counter() - t0
-            out["autotune_changed"] = sum(1 for k, i, v, _, _ in table if (k < 2 and v != 1.0) or (k == 2 and v != 12288))
+            out["autotune_changed"] = sum(1 for k, i, v, a, b in table if (k < 2 and v != 1.0) or (k == 2 and v != 12288) or (k == 4 and b < a))
             log("batch autotune: %.1f s, %d of %d entries changed" % (out["autotune_s"], out["autotune_changed"], len(table)))
         t0 = time.perf_counter()
         batch.jk_device(ew, dD.ptr, 1e-12, td, dJ.ptr, dK.ptr)  # eager run + graph capture
@@ -422,7 +423,7 @@ def main():
         log("autotune, mixed route: %d of %d launches on the FP64 kernel" % (
             n_plain, sum(1 for k, i, v, _, _ in table_mixed if k == 3)))
         t_tune = time.perf_counter() - t0
-        changed = [(k, i, v) for k, i, v, _, _ in table if (k < 2 and v != 1.0) or (k == 2 and v != 12288)]
+        changed = [(k, i, v) for k, i, v, a, b in table if (k < 2 and v != 1.0) or (k == 2 and v != 12288) or (k == 4 and b < a)]
         log("autotune: %.1f s, %d entries, %d changed" % (t_tune, len(table), len(changed)))
         tune_info = {"seconds": t_tune, "entries": len(table), "changed": len(changed),
                      "mixed_route_fp64_kernel_launches": n_plain,
@@ -439,160 +440,166 @@ def main():
             ok = ok and a["max_abs_dJ_vs_oracle_sample"] <= 1e-7 and a["max_abs_dK_vs_oracle_sample"] <= 1e-7
         return ok
 
-    # (another route takes the headline only when it is at least 5 % faster: run-to-run spread of
-    # one route on one box is about 2 %)
-    head = min((r for r in routes if r == "fp64" or meets(r)),
-               key=lambda r: timing[r] if r == "fp64" else timing[r] * 1.05)
+    # the headline is the metric's subject, the mixed route, when it meets the accuracy bar (else
+    # the FP64 route); the FP64 route's time and roofline are reported beside it
+    head = "mixed" if "mixed" in routes and meets("mixed") else "fp64"
     t_step = timing[head]
 
-    # kernel attribution: the headline route with the class launches serialised on one stream, so
-    # that each launch's HIP-event duration is its own (under concurrency launches share the device)
-    ls.set_streams(1)
-    ls.set_profiling(True)
-    launch_ms = {}
-    for _ in range(args.attribution_steps):
-        wl.dJ.zero()
-        wl.dK.zero()
-        thredp, scr = routes[head]
-        ls.IntBox.coulomb_device(wl.ew, pl, pl, wl.dD.ptr, thresp, thresp if scr else thredp, wl.dJ.ptr)
-        for (i, ms) in ls.last_launches():
-            launch_ms.setdefault(("J", i), []).append(ms)
-        if scr:
-            scr.exchange_device(wl.ew, pl, pl, wl.dD.ptr, True, thresp, thredp, wl.dK.ptr)
-        else:
-            ls.IntBox.exchange_device(wl.ew, pl, pl, wl.dD.ptr, True, thresp, thredp, wl.dK.ptr)
-        for (i, ms) in ls.last_launches():
-            launch_ms.setdefault(("K", i), []).append(ms)
-    ls.set_profiling(False)
-    ls.set_streams(4)
+    def attribute(rname, t_route, table_path):
+        """Per-launch attribution and the roofline of route rname (see the module docstring)."""
+        # kernel attribution: the route with the class launches serialised on one stream, so that
+        # each launch's HIP-event duration is its own (under concurrency launches share the device)
+        ls.set_streams(1)
+        ls.set_profiling(True)
+        launch_ms = {}
+        for _ in range(args.attribution_steps):
+            wl.dJ.zero()
+            wl.dK.zero()
+            thredp, scr = routes[rname]
+            ls.IntBox.coulomb_device(wl.ew, pl, pl, wl.dD.ptr, thresp, thresp if scr else thredp, wl.dJ.ptr)
+            for (i, ms) in ls.last_launches():
+                launch_ms.setdefault(("J", i), []).append(ms)
+            if scr:
+                scr.exchange_device(wl.ew, pl, pl, wl.dD.ptr, True, thresp, thredp, wl.dK.ptr)
+            else:
+                ls.IntBox.exchange_device(wl.ew, pl, pl, wl.dD.ptr, True, thresp, thredp, wl.dK.ptr)
+            for (i, ms) in ls.last_launches():
+                launch_ms.setdefault(("K", i), []).append(ms)
+        ls.set_profiling(False)
+        ls.set_streams(4)
 
-    def cls_flops(c):
-        return float(flops_per_quartet(*cls_of(c)))
+        def cls_flops(c):
+            return float(flops_per_quartet(*cls_of(c)))
 
-    cj, cj32, ck, ck32 = cnt[head]
-    j_mixed = routes[head][0] > thresp and routes[head][1] is None
+        cj, cj32, ck, ck32 = cnt[rname]
+        j_mixed = routes[rname][0] > thresp and routes[rname][1] is None
 
-    def launch_work(kind, lid):
-        """(N, F-weighted FP64 FLOPs, FP32 FLOPs, evaluated quartets) of one launch: N and the FLOPs
-        under the reference's conventions (lsjk_count_quartets), the FP32 part in the proportion the
-        launch's kernels evaluated in FP32."""
-        if kind == "J":
-            # a symmetric J launch (bit 24) evaluates two group pairs (the second id, + 1, in bits 12..23)
-            N = f64 = f32s = 0.0
-            ev = 0
-            for i in (lid & 0xfff, ((lid >> 12) & 0xfff) - 1):
-                if i < 0 or i >= 25:
-                    continue
-                L12, L34 = i // 5, i % 5
-                cs = [c for c in range(625) if sum(cls_of(c)[:2]) == L12 and sum(cls_of(c)[2:]) == L34]
-                n_ = float(sum(NJ[c] for c in cs))
-                full = sum(NJ[c] * cls_flops(c) for c in cs)
-                f32 = cj32[i] / cj[i] if cj[i] else 0.0
-                N += n_
-                f64 += full * (1.0 - f32)
-                f32s += full * f32
-                ev += int(cj[i])
-            return N, f64, f32s, ev
-        # a K launch of a class with a distinct mirror class evaluates both (symmetric mode)
-        c = lid - 256
-        m = mirror(lid)
-        N = float(NK[c] + (NK[m - 256] if m != lid else 0))
-        # the counting call ran before the autotuner, which may have turned the mirror pair around:
-        # its counts sit at one of the two indices
-        n_ev = int(ck[lid] + (ck[m] if m != lid else 0))
-        n_32 = int(ck32[lid] + (ck32[m] if m != lid else 0))
-        f32 = n_32 / n_ev if n_ev else 0.0
-        return N, N * cls_flops(c) * (1.0 - f32), N * cls_flops(c) * f32, n_ev
-
-    def launch_name(kind, lid):
-        # template arguments as the profiler prints them: route 0 plain FP64, 4 mixed (FP64 and FP32
-        # parts in one launch)
-        if kind == "J":
-            if (lid >> 24) & 1:
-                return "j_quartets_sym<%d, %d>" % ((lid & 0xfff) // 5, (lid & 0xfff) % 5)
-            return "j_quartets<%d, %d, %d>" % (lid // 5, lid % 5, 4 if j_mixed else 0)
-        m = mirror(lid)
-        k_route = 4 if (routes[head][0] > thresp and ck32[lid] + (ck32[m] if m != lid else 0) > 0) else 0
-        return "k_ket<%d, %d, %d, %d, %d>" % (cls_of(lid - 256) + (k_route,))
-
-    def lane_use(kind, lid):
-        """Share of the wave's lanes evaluating an integral in the launch's busy wave steps."""
-        wsj, wlj, wsk, wlk = occ[head]
-        if kind == "J":
-            i = lid & 0xfff
-            steps, lanes = wsj[i], wlj[i]
-        else:
+        def launch_work(kind, lid):
+            """(N, F-weighted FP64 FLOPs, FP32 FLOPs, evaluated quartets) of one launch: N and the FLOPs
+            under the reference's conventions (lsjk_count_quartets), the FP32 part in the proportion the
+            launch's kernels evaluated in FP32."""
+            if kind == "J":
+                # a symmetric J launch (bit 24) evaluates two group pairs (the second id, + 1, in bits 12..23)
+                N = f64 = f32s = 0.0
+                ev = 0
+                for i in (lid & 0xfff, ((lid >> 12) & 0xfff) - 1):
+                    if i < 0 or i >= 25:
+                        continue
+                    L12, L34 = i // 5, i % 5
+                    cs = [c for c in range(625) if sum(cls_of(c)[:2]) == L12 and sum(cls_of(c)[2:]) == L34]
+                    n_ = float(sum(NJ[c] for c in cs))
+                    full = sum(NJ[c] * cls_flops(c) for c in cs)
+                    f32 = cj32[i] / cj[i] if cj[i] else 0.0
+                    N += n_
+                    f64 += full * (1.0 - f32)
+                    f32s += full * f32
+                    ev += int(cj[i])
+                return N, f64, f32s, ev
+            # a K launch of a class with a distinct mirror class evaluates both (symmetric mode)
+            c = lid - 256
             m = mirror(lid)
-            steps = wsk[lid] + (wsk[m] if m != lid else 0)
-            lanes = wlk[lid] + (wlk[m] if m != lid else 0)
-        return float(lanes) / (64.0 * steps) if steps else float("nan")
+            N = float(NK[c] + (NK[m - 256] if m != lid else 0))
+            # the counting call ran before the autotuner, which may have turned the mirror pair around:
+            # its counts sit at one of the two indices
+            n_ev = int(ck[lid] + (ck[m] if m != lid else 0))
+            n_32 = int(ck32[lid] + (ck32[m] if m != lid else 0))
+            f32 = n_32 / n_ev if n_ev else 0.0
+            return N, N * cls_flops(c) * (1.0 - f32), N * cls_flops(c) * f32, n_ev
 
-    means = {k: float(np.mean(v)) for k, v in launch_ms.items()}
-    log("serialised quartet kernels per step: J %.3f ms, K %.3f ms" % (
-        sum(v for k, v in means.items() if k[0] == "J"), sum(v for k, v in means.items() if k[0] == "K")))
-    rows = []
-    for k in sorted(means, key=lambda k: -means[k]):
-        N, f64, f32, ev = launch_work(*k)
-        fl = f64 + f32
-        pk = fl / (f64 / FP64_PEAK + f32 / FP32_PEAK) if fl > 0 else FP64_PEAK
-        rows.append((launch_name(*k), N, fl / max(N, 1.0), ev, means[k], fl / (means[k] * 1e-3) / 1e12,
-                     fl / (means[k] * 1e-3) / pk, lane_use(*k)))
-        log("  launch %-36s %9.3f ms  %8.3f TFLOP/s" % (rows[-1][0], means[k], rows[-1][5]))
-    if args.table:
-        with open(args.table, "w") as f:
-            f.write("| launch | N (reference convention) | FLOP/quartet | evaluated | ms | TFLOP/s | frac "
-                    "| lane use |\n")
-            f.write("|---|---:|---:|---:|---:|---:|---:|---:|\n")
-            for r in rows:
-                f.write("| %s | %d | %.0f | %d | %.3f | %.2f | %.3f | %.3f |\n" % r)
-    dom = max(means, key=lambda k: means[k])
-    Nd, dom64, dom32, _ = launch_work(*dom)
-    dom_fl = dom64 + dom32
-    # peak of the launch's mix: FP64 FLOPs at the FP64 peak, FP32 FLOPs at the FP32 peak
-    peak = dom_fl / (dom64 / FP64_PEAK + dom32 / FP32_PEAK) if dom_fl > 0 else FP64_PEAK
-    achieved = dom_fl / (means[dom] * 1e-3)
-    # whole-step fractions: algorithmic (reference counts) and executed (the kernels' counters)
-    n32 = np.zeros(625)
-    for c in range(625):
-        n32[c] = float(ck32[256 + c]) / max(1, ck[256 + c]) * NK[c] if ck[256 + c] else 0.0
-    for idx in range(25):
-        if cj[idx]:
-            f32 = cj32[idx] / cj[idx]
-            for c in range(625):
-                L1, L2, L3, L4 = cls_of(c)
-                if (L1 + L2) * 5 + (L3 + L4) == idx:
-                    n32[c] += f32 * NJ[c]
-    t_exec = 0.0
-    for idx in range(25):
-        if cj[idx]:
-            L12, L34 = idx // 5, idx % 5
-            cs = [c for c in range(625) if sum(cls_of(c)[:2]) == L12 and sum(cls_of(c)[2:]) == L34 and NJ[c]]
-            if cs:
-                Fm = sum(NJ[c] * cls_flops(c) for c in cs) / max(1, sum(NJ[c] for c in cs))
-                t_exec += (cj[idx] - cj32[idx]) * Fm / FP64_PEAK + cj32[idx] * Fm / FP32_PEAK
-    for c in range(625):
-        if ck[256 + c]:
-            t_exec += (ck[256 + c] - ck32[256 + c]) * cls_flops(c) / FP64_PEAK + ck32[256 + c] * cls_flops(c) / FP32_PEAK
-    roofline = {"bound": "valu", "pipe": "%s vector ALU (MI355X peak; the ERI core is not a dense contraction, "
-                                          "no MFMA)" % ("fp32+fp64" if dom32 > 0 else "fp64"),
-                "kernel": launch_name(*dom), "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "launch_ms": means[dom], "flops_per_launch": float(dom_fl),
-                "quartets_per_launch": float(Nd),
-                "flops_source": "lsjk_count_quartets (reference convention); a K launch of a class with a distinct "
-                                "mirror class counts N_K(class) + N_K(mirror)",
-                "launch_timing": "HIP events around each launch on its stream, class launches serialised "
-                                 "(%d steps after the timed region)" % args.attribution_steps,
-                "jk_frac": wl.ideal_time(n32) / t_step,
-                "jk_exec_frac": t_exec / t_step}
-    # HBM bytes per launch of the dominant kernel, from the committed counter summary of this
-    # workload (two rocprofv3 passes over `bench.py --pmc-pass`, calibrated on copy_bytes)
-    traffic_md = os.path.join(ROOT, "profiles", "jk_%s_hbm_traffic.md" % args.workload)
-    if os.path.exists(traffic_md):
-        for line in open(traffic_md):
-            cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) >= 5 and cells[0] == roofline["kernel"]:
-                roofline["traffic"] = float(cells[4]) * 1e6
-                roofline["traffic_source"] = os.path.relpath(traffic_md, ROOT)
+        def launch_name(kind, lid):
+            # template arguments as the profiler prints them: route 0 plain FP64, 4 mixed (FP64 and FP32
+            # parts in one launch)
+            if kind == "J":
+                if (lid >> 24) & 1:
+                    return "j_quartets_sym<%d, %d>" % ((lid & 0xfff) // 5, (lid & 0xfff) % 5)
+                return "j_quartets<%d, %d, %d>" % (lid // 5, lid % 5, 4 if j_mixed else 0)
+            m = mirror(lid)
+            k_route = 4 if (routes[rname][0] > thresp and ck32[lid] + (ck32[m] if m != lid else 0) > 0) else 0
+            return "k_ket<%d, %d, %d, %d, %d>" % (cls_of(lid - 256) + (k_route,))
+
+        def lane_use(kind, lid):
+            """Share of the wave's lanes evaluating an integral in the launch's busy wave steps."""
+            wsj, wlj, wsk, wlk = occ[rname]
+            if kind == "J":
+                i = lid & 0xfff
+                steps, lanes = wsj[i], wlj[i]
+            else:
+                m = mirror(lid)
+                steps = wsk[lid] + (wsk[m] if m != lid else 0)
+                lanes = wlk[lid] + (wlk[m] if m != lid else 0)
+            return float(lanes) / (64.0 * steps) if steps else float("nan")
+
+        means = {k: float(np.mean(v)) for k, v in launch_ms.items()}
+        log("%s route, serialised quartet kernels per step: J %.3f ms, K %.3f ms" % (rname,
+            sum(v for k, v in means.items() if k[0] == "J"), sum(v for k, v in means.items() if k[0] == "K")))
+        rows = []
+        for k in sorted(means, key=lambda k: -means[k]):
+            N, f64, f32, ev = launch_work(*k)
+            fl = f64 + f32
+            pk = fl / (f64 / FP64_PEAK + f32 / FP32_PEAK) if fl > 0 else FP64_PEAK
+            rows.append((launch_name(*k), N, fl / max(N, 1.0), ev, means[k], fl / (means[k] * 1e-3) / 1e12,
+                         fl / (means[k] * 1e-3) / pk, lane_use(*k)))
+            log("  launch %-36s %9.3f ms  %8.3f TFLOP/s" % (rows[-1][0], means[k], rows[-1][5]))
+        if table_path:
+            with open(table_path, "w") as f:
+                f.write("| launch | N (reference convention) | FLOP/quartet | evaluated | ms | TFLOP/s | frac "
+                        "| lane use |\n")
+                f.write("|---|---:|---:|---:|---:|---:|---:|---:|\n")
+                for r in rows:
+                    f.write("| %s | %d | %.0f | %d | %.3f | %.2f | %.3f | %.3f |\n" % r)
+        dom = max(means, key=lambda k: means[k])
+        Nd, dom64, dom32, _ = launch_work(*dom)
+        dom_fl = dom64 + dom32
+        # peak of the launch's mix: FP64 FLOPs at the FP64 peak, FP32 FLOPs at the FP32 peak
+        peak = dom_fl / (dom64 / FP64_PEAK + dom32 / FP32_PEAK) if dom_fl > 0 else FP64_PEAK
+        achieved = dom_fl / (means[dom] * 1e-3)
+        # whole-step fractions: algorithmic (reference counts) and executed (the kernels' counters)
+        n32 = np.zeros(625)
+        for c in range(625):
+            n32[c] = float(ck32[256 + c]) / max(1, ck[256 + c]) * NK[c] if ck[256 + c] else 0.0
+        for idx in range(25):
+            if cj[idx]:
+                f32 = cj32[idx] / cj[idx]
+                for c in range(625):
+                    L1, L2, L3, L4 = cls_of(c)
+                    if (L1 + L2) * 5 + (L3 + L4) == idx:
+                        n32[c] += f32 * NJ[c]
+        t_exec = 0.0
+        for idx in range(25):
+            if cj[idx]:
+                L12, L34 = idx // 5, idx % 5
+                cs = [c for c in range(625) if sum(cls_of(c)[:2]) == L12 and sum(cls_of(c)[2:]) == L34 and NJ[c]]
+                if cs:
+                    Fm = sum(NJ[c] * cls_flops(c) for c in cs) / max(1, sum(NJ[c] for c in cs))
+                    t_exec += (cj[idx] - cj32[idx]) * Fm / FP64_PEAK + cj32[idx] * Fm / FP32_PEAK
+        for c in range(625):
+            if ck[256 + c]:
+                t_exec += (ck[256 + c] - ck32[256 + c]) * cls_flops(c) / FP64_PEAK + ck32[256 + c] * cls_flops(c) / FP32_PEAK
+        roofline = {"bound": "valu", "pipe": "%s vector ALU (MI355X peak; the ERI core is not a dense contraction, "
+                                              "no MFMA)" % ("fp32+fp64" if dom32 > 0 else "fp64"),
+                    "kernel": launch_name(*dom), "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                    "frac": achieved / peak, "traffic": None, "launch_ms": means[dom], "flops_per_launch": float(dom_fl),
+                    "quartets_per_launch": float(Nd),
+                    "flops_source": "lsjk_count_quartets (reference convention); a K launch of a class with a distinct "
+                                    "mirror class counts N_K(class) + N_K(mirror)",
+                    "launch_timing": "HIP events around each launch on its stream, class launches serialised "
+                                     "(%d steps after the timed region)" % args.attribution_steps,
+                    "jk_frac": wl.ideal_time(n32) / t_route,
+                    "jk_exec_frac": t_exec / t_route}
+        # HBM bytes per launch of the dominant kernel, from the committed counter summary of this
+        # workload (two rocprofv3 passes over `bench.py --pmc-pass`, calibrated on copy_bytes)
+        traffic_md = os.path.join(ROOT, "profiles", "jk_%s_hbm_traffic.md" % args.workload)
+        if os.path.exists(traffic_md):
+            for line in open(traffic_md):
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 5 and cells[0] == roofline["kernel"]:
+                    roofline["traffic"] = float(cells[4]) * 1e6
+                    roofline["traffic_source"] = os.path.relpath(traffic_md, ROOT)
+        return roofline
+
+    roofline = attribute(head, t_step, args.table)
+    # the FP64 route's dominant kernel as well, when another route holds the headline
+    roofline_fp64 = attribute("fp64", timing["fp64"], None) if head != "fp64" else None
 
     route_info = {}
     for name in routes:
@@ -616,6 +623,8 @@ def main():
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000 * t_step, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32" if head_fp32 > 0 else "f64",
            "data": "synthetic", "config": config, "roofline": roofline}
+    if roofline_fp64 is not None:
+        out["roofline_fp64_route"] = roofline_fp64
     if cpu is not None:
         out["cpu_baseline"] = cpu
     if not args.quick:

@@ -207,13 +207,17 @@ int lsjk_exchange_screened(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34,
  * task scaled by 1/4, 1/2, 1, 2 or 4, and the K launches with the work-unit split target of every
  * class at 3072, 12288 or 49152 units (class launches serialised on one stream, HIP events around
  * each, best of two), and keeps the fastest choice per J group pair and per K class for later
- * builds of the list on both sides (the FP64 and mixed routes share these choices).  Called with a
+ * builds of the list on both sides (the FP64 and mixed routes share these choices); before the K
+ * splits it times the rows per exchange ket group (16, 32 or 64 per (row L, partner L) of the list,
+ * where every class over that ket pair fits 64 KB of LDS) and keeps the size with the least summed
+ * time of the ket pair's class launches.  Called with a
  * mixed route (thredp > thresp), it instead times each launch of that route with its mixed kernel
  * and with the FP64 kernel (the same quartets, all in FP64) and keeps the faster per launch, and
  * times the route's mirror-class orientations; call it after the FP64 route's tuning.
  * lsjk_tuned_params: the choices, n entries: kind (0 J one-way, 1 J symmetric, 2 K, 3 mixed-route
- * kernel), counter index (J: L12*5 + L34, bit 24 for the symmetric kernel; K: as lsjk_last_counts),
- * value (chunk scale, split target, or 1 = FP64 kernel / 0 = mixed kernel), and the launch's ms
+ * kernel, 4 K ket group rows), counter index (J: L12*5 + L34, bit 24 for the symmetric kernel; K: as
+ * lsjk_last_counts; kind 4: L3*5 + L4), value (chunk scale, split target, 1 = FP64 kernel / 0 =
+ * mixed kernel, or rows per group), and the launch's ms (kind 4: the ket pair's launches summed)
  * with the default and with the choice.
  * lsjk_batch_autotune: the same on a batch's combined list (D_dev in the batch layout); drops the
  * batch's captured graphs.  lsjk_batch_pairlist: the batch's combined list (for lsjk_tuned_params). */

@@ -706,15 +706,22 @@ const std::vector<int>& get_mrank(lsjk_pairlist* pl)
 }
 
 // Ket groups of the (row L, partner Lp) class of a list (see KGroups): the class's rows in spatial
-// order, kgroup_size(L, Lp) per group, each group's partner entries merged by bound (descending, ties by
+// order, gsize per group, each group's partner entries merged by bound (descending, ties by
 // entry index) and padded to a multiple of 64 slots.
-const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp)
+// Rows per ket group of (row L, partner L) of a list: the autotuned value or kgroup_size.
+int kgroup_for(const lsjk_pairlist* pl, int L, int Lp)
 {
-    const int gsize = lsjk::kgroup_size(L, Lp);
-    const int key = L * 8 + Lp;
+    auto it = pl->kgsize.find(L * 8 + Lp);
+    return it != pl->kgsize.end() ? it->second : lsjk::kgroup_size(L, Lp);
+}
+
+const lsjk_pairlist::KGroupData& get_kgroups(lsjk_pairlist* pl, int L, int Lp, int gsize)
+{
+    const int key = (L * 8 + Lp) * 256 + gsize;
     auto it = pl->kgroups.find(key);
     if (it != pl->kgroups.end()) return it->second;
     lsjk_pairlist::KGroupData G;
+    G.gsize = gsize;
     const auto& rows = pl->h_rows[L][Lp];
     const auto& mrank = get_mrank(pl);
     const int nr = (int)rows.size();
@@ -797,12 +804,14 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
                                                  int L2, int L3, int L4, bool tri, float thresp, double* cost_out)
 {
     using lsjk::KUnitG;
-    const int gsize = lsjk::kgroup_size(L3, L4);
     lsjk_pairlist* pl12 = const_cast<lsjk_pairlist*>(pl12c);
     lsjk_pairlist* pl34 = const_cast<lsjk_pairlist*>(pl34c);
+    const int gsize = kgroup_for(pl34, L3, L4);
     unsigned int tb;
     std::memcpy(&tb, &thresp, sizeof(tb));
-    const long long cls = ((long long)tb << 8) | (((((L1 * 3 + L2) * 3 + L3) * 3 + L4) << 1) | (tri ? 1 : 0));
+    // key: rows per ket group (bits 40..47), thresp bits (8..39), class and triangle flag
+    const long long cls = ((long long)gsize << 40) | ((long long)tb << 8) |
+                          (((((L1 * 3 + L2) * 3 + L3) * 3 + L4) << 1) | (tri ? 1 : 0));
     const auto key = std::make_pair(pl34->uid, cls);
     auto it = pl12->kunits_g.find(key);
     if (it == pl12->kunits_g.end()) {
@@ -810,8 +819,8 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
         {
             std::set<long long> seen;
             for (const auto& kv : pl12->kunits_g)
-                if (kv.first.first == pl34->uid) seen.insert(kv.first.second >> 8);
-            if (!seen.count(cls >> 8) && (int)seen.size() >= kKThrespCached) {
+                if (kv.first.first == pl34->uid) seen.insert((kv.first.second >> 8) & 0xffffffffLL);
+            if (!seen.count((cls >> 8) & 0xffffffffLL) && (int)seen.size() >= kKThrespCached) {
                 auto same = [&](const std::pair<unsigned long long, long long>& k) { return k.first == pl34->uid; };
                 erase_if_key(pl12->kunits_g, same, true);
                 erase_if_key(pl12->kcost_g, same, false);
@@ -819,8 +828,8 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
             }
         }
         const auto& rows1 = pl12->h_rows[L1][L2];
-        const auto& G34 = get_kgroups(pl34, L3, L4);
-        const auto& G12 = get_kgroups(pl12, L1, L2);
+        const auto& G34 = get_kgroups(pl34, L3, L4, gsize);
+        const auto& G12 = get_kgroups(pl12, L1, L2, kgroup_for(pl12, L1, L2));  // its row order only
         // the class's unit-split target: autotuned (this list on both sides) or the default
         long long kKUnitsTarget = kKUnitsTargetDefault;
         if (pl12 == pl34) {
@@ -993,6 +1002,7 @@ std::pair<const lsjk::KUnitG*, int> get_kunits_g(const lsjk_pairlist* pl12c, con
 lsjk::KGroups kgroups_args(const lsjk_pairlist::KGroupData& G)
 {
     lsjk::KGroups k;
+    k.gsize = G.gsize;
     k.slot = G.d_slot;
     k.grp_slot = G.d_grp_slot;
     k.grp_rows = G.d_grp_rows;
@@ -1246,8 +1256,8 @@ void unit_features(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34c, int L1
 {
     constexpr int NF = LSJK_SCREEN_NFEAT;
     lsjk_pairlist* pl34 = const_cast<lsjk_pairlist*>(pl34c);
-    const auto& G = get_kgroups(pl34, L3, L4);
-    const int gsize = lsjk::kgroup_size(L3, L4);
+    const auto& G = get_kgroups(pl34, L3, L4, kgroup_for(pl34, L3, L4));
+    const int gsize = G.gsize;
     const lsjk::Basis *b1 = pl12->b1, *b2 = pl12->b2, *b3 = pl34->b1, *b4 = pl34->b2;
     const auto& rows1 = pl12->h_rows[L1][L2];
     const int m12 = pl12->kmaxL2 + 1;
@@ -1822,7 +1832,8 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
                 double cost = 0.0;
                 const auto units = get_kunits_g(pl12, pl34, L1, L2, L3, L4, tri, thresp, &cost);
                 if (units.second == 0) return false;
-                const KGroups kg = kgroups_args(get_kgroups(const_cast<lsjk_pairlist*>(pl34), L3, L4));
+                const KGroups kg = kgroups_args(get_kgroups(const_cast<lsjk_pairlist*>(pl34), L3, L4,
+                                                            kgroup_for(pl34, L3, L4)));
                 KArgs ka{};
                 ka.rys = S.rys;
                 ka.ew = ew;
@@ -2143,6 +2154,48 @@ void autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const double* s
         pl->tuned.push_back({sym ? 1 : 0, idx, (double)best, kv.second[1.0f], kv.second[best]});
     }
     rebuild_jtasks(pl);
+    // K: rows per ket group, per (row L, partner L) of the ket list: 16, 32 or 64 where every class
+    // over that ket pair fits the LDS limit on both routes; each candidate is timed for all ket
+    // pairs at once and a pair keeps the size with the least summed time of its classes' launches
+    {
+        const int maxL = pl->b1->maxL;
+        auto fits = [&](int c, int L3, int L4) {
+            for (int L1 = 0; L1 <= maxL; L1++)
+                for (int L2 = 0; L2 <= maxL; L2++)
+                    for (int route : {kRoutePlain, kRouteMixed})
+                        if (kket_lds_bytes(S.rys, L1, L2, L3, L4, route, c) > kMaxLdsPerGroup) return false;
+            return true;
+        };
+        pl->kgsize.clear();
+        std::map<int, std::map<int, double>> gt;  // ket pair (L3 * 8 + L4) -> rows per group -> ms
+        for (int c : {16, 32, 64}) {
+            for (int L3 = 0; L3 <= maxL; L3++)
+                for (int L4 = 0; L4 <= maxL; L4++) {
+                    if (fits(c, L3, L4))
+                        pl->kgsize[L3 * 8 + L4] = c;
+                    else
+                        pl->kgsize.erase(L3 * 8 + L4);
+                }
+            drop_own_kunits(pl);
+            std::map<int, double> sum;
+            for (auto& kv : launches(false)) {
+                const int cls = kv.first - 256, L3 = (cls / 5) % 5, L4 = cls % 5;
+                if (pl->kgsize.count(L3 * 8 + L4) && pl->kgsize[L3 * 8 + L4] == c) sum[L3 * 8 + L4] += kv.second;
+            }
+            for (auto& kv : sum) gt[kv.first][c] = kv.second;
+        }
+        pl->kgsize.clear();
+        for (auto& kv : gt) {
+            const int L3 = kv.first / 8, L4 = kv.first % 8, def = kgroup_size(L3, L4);
+            int best = kv.second.count(def) ? def : kv.second.begin()->first;
+            for (auto& c : kv.second)
+                if (c.second < kv.second[best]) best = c.first;
+            if (best != def) pl->kgsize[kv.first] = best;
+            if (kv.second.count(def))
+                pl->tuned.push_back({4, L3 * 5 + L4, (double)best, kv.second[def], kv.second[best]});
+        }
+        drop_own_kunits(pl);
+    }
     // K: one candidate split target for every class at a time
     std::map<int, std::map<long long, double>> kt;  // class counter index -> target -> ms
     for (long long c : kKSplits) {
@@ -2707,7 +2760,7 @@ int lsjk_unit_magnitudes(const lsjk_pairlist* pl, int L1, int L2, int L3, int L4
         if (pl->h_rows[L1][L2].empty() || pl->h_rows[L3][L4].empty()) return;
         const auto units = get_kunits_g(pl, pl, L1, L2, L3, L4, false, thresp, nullptr);
         if (units.second == 0) return;
-        const KGroups kg = kgroups_args(get_kgroups(const_cast<lsjk_pairlist*>(pl), L3, L4));
+        const KGroups kg = kgroups_args(get_kgroups(const_cast<lsjk_pairlist*>(pl), L3, L4, kgroup_for(pl, L3, L4)));
         KArgs ka{};
         ka.rys = S.rys;
         ka.ew.n = 1;

@@ -573,7 +573,7 @@ __device__ __forceinline__ KEnt<float> load_ent32_uniform(const double* __restri
 
 // LDS of one k_ket workgroup: the staged table blocks first, FP64 (routes with FP64 quartets,
 // nrys <= 3) then FP32 (routes with FP32 quartets, nrys <= 5), then kKWaves group accumulators
-// (G x N1 x N3 doubles each).  The FP64 table at address 0 keeps a bin's coefficient offsets
+// (gsize x N1 x N3 doubles each, gsize = rows per ket group).  The FP64 table at address 0 keeps a bin's coefficient offsets
 // (0..8 nrys * 24 bytes) inside the immediate offset field of the paired LDS reads, so the table
 // lookup needs one address per bin.
 template <int L1, int L2, int L3, int L4, int ROUTE>
@@ -583,8 +583,7 @@ struct KKetLds {
     static constexpr bool HAS64 = ROUTE != kRouteLo;
     static constexpr bool STAGED64 = HAS64 && rys_staged(NR);
     static constexpr bool STAGED32 = HAS32 && rys32_staged(NR);
-    static constexpr int G = kgroup_size(L3, L4);              // rows per ket group
-    static constexpr int ACC = G * ncart(L1) * ncart(L3);  // doubles per wave
+    static constexpr int N13 = ncart(L1) * ncart(L3);  // accumulator doubles per group row
     static __host__ __device__ size_t bytes64(const RysTable& tab)
     {
         return STAGED64 ? rys_lds_doubles(tab.nbin[NR], NR) * sizeof(double) : 0;
@@ -595,7 +594,10 @@ struct KKetLds {
         const size_t b = bytes64(tab) + (STAGED32 ? rys32_lds_floats(tab.nbin[NR], NR) * sizeof(float) : 0);
         return (b + 7) & ~(size_t)7;
     }
-    static size_t bytes(const RysTable& tab) { return acc_offset(tab) + (size_t)kKWaves * ACC * sizeof(double); }
+    static size_t bytes(const RysTable& tab, int gsize)
+    {
+        return acc_offset(tab) + (size_t)kKWaves * gsize * N13 * sizeof(double);
+    }
 };
 
 // An FP64 entry rounded to the FP32 route's arithmetic (the values of its FP32 record).
@@ -664,10 +666,12 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     double* s_tab64 = s_dyn;
     float* s_tab32 = reinterpret_cast<float*>(reinterpret_cast<char*>(s_dyn) + Lds::bytes64(a.rys));
-    double* s_acc = reinterpret_cast<double*>(reinterpret_cast<char*>(s_dyn) + Lds::acc_offset(a.rys)) + w * Lds::ACC;
+    const int G = kg.gsize;  // rows per ket group
+    const int ACC = G * N13;  // accumulator doubles per wave
+    double* s_acc = reinterpret_cast<double*>(reinterpret_cast<char*>(s_dyn) + Lds::acc_offset(a.rys)) + w * ACC;
     if constexpr (Lds::STAGED64) rys_stage_lds<NR>(a.rys, s_tab64);
     if constexpr (Lds::STAGED32) rys32_stage_lds<NR>(a.rys, s_tab32);
-    for (int q = lane; q < Lds::ACC; q += 64) s_acc[q] = 0.0;
+    for (int q = lane; q < ACC; q += 64) s_acc[q] = 0.0;
     __syncthreads();
     // no workgroup barrier after this point: the waves are independent
     const int u = blockIdx.x * kKWaves + w;
@@ -709,7 +713,7 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
         const int2 sl = kg.slot[c0 + lane];
         const int e4 = max(sl.x, 0);
         const float b34 = a.s34.eb[e4];
-        const bool valid = sl.x >= 0 && kg.grp_rank[g * Lds::G + sl.y] >= rank1 && b34 >= thre34;
+        const bool valid = sl.x >= 0 && kg.grp_rank[g * G + sl.y] >= rank1 && b34 >= thre34;
         const int blk4 = a.s34.eblk[e4];
         KEnt<double> E34;
         KEnt<float> E34f;
@@ -814,11 +818,11 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int o1 = ld_const(a.s12.cart + r1);
-        for (int q = lane; q < Lds::ACC; q += 64) {
+        for (int q = lane; q < ACC; q += 64) {
             const double v = s_acc[q];
             if (v == 0.0) continue;
             const int t = q / N13, ij = q - t * N13, i = ij / N3, j = ij - i * N3;
-            const int r3 = kg.grp_rows[g * Lds::G + t];
+            const int r3 = kg.grp_rows[g * G + t];
             const int o3 = a.s34.cart[r3];
             atomic_add(a.K + (size_t)(o1 + i) * a.ldK + (o3 + j), v);
             if (a.symmetric && (r1 != r3 || a.mirror_diag)) atomic_add(a.K + (size_t)(o3 + j) * a.ldK + (o1 + i), v);
@@ -850,7 +854,7 @@ __global__ __launch_bounds__(64 * kKWaves) void k_unit_mag(KArgs a, KGroups kg, 
                                                            int nunits, float smin, float* __restrict__ out)
 {
     constexpr int N1 = ncart(L1), N2 = ncart(L2), N3 = ncart(L3), N4 = ncart(L4);
-    constexpr int G = kgroup_size(L3, L4);
+    const int G = kg.gsize;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int u = blockIdx.x * kKWaves + w;
@@ -914,8 +918,9 @@ void launch_k_ket_l1(int L2, int L3, int L4, int route, const KArgs& a, const KG
     dim3 grid((unsigned)((nunits + kKWaves - 1) / kKWaves)), block(64 * kKWaves);
 #define ONE(B, C, D, R)                                                                             \
     {                                                                                               \
-        const size_t lds = KKetLds<L1, B, C, D, R>::bytes(a.rys);                                   \
-        if (lds > kMaxLdsPerGroup) throw std::runtime_error("lsjk: k_ket LDS request too large");  \
+        const size_t lds = KKetLds<L1, B, C, D, R>::bytes(a.rys, kg.gsize);                         \
+        if (lds > kMaxLdsPerGroup || lds != kket_lds_bytes(a.rys, L1, B, C, D, R, kg.gsize))       \
+            throw std::runtime_error("lsjk: k_ket LDS request too large or inconsistent");         \
         hipLaunchKernelGGL((k_ket<L1, B, C, D, R>), grid, block, lds, s, a, kg, units, nunits);     \
     }
 #define CASE(B, C, D)                                                                               \

@@ -155,6 +155,7 @@ struct PairList {
     std::vector<int> h_mrank;
     // K ket groups per (row L, partner L) (kernels.hpp KGroups), built on first use
     struct KGroupData {
+        int gsize = 16;  // rows per group
         int2* d_slot = nullptr;
         int* d_grp_slot = nullptr;
         int* d_grp_rows = nullptr;
@@ -169,7 +170,10 @@ struct PairList {
         std::vector<int2> slot;     // host copies of the slots and group rows
         std::vector<int> grp_rows;
     };
-    std::map<int, KGroupData> kgroups;
+    std::map<int, KGroupData> kgroups;  // keyed by ((row L) * 8 + partner L) * 256 + rows per group
+    // autotuned rows per ket group, per (row L) * 8 + partner L of this list as the ket list
+    // (absent: kgroup_size)
+    std::map<int, int> kgsize;
     // cached K build unit lists (device pointer, count) keyed by (ket list uid, class and
     // triangle flag, thresp bits), and their estimated cost
     std::map<std::pair<unsigned long long, long long>, std::pair<void*, int>> kunits_g;

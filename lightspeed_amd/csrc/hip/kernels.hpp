@@ -155,8 +155,11 @@ struct KUnit {
 // p rows with d partners (measured up to 42 % faster), 16 otherwise (larger groups measured
 // slower there: longer units, and fewer ket rows of the bra row's triangle per group).
 constexpr int kgroup_size(int L, int Lp) { return (L == 0 && Lp > 0) ? 64 : ((L == 1 && Lp == 2) ? 32 : 16); }
+// These are the defaults: lsjk_autotune picks 16, 32 or 64 rows per (row L, partner L) of a list
+// (KGroups::gsize), where every class over that ket pair fits the LDS limit (kket_lds_bytes).
 constexpr int kKWaves = 4;  // waves (independent work units) per workgroup
 struct KGroups {
+    int gsize;             // rows per group
     const int2* slot;      // [nslot] {entry index or -1 (padding), position of the row in the group}
     const int* grp_slot;   // [ngroup + 1] first slot of each group
     const int* grp_rows;   // [ngroup * group size] row primitive at each position (-1: none)
@@ -290,6 +293,20 @@ int persistent_grid(const void* kernel, int block, size_t lds, long long nwork_g
 
 // blk_lo / blk_hi (nullptr: dense): per block 2, the range of blocks 4 of its molecule (batch
 // basis); band = the largest such range.
+// Dynamic LDS of one k_ket workgroup (as KKetLds lays it out): the staged Rys table blocks (FP64
+// for routes with FP64 quartets and nrys <= 3, FP32 for routes with FP32 quartets and nrys <= 5),
+// 8-byte aligned, then kKWaves accumulators of gsize x N1 x N3 doubles.
+inline size_t kket_lds_bytes(const RysTable& tab, int L1, int L2, int L3, int L4, int route, int gsize)
+{
+    const int NR = (L1 + L2 + L3 + L4) / 2 + 1;
+    const bool has32 = route == kRouteLo || route == kRouteMixed, has64 = route != kRouteLo;
+    size_t b = 0;
+    if (has64 && rys_staged(NR)) b += rys_lds_doubles(tab.nbin[NR], NR) * sizeof(double);
+    if (has32 && rys32_staged(NR)) b += rys32_lds_floats(tab.nbin[NR], NR) * sizeof(float);
+    b = (b + 7) & ~(size_t)7;
+    return b + (size_t)kKWaves * gsize * ncart(L1) * ncart(L3) * sizeof(double);
+}
+
 // out[pr[i] * ldo + pc[j]] = in[i * ldi + j] (the K view copies of D and of its block maxima).
 void launch_permute(const double* in, int ldi, int nr, int nc, const int* pr, const int* pc, double* out, int ldo,
                     hipStream_t s);

@@ -27,9 +27,10 @@ def test_autotune_single_list(product_backend, oracle_backend):
     kinds = {k for k, _, _, _, _ in table}
     print("tuned entries: %d (kinds %s)" % (len(table), sorted(kinds)), flush=True)
     assert 0 in kinds or 1 in kinds
-    assert 2 in kinds
+    assert 2 in kinds and 4 in kinds
     for kind, idx, val, m0, m1 in table:
-        assert (kind < 2 and val in (0.25, 0.5, 1.0, 2.0, 4.0)) or (kind == 2 and val in (3072, 12288, 49152))
+        assert ((kind < 2 and val in (0.25, 0.5, 1.0, 2.0, 4.0)) or (kind == 2 and val in (3072, 12288, 49152))
+                or (kind == 4 and val in (16, 32, 64)))
         assert m1 <= m0 + 1e-9
     for thredp in (THRE, 1e-6):
         J = np.array(ls.IntBox.coulomb(res, ew, pl, pl, ls.Tensor.array(D), THRE, thredp).np)
