@@ -35,8 +35,9 @@ nrys = (L1+L2+L3+L4)/2 + 1), against the peak of the launch's mix: its FP64 FLOP
 78.6 TFLOP/s and its FP32 FLOPs (the launch's evaluated FP32 share) at the FP32 peak 157.3 TFLOP/s.
 jk_frac = [sum_FP64 N F / 78.6e12 + sum_FP32 N F / 157.3e12] / t_JK (algorithmic counts);
 jk_exec_frac = the same over the quartets the kernels evaluated (their own counters).
-`--table PATH` writes the per-launch table (N, F, evaluated, ms, TFLOP/s, fraction, lane use = the
-share of the wave evaluating integrals in its busy steps).
+`--table PATH` writes the headline route's per-launch table (N, F, evaluated, ms, TFLOP/s, fraction,
+lane use = the share of the wave evaluating integrals in its busy steps), and PATH_fp64 the FP64
+route's when the headline is another route.
 cpu_baseline: the reference's own CPU IntBox::coulombCPU + IntBox::exchangeCPU, compiled from
 its sources into oracle/_ref/ref_jk, timed on a bounded sample of the same workload (the bra list
 keeps every `--cpu-stride`-th pair of each class, the ket list is whole), all threads of the host.
@@ -351,6 +352,27 @@ def main():
     else:
         log("no trained screen at %s: learned routes not measured" % screen_path)
 
+    # the autotuner (J ket chunks per group pair, K ket-group rows and unit splits, timed on this
+    # list and density on the FP64 route, then the mixed route's kernel per launch), before the
+    # counted calls and accuracy checks, so that these describe the kernels the timed steps run;
+    # the default parameters' FP64 time is measured first
+    tune_info = None
+    if not args.no_autotune:
+        t_def = wl.time_routes({"fp64_default": routes["fp64"]}, args.steps, args.warmup)["fp64_default"]
+        t0 = time.perf_counter()
+        table = ls.autotune(pl, wl.ew, wl.dD.ptr, thresp, thresp)
+        # the mixed route: per launch, its mixed kernel or the FP64 kernel, whichever is faster
+        table_mixed = ls.autotune(pl, wl.ew, wl.dD.ptr, thresp, args.thredp)
+        n_plain = sum(1 for k, i, v, _, _ in table_mixed if k == 3 and v == 1.0)
+        log("autotune, mixed route: %d of %d launches on the FP64 kernel" % (
+            n_plain, sum(1 for k, i, v, _, _ in table_mixed if k == 3)))
+        t_tune = time.perf_counter() - t0
+        changed = [(k, i, v) for k, i, v, a, b in table if (k < 2 and v != 1.0) or (k == 2 and v != 12288) or (k == 4 and b < a)]
+        log("autotune: %.1f s, %d entries, %d changed" % (t_tune, len(table), len(changed)))
+        tune_info = {"seconds": t_tune, "entries": len(table), "changed": len(changed),
+                     "mixed_route_fp64_kernel_launches": n_plain,
+                     "fp64_default_ms_per_step": 1000 * t_def,
+                     "table": [[int(k), int(i), float(v), float(a), float(b)] for k, i, v, a, b in table]}
     # evaluated counts per route and results for the accuracy checks (whole matrices vs the FP64 route)
     res_JK, cnt, occ = {}, {}, {}
     ls.set_counting(True)
@@ -410,25 +432,6 @@ def main():
             log("%s vs FP64 oracle (sample): max|dJ|=%.2e max|dK|=%.2e" % (
                 name, acc[name]["max_abs_dJ_vs_oracle_sample"], acc[name]["max_abs_dK_vs_oracle_sample"]))
 
-    # the autotuner (J ket chunks per group pair, K unit splits per class, timed on this list and
-    # density on the FP64 route); the default parameters' FP64 time is measured first
-    tune_info = None
-    if not args.no_autotune:
-        t_def = wl.time_routes({"fp64_default": routes["fp64"]}, args.steps, args.warmup)["fp64_default"]
-        t0 = time.perf_counter()
-        table = ls.autotune(pl, wl.ew, wl.dD.ptr, thresp, thresp)
-        # the mixed route: per launch, its mixed kernel or the FP64 kernel, whichever is faster
-        table_mixed = ls.autotune(pl, wl.ew, wl.dD.ptr, thresp, args.thredp)
-        n_plain = sum(1 for k, i, v, _, _ in table_mixed if k == 3 and v == 1.0)
-        log("autotune, mixed route: %d of %d launches on the FP64 kernel" % (
-            n_plain, sum(1 for k, i, v, _, _ in table_mixed if k == 3)))
-        t_tune = time.perf_counter() - t0
-        changed = [(k, i, v) for k, i, v, a, b in table if (k < 2 and v != 1.0) or (k == 2 and v != 12288) or (k == 4 and b < a)]
-        log("autotune: %.1f s, %d entries, %d changed" % (t_tune, len(table), len(changed)))
-        tune_info = {"seconds": t_tune, "entries": len(table), "changed": len(changed),
-                     "mixed_route_fp64_kernel_launches": n_plain,
-                     "fp64_default_ms_per_step": 1000 * t_def,
-                     "table": [[int(k), int(i), float(v), float(a), float(b)] for k, i, v, a, b in table]}
     timing = wl.time_routes(routes, args.steps, args.warmup)
 
     # headline route: the fastest among those that meet the accuracy bar (max|dJ|, max|dK| <= 1e-7
@@ -499,8 +502,8 @@ def main():
             c = lid - 256
             m = mirror(lid)
             N = float(NK[c] + (NK[m - 256] if m != lid else 0))
-            # the counting call ran before the autotuner, which may have turned the mirror pair around:
-            # its counts sit at one of the two indices
+            # the counts sit at the index of the orientation that ran in the counting call (without
+            # the autotuner the first call picks it): take both indices
             n_ev = int(ck[lid] + (ck[m] if m != lid else 0))
             n_32 = int(ck32[lid] + (ck32[m] if m != lid else 0))
             f32 = n_32 / n_ev if n_ev else 0.0
@@ -599,7 +602,10 @@ def main():
 
     roofline = attribute(head, t_step, args.table)
     # the FP64 route's dominant kernel as well, when another route holds the headline
-    roofline_fp64 = attribute("fp64", timing["fp64"], None) if head != "fp64" else None
+    roofline_fp64 = None
+    if head != "fp64":
+        roofline_fp64 = attribute("fp64", timing["fp64"],
+                                  (os.path.splitext(args.table)[0] + "_fp64.md") if args.table else None)
 
     route_info = {}
     for name in routes:
