@@ -228,6 +228,11 @@ int lsjk_tuned_params(const lsjk_pairlist* pl, int capacity, int* n, int* kind, 
 int lsjk_batch_autotune(lsjk_batch* b, const double* D_dev, int nterm, const double* scales, const double* omegas,
                         float thresp, float thredp);
 int lsjk_batch_pairlist(const lsjk_batch* b, const lsjk_pairlist** out);
+/* Sets the rows per exchange ket group of (row L, partner Lp) of the list, as lsjk_autotune would
+ * (rows 16, 32 or 64; 0 restores the default); LSJK_E_INVALID when a class over that ket pair
+ * would not fit 64 KB of LDS.  For reapplying saved tuning, or fixing it.  No counterpart in the
+ * reference. */
+int lsjk_set_kgroup_rows(lsjk_pairlist* pl, int L, int Lp, int rows);
 
 /* Boys functions F_n(T) = int_0^1 t^(2n) exp(-T t^2) dt for n = 0..nboys (nboys <= 16) at npoint
  * arguments: F_dev (npoint x (nboys + 1), row-major) from T_dev, FP64.  Replaces the reference's

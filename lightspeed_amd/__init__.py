@@ -55,6 +55,7 @@ set_profiling = _native.set_profiling
 last_launches = _native.last_launches
 count_quartets = _native.count_quartets
 autotune = _native.autotune
+set_kgroup_rows = _native.set_kgroup_rows
 
 NATIVE_LIBRARY = _os.path.join(_os.path.dirname(__file__), "liblsjk.so")
 

@@ -2876,6 +2876,30 @@ int lsjk_autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const doubl
     });
 }
 
+int lsjk_set_kgroup_rows(lsjk_pairlist* pl, int L, int Lp, int rows)
+{
+    return guarded([&] {
+        using namespace lsjk;
+        require_init();
+        if (!pl || L < 0 || L > kMaxL || Lp < 0 || Lp > kMaxL)
+            throw LsjkError(LSJK_E_INVALID, "lsjk_set_kgroup_rows: bad list or angular momentum");
+        if (rows == 0) {
+            pl->kgsize.erase(L * 8 + Lp);
+        } else {
+            if (rows != 16 && rows != 32 && rows != 64)
+                throw LsjkError(LSJK_E_INVALID, "lsjk_set_kgroup_rows: rows must be 16, 32 or 64");
+            for (int L1 = 0; L1 <= kMaxL; L1++)
+                for (int L2 = 0; L2 <= kMaxL; L2++)
+                    for (int route : {kRoutePlain, kRouteMixed})
+                        if (kket_lds_bytes(S.rys, L1, L2, L, Lp, route, rows) > kMaxLdsPerGroup)
+                            throw LsjkError(LSJK_E_INVALID, "lsjk_set_kgroup_rows: the classes over this ket pair "
+                                                            "would exceed the LDS limit");
+            pl->kgsize[L * 8 + Lp] = rows;
+        }
+        drop_own_kunits(pl);
+    });
+}
+
 int lsjk_tuned_params(const lsjk_pairlist* pl, int capacity, int* n, int* kind, int* idx, double* value,
                       double* ms_default, double* ms_tuned)
 {

@@ -295,6 +295,12 @@ PYBIND11_MODULE(_lightspeed, m)
             py::arg("ewald"), py::arg("D_ptr"), py::arg("thresp"), py::arg("thredp"));
 
     py::class_<Boys>(m, "Boys").def_static("compute", &Boys::compute, py::arg("nboys"), py::arg("T"));
+    m.def(
+        "set_kgroup_rows",
+        [](std::shared_ptr<PairList> pl, int L, int Lp, int rows) {
+            check(lsjk_set_kgroup_rows(pl->device_handle(), L, Lp, rows));
+        },
+        py::arg("pl"), py::arg("L"), py::arg("Lp"), py::arg("rows"));
 
     auto tuned_list = [](const lsjk_pairlist* d) {
         int n = 0;

@@ -105,3 +105,37 @@ def test_autotune_mixed_route_kernels(product_backend):
     for i, v, _, _ in routes:
         if v == 1.0 and 256 <= i < 256 + 625:
             assert c32[i] == 0, i
+
+
+@pytest.mark.gpu
+def test_every_kgroup_size_gives_the_same_exchange(product_backend):
+    """The exchange ket-group rows are a launch parameter only: K with 16, 32 and 64 rows per group
+    for every ket pair (where the classes fit the LDS limit) equals the default build, on both
+    routes and in the symmetric and full modes."""
+    ls = product_backend.ls
+    from lightspeed_amd import workloads as W
+    mol = W.water_cluster(6, 8.0, 47)
+    bas = ls.Basis.from_gbs_file(mol, "6-31gss")
+    D = W.random_density(bas.nao, 30, 48)
+    pl = ls.PairList.build_schwarz(bas, bas, True, THRE)
+    res = ls.ResourceList.build()
+    ew = ls.Ewald([1.0], [-1.0])
+    Dt = ls.Tensor.array(D)
+    ref = {(sym, td): np.array(ls.IntBox.exchange(res, ew, pl, pl, Dt, sym, THRE, td).np)
+           for sym in (True, False) for td in (THRE, 1e-6)}
+    for rows in (16, 32, 64):
+        applied = 0
+        for L in range(3):
+            for Lp in range(3):
+                try:
+                    ls.set_kgroup_rows(pl, L, Lp, rows)
+                    applied += 1
+                except RuntimeError:
+                    ls.set_kgroup_rows(pl, L, Lp, 0)  # does not fit: the default
+        print("rows %d applied to %d of 9 ket pairs" % (rows, applied), flush=True)
+        assert applied > 0
+        for (sym, td), K0 in ref.items():
+            K = np.array(ls.IntBox.exchange(res, ew, pl, pl, Dt, sym, THRE, td).np)
+            assert np.abs(K - K0).max() < (1e-12 if td == THRE else 1e-9), (rows, sym, td)
+    with pytest.raises(RuntimeError):
+        ls.set_kgroup_rows(pl, 0, 0, 48)
