@@ -209,11 +209,14 @@ int lsjk_exchange_screened(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34,
  * each, best of two), and keeps the fastest choice per J group pair and per K class for later
  * builds of the list on both sides (the FP64 and mixed routes share these choices); before the K
  * splits it times the rows per exchange ket group (16, 32 or 64 per (row L, partner L) of the list,
- * where every class over that ket pair fits 64 KB of LDS) and keeps the size with the least summed
- * time of the ket pair's class launches.  Called with a
+ * where every class over that ket pair fits 64 KB of LDS) together with the orientations of the
+ * symmetric mode's mirror classes: at each size every class is timed, those of a mirror pair in
+ * both orientations; a ket pair keeps the size with the least summed time of its classes, and each
+ * mirror pair then the faster orientation at the kept sizes.  Called with a
  * mixed route (thredp > thresp), it instead times each launch of that route with its mixed kernel
  * and with the FP64 kernel (the same quartets, all in FP64) and keeps the faster per launch, and
- * times the route's mirror-class orientations; call it after the FP64 route's tuning.
+ * times the route's mirror-class orientations; call it after the FP64 route's tuning.  Each route
+ * keeps its own orientations (a learned-route call uses those of its route).
  * lsjk_tuned_params: the choices, n entries: kind (0 J one-way, 1 J symmetric, 2 K, 3 mixed-route
  * kernel, 4 K ket group rows), counter index (J: L12*5 + L34, bit 24 for the symmetric kernel; K: as
  * lsjk_last_counts; kind 4: L3*5 + L4), value (chunk scale, split target, 1 = FP64 kernel / 0 =

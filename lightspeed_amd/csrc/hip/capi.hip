@@ -1421,6 +1421,8 @@ struct CallOpts {
     const char* who = nullptr;
     lsjk_screen* screen = nullptr;  // exchange: the learned route (certified classes)
     bool tune_orient = false;       // exchange: time both orientations of each mirror-class pair
+    std::map<int, double>* orient_ms = nullptr;     // with tune_orient: each timed class's ms
+    const std::map<int, int>* orient_set = nullptr; // exchange: store and use these orientations
 };
 
 void commit_ws(Workspace& wsp, const CallOpts& o, const char* who)
@@ -1771,11 +1773,15 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
         auto& ocache = const_cast<lsjk_pairlist*>(pl12)->korient;
         // the mirror-class orientations are timed only when the autotuner asks (o.tune_orient); a
         // first call otherwise takes the deterministic cost-model choice below
-        const bool tune = symmode && ew.n > 0 && o.tune_orient && !o.graph && !o.noalloc;
+        const bool tune = symmode && ew.n > 0 && o.tune_orient && !o.orient_set && !o.graph && !o.noalloc;
         std::map<int, int> tuned;  // filled by the tuning below, stored only once it is complete
         const std::map<int, int>* orient = nullptr;
-        if (symmode && !tune) {
+        if (symmode && o.orient_set) {
+            orient = &(ocache[okey] = *o.orient_set);
+        } else if (symmode && !tune) {
             auto it = ocache.find(okey);
+            // a learned-route call without choices of its own takes its route's tuned ones
+            if (it == ocache.end() && o.screen) it = ocache.find(std::make_pair(okey.first, okey.second & ~2ull));
             if (it != ocache.end()) orient = &it->second;
         }
         double* Dsh = nullptr;
@@ -1968,6 +1974,10 @@ static void exchange_call(const lsjk_pairlist* pl12, const lsjk_pairlist* pl34, 
                                 tc = std::min(tc, time_class(L1, L2, L3, L4));
                                 tm = std::min(tm, time_class(L3, L4, L1, L2));
                                 tuned[c] = tuned[m] = (tc <= tm) ? c : m;
+                                if (o.orient_ms) {
+                                    (*o.orient_ms)[c] = tc;
+                                    (*o.orient_ms)[m] = tm;
+                                }
                             }
                 orient = &(ocache[okey] = tuned);
             }
@@ -2079,11 +2089,18 @@ void autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const double* s
     } force_plain_reset{pl};
     const bool mixed = thredp > thresp;
     if (mixed) pl->plain_on_mixed.clear();
-    // mirror-class orientations: timed (both, twice each) and stored for this route; then warm
-    // ket groups, task and unit lists of the defaults
-    {
-        auto self = [&](const std::pair<unsigned long long, unsigned long long>& k) { return k.first == pl->uid; };
-        erase_if_key(pl->korient, self, false);
+    // the stored mirror-class orientations of this route (the other route's stay)
+    auto drop_orient = [&]() {
+        auto route = [&](const std::pair<unsigned long long, unsigned long long>& k) {
+            return k.first == pl->uid && (k.second & 3ull) == (mixed ? 1ull : 0ull);
+        };
+        erase_if_key(pl->korient, route, false);
+    };
+    // the mixed route: its mirror-class orientations timed (both, twice each) and stored; then warm
+    // ket groups, task and unit lists of the defaults (the FP64 route times its orientations
+    // together with the ket group rows, below)
+    drop_orient();
+    if (mixed) {
         CallOpts ot;
         ot.tune_orient = true;
         exchange_call(pl, pl, D_dev, 1, nterm, scales, omegas, thresp, thredp, dK, ot);
@@ -2154,9 +2171,13 @@ void autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const double* s
         pl->tuned.push_back({sym ? 1 : 0, idx, (double)best, kv.second[1.0f], kv.second[best]});
     }
     rebuild_jtasks(pl);
-    // K: rows per ket group, per (row L, partner L) of the ket list: 16, 32 or 64 where every class
-    // over that ket pair fits the LDS limit on both routes; each candidate is timed for all ket
-    // pairs at once and a pair keeps the size with the least summed time of its classes' launches
+    // K: rows per ket group and the mirror-class orientations, together.  Rows per group: 16, 32
+    // or 64 per (row L, partner L) of the ket list, where every class over that ket pair fits the
+    // LDS limit on both routes.  For each size, every class is timed: those of a mirror pair in
+    // both orientations (twice each, the faster kept), the others (their own mirror) in the
+    // regular launches.  A ket pair keeps the size with the least summed time of its classes (both
+    // orientations counted), and each mirror pair then keeps its faster orientation at the kept
+    // sizes (an orientation's ket pair can differ from its mirror's, so the sizes come first).
     {
         const int maxL = pl->b1->maxL;
         auto fits = [&](int c, int L3, int L4) {
@@ -2166,26 +2187,35 @@ void autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const double* s
                         if (kket_lds_bytes(S.rys, L1, L2, L3, L4, route, c) > kMaxLdsPerGroup) return false;
             return true;
         };
-        pl->kgsize.clear();
-        std::map<int, std::map<int, double>> gt;  // ket pair (L3 * 8 + L4) -> rows per group -> ms
+        auto ket_of = [](int k) { return ((k - 256) / 5) % 5 * 8 + (k - 256) % 5; };  // L3 * 8 + L4
+        std::map<int, std::map<int, double>> ct;  // class counter index -> rows per group -> ms
         for (int c : {16, 32, 64}) {
+            pl->kgsize.clear();
             for (int L3 = 0; L3 <= maxL; L3++)
-                for (int L4 = 0; L4 <= maxL; L4++) {
-                    if (fits(c, L3, L4))
-                        pl->kgsize[L3 * 8 + L4] = c;
-                    else
-                        pl->kgsize.erase(L3 * 8 + L4);
-                }
+                for (int L4 = 0; L4 <= maxL; L4++)
+                    if (fits(c, L3, L4)) pl->kgsize[L3 * 8 + L4] = c;
             drop_own_kunits(pl);
-            std::map<int, double> sum;
-            for (auto& kv : launches(false)) {
-                const int cls = kv.first - 256, L3 = (cls / 5) % 5, L4 = cls % 5;
-                if (pl->kgsize.count(L3 * 8 + L4) && pl->kgsize[L3 * 8 + L4] == c) sum[L3 * 8 + L4] += kv.second;
-            }
-            for (auto& kv : sum) gt[kv.first][c] = kv.second;
+            drop_orient();
+            std::map<int, double> om;
+            CallOpts ot;
+            ot.tune_orient = true;
+            ot.orient_ms = &om;
+            exchange_call(pl, pl, D_dev, 1, nterm, scales, omegas, thresp, thredp, dK, ot);
+            for (auto& kv : launches(false))
+                if (!om.count(kv.first)) om[kv.first] = kv.second;
+            for (auto& kv : om)
+                if (pl->kgsize.count(ket_of(kv.first))) ct[kv.first][c] = kv.second;
         }
+        // per ket pair: the summed time of its classes at each size that every one of them has
+        std::map<int, std::map<int, double>> gt;
+        for (auto& kv : ct)
+            for (auto& t : kv.second) gt[ket_of(kv.first)][t.first] += t.second;
+        for (auto& kv : ct)
+            for (auto it = gt[ket_of(kv.first)].begin(); it != gt[ket_of(kv.first)].end();)
+                it = kv.second.count(it->first) ? std::next(it) : gt[ket_of(kv.first)].erase(it);
         pl->kgsize.clear();
         for (auto& kv : gt) {
+            if (kv.second.empty()) continue;
             const int L3 = kv.first / 8, L4 = kv.first % 8, def = kgroup_size(L3, L4);
             int best = kv.second.count(def) ? def : kv.second.begin()->first;
             for (auto& c : kv.second)
@@ -2194,7 +2224,26 @@ void autotune(lsjk_pairlist* pl, const double* D_dev, int nterm, const double* s
             if (kv.second.count(def))
                 pl->tuned.push_back({4, L3 * 5 + L4, (double)best, kv.second[def], kv.second[best]});
         }
+        // each mirror pair: the faster orientation at the kept sizes
+        std::map<int, int> pick;
+        auto at = [&](int k) {
+            const auto it = ct.find(k);
+            if (it == ct.end()) return 1e300;
+            const int key = ket_of(k);
+            const auto jt = it->second.find(kgroup_for(pl, key / 8, key % 8));
+            return jt != it->second.end() ? jt->second : 1e300;
+        };
+        for (auto& kv : ct) {
+            const int c = kv.first, q = c - 256;
+            const int m = 256 + (((q / 5) % 5 * 5 + q % 5) * 5 + q / 125) * 5 + (q / 25) % 5;
+            if (c >= m) continue;
+            pick[c] = pick[m] = at(c) <= at(m) ? c : m;
+        }
         drop_own_kunits(pl);
+        drop_orient();
+        CallOpts os;
+        os.orient_set = &pick;
+        exchange_call(pl, pl, D_dev, 1, nterm, scales, omegas, thresp, thredp, dK, os);
     }
     // K: one candidate split target for every class at a time
     std::map<int, std::map<long long, double>> kt;  // class counter index -> target -> ms
