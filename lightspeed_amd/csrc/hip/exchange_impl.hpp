@@ -183,8 +183,10 @@ __device__ __forceinline__ Real krsqrt(Real x)
 // One primitive quartet: acc[a][c] += sum_{b,d} (ab|cd) Db[b][d], evaluated in the route's
 // arithmetic (Real) and added to the FP64 accumulator.  STAGED: the table block of this nrys was
 // copied to LDS at s_tab (FP64 block for Real = double, FP32 block for Real = float); otherwise
-// the global table is read.
-template <typename Real, int L1, int L2, int L3, int L4, bool STAGED>
+// the global table is read.  KETD (L3 = 0 only): e34's P - A holds Q - D, and the ket's 2-D
+// integrals (ab|0d) are recurred at D directly, as (ab|d0) with D in the place of C (the same
+// integrals; no transfer from C, so C - D is not needed).
+template <typename Real, int L1, int L2, int L3, int L4, bool STAGED, bool KETD = false>
 __device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab, const Ewald& ew,
                                           const KEnt<Real>& e12, const KEnt<Real>& e34,
                                           const Real (&Db)[ncart(L2)][ncart(L4)], double (&acc)[ncart(L1)][ncart(L3)])
@@ -233,10 +235,25 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab
             const Real B10 = i2p * (Real(1) - u * qf);
             const Real B01 = i2q * (Real(1) - u * pf);
             const Real cq = u * qf, cp = u * pf;
-            Axis4<Real, L1, L2, L3, L4> Ix, Iy, Iz;
-            rys_axis<Real, L1, L2, L3, L4>(pref * w[i], PAx - cq * PQx, QCx + cp * PQx, B10, B01, B00, ABx, CDx, Ix);
-            rys_axis<Real, L1, L2, L3, L4>(Real(1), PAy - cq * PQy, QCy + cp * PQy, B10, B01, B00, ABy, CDy, Iy);
-            rys_axis<Real, L1, L2, L3, L4>(Real(1), PAz - cq * PQz, QCz + cp * PQz, B10, B01, B00, ABz, CDz, Iz);
+            static_assert(!KETD || L3 == 0, "KETD: no angular momentum on C");
+            using AX = std::conditional_t<KETD, Axis4<Real, L1, L2, L4, 0>, Axis4<Real, L1, L2, L3, L4>>;
+            AX Ix, Iy, Iz;
+            if constexpr (KETD) {
+                rys_axis<Real, L1, L2, L4, 0>(pref * w[i], PAx - cq * PQx, QCx + cp * PQx, B10, B01, B00, ABx, CDx, Ix);
+                rys_axis<Real, L1, L2, L4, 0>(Real(1), PAy - cq * PQy, QCy + cp * PQy, B10, B01, B00, ABy, CDy, Iy);
+                rys_axis<Real, L1, L2, L4, 0>(Real(1), PAz - cq * PQz, QCz + cp * PQz, B10, B01, B00, ABz, CDz, Iz);
+            } else {
+                rys_axis<Real, L1, L2, L3, L4>(pref * w[i], PAx - cq * PQx, QCx + cp * PQx, B10, B01, B00, ABx, CDx, Ix);
+                rys_axis<Real, L1, L2, L3, L4>(Real(1), PAy - cq * PQy, QCy + cp * PQy, B10, B01, B00, ABy, CDy, Iy);
+                rys_axis<Real, L1, L2, L3, L4>(Real(1), PAz - cq * PQz, QCz + cp * PQz, B10, B01, B00, ABz, CDz, Iz);
+            }
+            // (a b | c d) of one axis from either layout (indices are compile-time constants)
+            auto at = [](const AX& I, int a_, int b_, int c_, int d_) __attribute__((always_inline)) {
+                if constexpr (KETD)
+                    return I.v[a_][b_][d_][0];
+                else
+                    return I.v[a_][b_][c_][d_];
+            };
             static_for<N1>([&](auto A_) {
                 constexpr int a = decltype(A_)::value;
                 constexpr int ax = cart_lx(L1, a), ay = cart_ly(L1, a), az = cart_lz(L1, a);
@@ -250,7 +267,7 @@ __device__ __forceinline__ void k_quartet(const RysTable& tab, const void* s_tab
                         static_for<N4>([&](auto D_) {
                             constexpr int d = decltype(D_)::value;
                             constexpr int dx = cart_lx(L4, d), dy = cart_ly(L4, d), dz = cart_lz(L4, d);
-                            s = fma(Ix.v[ax][bx][cx][dx] * Iy.v[ay][by][cy][dy] * Iz.v[az][bz][cz][dz], Db[b][d], s);
+                            s = fma(at(Ix, ax, bx, cx, dx) * at(Iy, ay, by, cy, dy) * at(Iz, az, bz, cz, dz), Db[b][d], s);
                         });
                     });
                     if constexpr (FP32)
@@ -717,10 +734,22 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
         const int blk4 = a.s34.eblk[e4];
         KEnt<double> E34;
         KEnt<float> E34f;
+        // s-type C with a higher D: the quartets recur at D (k_quartet KETD), from Q - D = (Q - C) + (C - D)
+        constexpr bool KETD = L3 == 0 && L4 > 0;
         if constexpr (LO) {
             E34f = load_ent32<true>(a.s34.eg32, e4);
+            if constexpr (KETD) {
+                E34f.PAx += E34f.ABx;
+                E34f.PAy += E34f.ABy;
+                E34f.PAz += E34f.ABz;
+            }
         } else {
             E34 = load_ent<true>(a.s34.eg, e4);
+            if constexpr (KETD) {
+                E34.PAx += E34.ABx;
+                E34.PAy += E34.ABy;
+                E34.PAz += E34.ABz;
+            }
             if constexpr (SPLIT) E34f = ent_to_f32(E34);
         }
         const int cart4 = LO ? E34f.cart : E34.cart;
@@ -768,7 +797,7 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
                     for (int b = 0; b < N2; b++)
 #pragma unroll
                         for (int d = 0; d < N4; d++) Db[b][d] = dcol[(size_t)(E12.cart + b) * a.ldD + d];
-                    k_quartet<double, L1, L2, L3, L4, Lds::STAGED64>(a.rys, s_tab64, a.ew, E12, E34, Db, acc);
+                    k_quartet<double, L1, L2, L3, L4, Lds::STAGED64, KETD>(a.rys, s_tab64, a.ew, E12, E34, Db, acc);
                     nq++;
                     hit = true;
                 }
@@ -785,7 +814,7 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
                     for (int b = 0; b < N2; b++)
 #pragma unroll
                         for (int d = 0; d < N4; d++) Db[b][d] = dcol[(size_t)(E12.cart + b) * a.ldD + d];
-                    k_quartet<double, L1, L2, L3, L4, Lds::STAGED64>(a.rys, s_tab64, a.ew, E12, E34, Db, acc);
+                    k_quartet<double, L1, L2, L3, L4, Lds::STAGED64, KETD>(a.rys, s_tab64, a.ew, E12, E34, Db, acc);
                 }
             } else {
                 const KEnt<float> E12f = load_ent32_uniform(a.s12.eg32, k);
@@ -795,7 +824,7 @@ __global__ __launch_bounds__(64 * kKWaves) __attribute__((amdgpu_waves_per_eu(kk
                     for (int b = 0; b < N2; b++)
 #pragma unroll
                         for (int d = 0; d < N4; d++) Dbf[b][d] = (float)dcol[(size_t)(E12f.cart + b) * a.ldD + d];
-                    k_quartet<float, L1, L2, L3, L4, Lds::STAGED32>(a.rys, s_tab32, a.ew, E12f, E34f, Dbf, acc);
+                    k_quartet<float, L1, L2, L3, L4, Lds::STAGED32, KETD>(a.rys, s_tab32, a.ew, E12f, E34f, Dbf, acc);
                 }
             }
             if (pass) {
